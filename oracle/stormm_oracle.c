@@ -1,0 +1,1140 @@
+/*
+ * stormm_oracle.c -- CPU double-precision restatement of the reference's energy/force path and
+ * of the GPU minimizer's semantics.
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the checker.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it; the stormm_amd product never links or calls it.
+ *
+ * The loops keep the reference's tile structure (16-atom tiles inside 256-atom supertiles) so
+ * that the fixed-precision energy accumulators, which round one partial sum per tile row, match
+ * the reference's ScoreCard contents and not only its double-precision sums.  Paths are relative
+ * to the reference repository root.
+ */
+#include "stormm_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#define TILE_LEN 16      /* src/Potential/static_exclusionmask.h:26-41 (tile_length)       */
+#define STILE_LEN 256    /* src/Potential/static_exclusionmask.h:26-41 (supertile_length)  */
+
+/* Generalized Born Taylor-series coefficients, src/Constants/generalized_born.h:27-36. */
+static const double GTA = 0.33333333333333333333, GTB = 0.4, GTC = 0.42857142857142857143,
+                    GTD = 0.44444444444444444444, GTDD = 0.45454545454545454545;
+static const double GTE = 1.33333333333333333333, GTF = 2.4, GTG = 3.42857142857142857143,
+                    GTH = 4.44444444444444444444, GTHH = 5.45454545454545454545;
+/* Kappa scaling, src/Constants/generalized_born.h:120. */
+static const double GB_KSCALE = 0.73;
+/* Dihedral guards, src/Constants/symbol_values.h:72-82.  The double-precision near-to-one
+ * threshold is declared as a float in the reference and therefore equals 1.0f exactly. */
+static const double ASYMPTOTIC_TO_ONE_LF = 0.99999904632568359375;
+static const double INV_ONE_MINUS_ASYMPTOTE_LF = 1048576.0;
+static const double PI_LF = 3.14159265358979323846;
+
+static inline int imin(int a, int b) { return a < b ? a : b; }
+
+/* ----------------------------------------------------------------------------------------- */
+/* Coulomb + Lennard-Jones, src/Potential/nonbonded_potential.tpp:9-425 (the branches without  */
+/* clash damping: lines 174-231 and 312-373).                                                  */
+/* ----------------------------------------------------------------------------------------- */
+static void nonbonded_impl(const orc_system *s, const double *x, const double *y,
+                           const double *z, double *fx, double *fy, double *fz, int frc_elec,
+                           int frc_vdw, double nrg_scale, orc_energy *e) {
+  const int eval_force = (frc_elec || frc_vdw);
+  const int n = s->natom;
+  const int nst = (n + STILE_LEN - 1) / STILE_LEN;
+  const int nt = s->n_lj_types;
+  double ele_energy = 0.0, vdw_energy = 0.0;
+  long long ele_acc = 0, vdw_acc = 0;
+  double cix[TILE_LEN], ciy[TILE_LEN], ciz[TILE_LEN], ciq[TILE_LEN];
+  double cjx[TILE_LEN], cjy[TILE_LEN], cjz[TILE_LEN], cjq[TILE_LEN];
+  double cifx[TILE_LEN], cify[TILE_LEN], cifz[TILE_LEN];
+  double cjfx[TILE_LEN], cjfy[TILE_LEN], cjfz[TILE_LEN];
+  int cilj[TILE_LEN], cjlj[TILE_LEN];
+  for (int sti = 0; sti < nst; sti++) {
+    for (int stj = 0; stj <= sti; stj++) {
+      const int stni = imin(n - (STILE_LEN * sti), STILE_LEN);
+      const int stnj = imin(n - (STILE_LEN * stj), STILE_LEN);
+      const int ni_tiles = (stni + TILE_LEN - 1) / TILE_LEN;
+      const int nj_tiles = (stnj + TILE_LEN - 1) / TILE_LEN;
+      const int diag_st = (sti == stj);
+      for (int ti = 0; ti < ni_tiles; ti++) {
+        const int ni = imin(stni - (ti * TILE_LEN), TILE_LEN);
+        const int tjlim = (nj_tiles * (1 - diag_st)) + (diag_st * ti);
+        for (int tj = 0; tj <= tjlim; tj++) {
+          const int nj = imin(stnj - (tj * TILE_LEN), TILE_LEN);
+          const int diag_tile = diag_st * (ti == tj);
+          for (int i = 0; i < ni; i++) {
+            const int ai = i + (ti * TILE_LEN) + (sti * STILE_LEN);
+            cix[i] = x[ai]; ciy[i] = y[ai]; ciz[i] = z[ai];
+            cifx[i] = 0.0; cify[i] = 0.0; cifz[i] = 0.0;
+            ciq[i] = s->coulomb * s->charge[ai];
+            cilj[i] = s->lj_idx[ai];
+          }
+          for (int j = 0; j < nj; j++) {
+            const int aj = j + (tj * TILE_LEN) + (stj * STILE_LEN);
+            cjx[j] = x[aj]; cjy[j] = y[aj]; cjz[j] = z[aj];
+            cjfx[j] = 0.0; cjfy[j] = 0.0; cjfz[j] = 0.0;
+            cjq[j] = s->charge[aj];
+            cjlj[j] = s->lj_idx[aj];
+          }
+          for (int i = 0; i < ni; i++) {
+            const int ai = i + (ti * TILE_LEN) + (sti * STILE_LEN);
+            const double qi = ciq[i];
+            const int ljt_i = cilj[i];
+            const int jlim = (nj * (1 - diag_tile)) + (diag_tile * i);
+            const double xi = cix[i], yi = ciy[i], zi = ciz[i];
+            double ele_c = 0.0, vdw_c = 0.0;
+            for (int j = 0; j < jlim; j++) {
+              const int aj = j + (tj * TILE_LEN) + (stj * STILE_LEN);
+              if (s->excl[((size_t)ai * n) + aj]) {
+                continue;
+              }
+              const double dx = cjx[j] - xi;
+              const double dy = cjy[j] - yi;
+              const double dz = cjz[j] - zi;
+              const double invr2 = 1.0 / ((dx * dx) + (dy * dy) + (dz * dz));
+              const double invr = sqrt(invr2);
+              const double invr4 = invr2 * invr2;
+              const double qiqj = qi * cjq[j];
+              ele_c += qiqj * invr;
+              const int ljt_j = cjlj[j];
+              const double lja = s->lja[(ljt_j * nt) + ljt_i];
+              const double ljb = s->ljb[(ljt_j * nt) + ljt_i];
+              vdw_c += ((lja * invr4 * invr4) - (ljb * invr2)) * invr4;
+              if (eval_force) {
+                double fmag = frc_elec ? -(qiqj * invr * invr2) : 0.0;
+                if (frc_vdw) {
+                  fmag += ((6.0 * ljb) - (12.0 * lja * invr4 * invr2)) * invr4 * invr4;
+                }
+                const double fdx = fmag * dx, fdy = fmag * dy, fdz = fmag * dz;
+                cifx[i] += fdx; cify[i] += fdy; cifz[i] += fdz;
+                cjfx[j] -= fdx; cjfy[j] -= fdy; cjfz[j] -= fdz;
+              }
+            }
+            ele_energy += ele_c;
+            ele_acc += llround(ele_c * nrg_scale);
+            vdw_energy += vdw_c;
+            vdw_acc += llround(vdw_c * nrg_scale);
+          }
+          if (eval_force) {
+            for (int i = 0; i < ni; i++) {
+              const int ai = i + (ti * TILE_LEN) + (sti * STILE_LEN);
+              fx[ai] += cifx[i]; fy[ai] += cify[i]; fz[ai] += cifz[i];
+            }
+            for (int j = 0; j < nj; j++) {
+              const int aj = j + (tj * TILE_LEN) + (stj * STILE_LEN);
+              fx[aj] += cjfx[j]; fy[aj] += cjfy[j]; fz[aj] += cjfz[j];
+            }
+          }
+        }
+      }
+    }
+  }
+  e->nrg[ORC_ELECTROSTATIC] += ele_energy;
+  e->nrg[ORC_VDW] += vdw_energy;
+  e->acc[ORC_ELECTROSTATIC] += ele_acc;
+  e->acc[ORC_VDW] += vdw_acc;
+}
+
+void orc_nonbonded(const orc_system *s, const double *x, const double *y, const double *z,
+                   double *fx, double *fy, double *fz, int eval_force, double nrg_scale,
+                   orc_energy *e) {
+  nonbonded_impl(s, x, y, z, fx, fy, fz, eval_force, eval_force, nrg_scale, e);
+}
+
+/* ----------------------------------------------------------------------------------------- */
+/* HCT / OBC Born-radius integral contribution of atom j (screened radius sj) onto atom i      */
+/* (intrinsic offset radius ri), src/Potential/nonbonded_potential.tpp:578-617.                */
+/* ----------------------------------------------------------------------------------------- */
+static inline double psi_term(double r, double r2, double invr, double ri, double inv_ri,
+                              double sj) {
+  const double sj2 = sj * sj;
+  if (r > 4.0 * sj) {
+    const double invr2 = invr * invr;
+    const double tmpsd = sj2 * invr2;
+    const double dumbo = GTA + tmpsd * (GTB + tmpsd * (GTC + tmpsd * (GTD + tmpsd * GTDD)));
+    return -(sj * tmpsd * invr2 * dumbo);
+  }
+  else if (r > ri + sj) {
+    return -(0.5 * ((sj / (r2 - sj2)) + (0.5 * invr * log((r - sj) / (r + sj)))));
+  }
+  else if (r > fabs(ri - sj)) {
+    const double theta = 0.5 * inv_ri * invr * (r2 + (ri * ri) - sj2);
+    const double uij = 1.0 / (r + sj);
+    return -(0.25 * ((inv_ri * (2.0 - theta)) - uij + (invr * log(ri * uij))));
+  }
+  else if (ri < sj) {
+    return -(0.5 * ((sj / (r2 - sj2)) + (2.0 * inv_ri) + (0.5 * invr * log((sj - r) / (sj + r)))));
+  }
+  return 0.0;
+}
+
+/* Radial derivative factor of the same integral, src/Potential/nonbonded_potential.tpp:
+ * 1138-1188. */
+static inline double dpsi_term(double r, double r2, double invr, double invr2, double ri,
+                               double inv_ri, double sj) {
+  const double sj2 = sj * sj;
+  if (r > 4.0 * sj) {
+    const double tmpsd = sj2 * invr2;
+    const double dumbo = GTE + tmpsd * (GTF + tmpsd * (GTG + tmpsd * (GTH + tmpsd * GTHH)));
+    return tmpsd * sj * invr2 * invr2 * dumbo;
+  }
+  else if (r > ri + sj) {
+    const double temp1 = 1.0 / (r2 - sj2);
+    return (temp1 * sj * (-0.5 * invr2 + temp1)) + (0.25 * invr * invr2 * log((r - sj) / (r + sj)));
+  }
+  else if (r > fabs(ri - sj)) {
+    const double temp1 = 1.0 / (r + sj);
+    const double invr3 = invr2 * invr;
+    return -0.25 * ((-0.5 * (r2 - ri * ri + sj2) * invr3 * inv_ri * inv_ri) +
+                    (invr * temp1 * (temp1 - invr)) - (invr3 * log(ri * temp1)));
+  }
+  else if (ri < sj) {
+    const double temp1 = 1.0 / (r2 - sj2);
+    return -0.5 * ((sj * invr2 * temp1) - (2.0 * sj * temp1 * temp1) -
+                   (0.5 * invr2 * invr * log((sj - r) / (sj + r))));
+  }
+  return 0.0;
+}
+
+/* ----------------------------------------------------------------------------------------- */
+/* Generalized Born, src/Potential/nonbonded_potential.tpp:449-1292 (neck variants omitted).   */
+/* ----------------------------------------------------------------------------------------- */
+void orc_gb(const orc_system *s, const double *x, const double *y, const double *z, double *fx,
+            double *fy, double *fz, int eval_force, double nrg_scale, orc_energy *e, double *psi,
+            double *radii, double *sumdeijda) {
+  const int n = s->natom;
+  if (s->igb == ORC_IGB_NONE || n == 0) {
+    return;
+  }
+  double *own = NULL;
+  if (psi == NULL || radii == NULL || sumdeijda == NULL) {
+    own = (double *)calloc((size_t)3 * n, sizeof(double));
+    psi = own;
+    radii = own + n;
+    sumdeijda = own + 2 * n;
+  }
+  const int nst = (n + STILE_LEN - 1) / STILE_LEN;
+  double egb_energy = 0.0;
+  long long egb_acc = 0;
+  for (int i = 0; i < n; i++) {
+    psi[i] = 0.0;
+  }
+
+  /* Pass 1: Born-radius integrals (lines 504-728). */
+  for (int sti = 0; sti < nst; sti++) {
+    for (int stj = 0; stj <= sti; stj++) {
+      const int stni = imin(n - (STILE_LEN * sti), STILE_LEN);
+      const int stnj = imin(n - (STILE_LEN * stj), STILE_LEN);
+      const int ni_tiles = (stni + TILE_LEN - 1) / TILE_LEN;
+      const int nj_tiles = (stnj + TILE_LEN - 1) / TILE_LEN;
+      const int diag_st = (sti == stj);
+      for (int ti = 0; ti < ni_tiles; ti++) {
+        const int ni = imin(stni - (ti * TILE_LEN), TILE_LEN);
+        const int tjlim = (nj_tiles * (1 - diag_st)) + (diag_st * ti);
+        for (int tj = 0; tj <= tjlim; tj++) {
+          const int nj = imin(stnj - (tj * TILE_LEN), TILE_LEN);
+          const int diag_tile = diag_st * (ti == tj);
+          double cpsi_i[TILE_LEN], cpsi_j[TILE_LEN];
+          for (int i = 0; i < TILE_LEN; i++) {
+            cpsi_i[i] = 0.0;
+            cpsi_j[i] = 0.0;
+          }
+          for (int i = 0; i < ni; i++) {
+            const int ai = i + (ti * TILE_LEN) + (sti * STILE_LEN);
+            const int jlim = (nj * (1 - diag_tile)) + (diag_tile * i);
+            const double ri = s->pb_radii[ai] - s->gb_offset;
+            const double inv_ri = 1.0 / ri;
+            for (int j = 0; j < jlim; j++) {
+              const int aj = j + (tj * TILE_LEN) + (stj * STILE_LEN);
+              const double dx = x[ai] - x[aj];
+              const double dy = y[ai] - y[aj];
+              const double dz = z[ai] - z[aj];
+              const double r2 = (dx * dx) + (dy * dy) + (dz * dz);
+              const double r = sqrt(r2);
+              const double invr = 1.0 / r;
+              const double rj = s->pb_radii[aj] - s->gb_offset;
+              const double inv_rj = 1.0 / rj;
+              const double sj = s->gb_screen[aj] * rj;
+              const double si = s->gb_screen[ai] * ri;
+              cpsi_i[i] += psi_term(r, r2, invr, ri, inv_ri, sj);
+              cpsi_j[j] += psi_term(r, r2, invr, rj, inv_rj, si);
+            }
+          }
+          for (int i = 0; i < ni; i++) {
+            psi[i + (ti * TILE_LEN) + (sti * STILE_LEN)] += cpsi_i[i];
+          }
+          for (int j = 0; j < nj; j++) {
+            psi[j + (tj * TILE_LEN) + (stj * STILE_LEN)] += cpsi_j[j];
+          }
+        }
+      }
+    }
+  }
+
+  /* Effective radii (lines 731-781). */
+  for (int i = 0; i < n; i++) {
+    if (s->igb == ORC_IGB_HCT) {
+      const double inv_ri = 1.0 / (s->pb_radii[i] - s->gb_offset);
+      double egbi = 1.0 / (inv_ri + psi[i]);
+      if (egbi < 0.0) {
+        egbi = 30.0;
+      }
+      radii[i] = egbi;
+    }
+    else {
+      const double ri = s->pb_radii[i] - s->gb_offset;
+      const double inv_ri = 1.0 / ri;
+      const double fipsi = psi[i] * (-ri);
+      radii[i] = 1.0 / (inv_ri - tanh((s->gb_alpha[i] - (s->gb_beta[i] * fipsi) +
+                                       (s->gb_gamma[i] * fipsi * fipsi)) * fipsi) /
+                                  s->pb_radii[i]);
+    }
+  }
+
+  /* Self energies (lines 784-806). */
+  for (int i = 0; i < n; i++) {
+    const double qi = s->charge[i];
+    const double ri = radii[i];
+    const double expmkf = exp(-GB_KSCALE * s->kappa * ri) / s->dielectric;
+    const double dielfac = 1.0 - expmkf;
+    const double atmq2h = 0.5 * qi * qi * s->coulomb;
+    const double atmqd2h = atmq2h * dielfac;
+    const double contrib = -atmqd2h / ri;
+    egb_energy += contrib;
+    egb_acc += (long long)(contrib * nrg_scale);
+    if (eval_force) {
+      sumdeijda[i] = atmqd2h - (GB_KSCALE * s->kappa * atmq2h * expmkf * ri);
+    }
+  }
+
+  /* Pass 2: pair energies and radius derivatives (lines 811-972). */
+  for (int sti = 0; sti < nst; sti++) {
+    for (int stj = 0; stj <= sti; stj++) {
+      const int stni = imin(n - (STILE_LEN * sti), STILE_LEN);
+      const int stnj = imin(n - (STILE_LEN * stj), STILE_LEN);
+      const int ni_tiles = (stni + TILE_LEN - 1) / TILE_LEN;
+      const int nj_tiles = (stnj + TILE_LEN - 1) / TILE_LEN;
+      const int diag_st = (sti == stj);
+      for (int ti = 0; ti < ni_tiles; ti++) {
+        const int ni = imin(stni - (ti * TILE_LEN), TILE_LEN);
+        const int tjlim = (nj_tiles * (1 - diag_st)) + (diag_st * ti);
+        for (int tj = 0; tj <= tjlim; tj++) {
+          const int nj = imin(stnj - (tj * TILE_LEN), TILE_LEN);
+          const int diag_tile = diag_st * (ti == tj);
+          double cifx[TILE_LEN], cify[TILE_LEN], cifz[TILE_LEN], cisd[TILE_LEN];
+          double cjfx[TILE_LEN], cjfy[TILE_LEN], cjfz[TILE_LEN], cjsd[TILE_LEN];
+          for (int i = 0; i < TILE_LEN; i++) {
+            cifx[i] = cify[i] = cifz[i] = cisd[i] = 0.0;
+            cjfx[i] = cjfy[i] = cjfz[i] = cjsd[i] = 0.0;
+          }
+          for (int i = 0; i < ni; i++) {
+            const int ai = i + (ti * TILE_LEN) + (sti * STILE_LEN);
+            const double xi = x[ai], yi = y[ai], zi = z[ai];
+            const double qi = s->coulomb * s->charge[ai];
+            const double Ri = radii[ai];
+            const int jlim = (nj * (1 - diag_tile)) + (diag_tile * i);
+            double contrib = 0.0;
+            for (int j = 0; j < jlim; j++) {
+              const int aj = j + (tj * TILE_LEN) + (stj * STILE_LEN);
+              const double dx = x[aj] - xi;
+              const double dy = y[aj] - yi;
+              const double dz = z[aj] - zi;
+              const double r2 = (dx * dx) + (dy * dy) + (dz * dz);
+              const double qiqj = qi * s->charge[aj];
+              const double Rj = radii[aj];
+              const double ij_born = Ri * Rj;
+              const double efac = exp(-r2 / (4.0 * ij_born));
+              const double fgbi = 1.0 / sqrt(r2 + (ij_born * efac));
+              const double fgbk = -s->kappa * GB_KSCALE / fgbi;
+              const double expmkf = exp(fgbk) / s->dielectric;
+              const double dielfac = 1.0 - expmkf;
+              contrib -= qiqj * dielfac * fgbi;
+              if (eval_force) {
+                const double temp4 = fgbi * fgbi * fgbi;
+                const double temp6 = qiqj * temp4 * (dielfac + (fgbk * expmkf));
+                const double fmag = temp6 * (1.0 - (0.25 * efac));
+                const double temp5 = 0.5 * efac * temp6 * (ij_born + (0.25 * r2));
+                cisd[i] += Ri * temp5;
+                cjsd[j] += Rj * temp5;
+                cifx[i] += fmag * dx; cify[i] += fmag * dy; cifz[i] += fmag * dz;
+                cjfx[j] -= fmag * dx; cjfy[j] -= fmag * dy; cjfz[j] -= fmag * dz;
+              }
+            }
+            egb_acc += llround(contrib * nrg_scale);
+            egb_energy += contrib;
+          }
+          if (eval_force) {
+            for (int i = 0; i < ni; i++) {
+              const int ai = i + (ti * TILE_LEN) + (sti * STILE_LEN);
+              fx[ai] += cifx[i]; fy[ai] += cify[i]; fz[ai] += cifz[i];
+              sumdeijda[ai] += cisd[i];
+            }
+            for (int j = 0; j < nj; j++) {
+              const int aj = j + (tj * TILE_LEN) + (stj * STILE_LEN);
+              fx[aj] += cjfx[j]; fy[aj] += cjfy[j]; fz[aj] += cjfz[j];
+              sumdeijda[aj] += cjsd[j];
+            }
+          }
+        }
+      }
+    }
+  }
+  e->nrg[ORC_GENERALIZED_BORN] += egb_energy;
+  e->acc[ORC_GENERALIZED_BORN] += egb_acc;
+  if (!eval_force) {
+    free(own);
+    return;
+  }
+
+  /* OBC chain rule on the radius derivatives (lines 985-1017). */
+  if (s->igb != ORC_IGB_HCT) {
+    for (int i = 0; i < n; i++) {
+      const double ri = s->pb_radii[i] - s->gb_offset;
+      const double fipsi = psi[i] * (-ri);
+      const double thi = tanh((s->gb_alpha[i] - (s->gb_beta[i] - (s->gb_gamma[i] * fipsi)) *
+                               fipsi) * fipsi);
+      const double mult = (s->gb_alpha[i] - ((2.0 * s->gb_beta[i]) -
+                                             (3.0 * s->gb_gamma[i] * fipsi)) * fipsi) *
+                          (1.0 - thi * thi) * ri / s->pb_radii[i];
+      sumdeijda[i] = sumdeijda[i] * mult;
+    }
+  }
+
+  /* Pass 3: forces from the radius derivatives (lines 1018-1292). */
+  for (int sti = 0; sti < nst; sti++) {
+    for (int stj = 0; stj <= sti; stj++) {
+      const int stni = imin(n - (STILE_LEN * sti), STILE_LEN);
+      const int stnj = imin(n - (STILE_LEN * stj), STILE_LEN);
+      const int ni_tiles = (stni + TILE_LEN - 1) / TILE_LEN;
+      const int nj_tiles = (stnj + TILE_LEN - 1) / TILE_LEN;
+      const int diag_st = (sti == stj);
+      for (int ti = 0; ti < ni_tiles; ti++) {
+        const int ni = imin(stni - (ti * TILE_LEN), TILE_LEN);
+        const int tjlim = (nj_tiles * (1 - diag_st)) + (diag_st * ti);
+        for (int tj = 0; tj <= tjlim; tj++) {
+          const int nj = imin(stnj - (tj * TILE_LEN), TILE_LEN);
+          const int diag_tile = diag_st * (ti == tj);
+          double cifx[TILE_LEN], cify[TILE_LEN], cifz[TILE_LEN];
+          double cjfx[TILE_LEN], cjfy[TILE_LEN], cjfz[TILE_LEN];
+          for (int i = 0; i < TILE_LEN; i++) {
+            cifx[i] = cify[i] = cifz[i] = 0.0;
+            cjfx[i] = cjfy[i] = cjfz[i] = 0.0;
+          }
+          for (int i = 0; i < ni; i++) {
+            const int ai = i + (ti * TILE_LEN) + (sti * STILE_LEN);
+            const double xi = x[ai], yi = y[ai], zi = z[ai];
+            const double ri = s->pb_radii[ai] - s->gb_offset;
+            const double inv_ri = 1.0 / ri;
+            const double si = s->gb_screen[ai] * ri;
+            const int jlim = (nj * (1 - diag_tile)) + (diag_tile * i);
+            for (int j = 0; j < jlim; j++) {
+              const int aj = j + (tj * TILE_LEN) + (stj * STILE_LEN);
+              const double dx = x[aj] - xi;
+              const double dy = y[aj] - yi;
+              const double dz = z[aj] - zi;
+              const double rj = s->pb_radii[aj] - s->gb_offset;
+              const double inv_rj = 1.0 / rj;
+              const double r2 = (dx * dx) + (dy * dy) + (dz * dz);
+              const double invr = 1.0 / sqrt(r2);
+              const double invr2 = invr * invr;
+              const double r = r2 * invr;
+              const double sj = s->gb_screen[aj] * rj;
+              const double datmpi = dpsi_term(r, r2, invr, invr2, ri, inv_ri, sj);
+              const double datmpj = dpsi_term(r, r2, invr, invr2, rj, inv_rj, si);
+              const double fmag = (datmpi * sumdeijda[ai]) + (datmpj * sumdeijda[aj]);
+              cifx[i] -= fmag * dx; cify[i] -= fmag * dy; cifz[i] -= fmag * dz;
+              cjfx[j] += fmag * dx; cjfy[j] += fmag * dy; cjfz[j] += fmag * dz;
+            }
+          }
+          for (int i = 0; i < ni; i++) {
+            const int ai = i + (ti * TILE_LEN) + (sti * STILE_LEN);
+            fx[ai] += cifx[i]; fy[ai] += cify[i]; fz[ai] += cifz[i];
+          }
+          for (int j = 0; j < nj; j++) {
+            const int aj = j + (tj * TILE_LEN) + (stj * STILE_LEN);
+            fx[aj] += cjfx[j]; fy[aj] += cjfy[j]; fz[aj] += cjfz[j];
+          }
+        }
+      }
+    }
+  }
+  free(own);
+}
+
+/* ----------------------------------------------------------------------------------------- */
+/* Valence terms                                                                               */
+/* ----------------------------------------------------------------------------------------- */
+
+/* src/Potential/valence_potential.tpp:9-96 */
+static double harmonic_stretch(int i, int j, double k, double r0, const double *x,
+                               const double *y, const double *z, double *fx, double *fy,
+                               double *fz, int eval_force) {
+  const double dx = x[j] - x[i];
+  const double dy = y[j] - y[i];
+  const double dz = z[j] - z[i];
+  const double dr = sqrt((dx * dx) + (dy * dy) + (dz * dz));
+  const double dl = dr - r0;
+  if (eval_force) {
+    const double fmag = 2.0 * k * dl / dr;
+    fx[i] += fmag * dx; fy[i] += fmag * dy; fz[i] += fmag * dz;
+    fx[j] -= fmag * dx; fy[j] -= fmag * dy; fz[j] -= fmag * dz;
+  }
+  return k * dl * dl;
+}
+
+/* src/Potential/valence_potential.tpp:170-287 */
+static double harmonic_bend(int i, int j, int k, double kb, double th0, const double *x,
+                            const double *y, const double *z, double *fx, double *fy, double *fz,
+                            int eval_force) {
+  double ba[3], bc[3];
+  ba[0] = x[i] - x[j]; ba[1] = y[i] - y[j]; ba[2] = z[i] - z[j];
+  bc[0] = x[k] - x[j]; bc[1] = y[k] - y[j]; bc[2] = z[k] - z[j];
+  const double mgba = (ba[0] * ba[0]) + (ba[1] * ba[1]) + (ba[2] * ba[2]);
+  const double mgbc = (bc[0] * bc[0]) + (bc[1] * bc[1]) + (bc[2] * bc[2]);
+  const double invbabc = 1.0 / sqrt(mgba * mgbc);
+  double costheta = ((ba[0] * bc[0]) + (ba[1] * bc[1]) + (ba[2] * bc[2])) * invbabc;
+  costheta = (costheta < -1.0) ? -1.0 : (costheta > 1.0) ? 1.0 : costheta;
+  const double theta = acos(costheta);
+  const double dtheta = theta - th0;
+  if (eval_force) {
+    const double dA = -2.0 * kb * dtheta / sqrt(1.0 - (costheta * costheta));
+    const double sqba = dA / mgba;
+    const double sqbc = dA / mgbc;
+    const double mbabc = dA * invbabc;
+    double adf[3], cdf[3];
+    for (int m = 0; m < 3; m++) {
+      adf[m] = (bc[m] * mbabc) - (costheta * ba[m] * sqba);
+      cdf[m] = (ba[m] * mbabc) - (costheta * bc[m] * sqbc);
+    }
+    fx[i] -= adf[0]; fy[i] -= adf[1]; fz[i] -= adf[2];
+    fx[j] += adf[0] + cdf[0]; fy[j] += adf[1] + cdf[1]; fz[j] += adf[2] + cdf[2];
+    fx[k] -= cdf[0]; fy[k] -= cdf[1]; fz[k] -= cdf[2];
+  }
+  return kb * dtheta * dtheta;
+}
+
+static inline void cross3(const double *a, const double *b, double *c) {
+  c[0] = (a[1] * b[2]) - (a[2] * b[1]);
+  c[1] = (a[2] * b[0]) - (a[0] * b[2]);
+  c[2] = (a[0] * b[1]) - (a[1] * b[0]);
+}
+
+/* src/Math/rounding.tpp:24-59 (double-precision branch) */
+static double angle_verification(double costheta, const double *crabbc, const double *crbccd,
+                                 const double *bc, const double *scr) {
+  const double scr_bc = (scr[0] * bc[0]) + (scr[1] * bc[1]) + (scr[2] * bc[2]);
+  if (fabs(costheta) >= (double)1.0f) {
+    const double mg_crabbc = 1.0 / sqrt(crabbc[0] * crabbc[0] + crabbc[1] * crabbc[1] +
+                                        crabbc[2] * crabbc[2]);
+    const double mg_crbccd = 1.0 / sqrt(crbccd[0] * crbccd[0] + crbccd[1] * crbccd[1] +
+                                        crbccd[2] * crbccd[2]);
+    double rdx = crbccd[0] * mg_crbccd - crabbc[0] * mg_crabbc;
+    double rdy = crbccd[1] * mg_crbccd - crabbc[1] * mg_crabbc;
+    double rdz = crbccd[2] * mg_crbccd - crabbc[2] * mg_crabbc;
+    double rs = sqrt((rdx * rdx) + (rdy * rdy) + (rdz * rdz));
+    if (fabs(rs) > 1.0) {
+      rdx = crbccd[0] * mg_crbccd + crabbc[0] * mg_crabbc;
+      rdy = crbccd[1] * mg_crbccd + crabbc[1] * mg_crabbc;
+      rdz = crbccd[2] * mg_crbccd + crabbc[2] * mg_crabbc;
+      rs = PI_LF - sqrt((rdx * rdx) + (rdy * rdy) + (rdz * rdz));
+    }
+    return (scr_bc > 0.0) ? rs : -rs;
+  }
+  return (scr_bc > 0.0) ? acos(costheta) : -acos(costheta);
+}
+
+/* Cosine dihedral, src/Potential/valence_potential.tpp:359-578 */
+static double dihedral_twist(int i, int j, int k, int l, double amp, double phase, double freq,
+                             const double *x, const double *y, const double *z, double *fx,
+                             double *fy, double *fz, int eval_force) {
+  double ab[3], bc[3], cd[3], crabbc[3], crbccd[3], scr[3];
+  ab[0] = x[j] - x[i]; ab[1] = y[j] - y[i]; ab[2] = z[j] - z[i];
+  bc[0] = x[k] - x[j]; bc[1] = y[k] - y[j]; bc[2] = z[k] - z[j];
+  cd[0] = x[l] - x[k]; cd[1] = y[l] - y[k]; cd[2] = z[l] - z[k];
+  cross3(ab, bc, crabbc);
+  cross3(bc, cd, crbccd);
+  double costheta = (crabbc[0] * crbccd[0]) + (crabbc[1] * crbccd[1]) + (crabbc[2] * crbccd[2]);
+  costheta /= sqrt((crabbc[0] * crabbc[0] + crabbc[1] * crabbc[1] + crabbc[2] * crabbc[2]) *
+                   (crbccd[0] * crbccd[0] + crbccd[1] * crbccd[1] + crbccd[2] * crbccd[2]));
+  cross3(crabbc, crbccd, scr);
+  costheta = (costheta < -1.0) ? -1.0 : (costheta > 1.0) ? 1.0 : costheta;
+  const double theta = angle_verification(costheta, crabbc, crbccd, bc, scr);
+  const double sangle = (freq * theta) - phase;
+  if (eval_force) {
+    const double fr = amp * freq * sin(sangle);
+    const double mgab = sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]);
+    const double mgbc = sqrt(bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]);
+    const double mgcd = sqrt(cd[0] * cd[0] + cd[1] * cd[1] + cd[2] * cd[2]);
+    const double invab = 1.0 / mgab;
+    const double invbc = 1.0 / mgbc;
+    const double invcd = 1.0 / mgcd;
+    const double cosb = -(ab[0] * bc[0] + ab[1] * bc[1] + ab[2] * bc[2]) * invab * invbc;
+    const double cosc = -(bc[0] * cd[0] + bc[1] * cd[1] + bc[2] * cd[2]) * invbc * invcd;
+    const double isinb2 = (cosb * cosb < ASYMPTOTIC_TO_ONE_LF) ?
+                          fr / (1.0 - (cosb * cosb)) : fr * INV_ONE_MINUS_ASYMPTOTE_LF;
+    const double isinc2 = (cosc * cosc < ASYMPTOTIC_TO_ONE_LF) ?
+                          fr / (1.0 - (cosc * cosc)) : fr * INV_ONE_MINUS_ASYMPTOTE_LF;
+    const double invabc = invab * invbc;
+    const double invbcd = invbc * invcd;
+    for (int m = 0; m < 3; m++) {
+      crabbc[m] *= invabc;
+      crbccd[m] *= invbcd;
+    }
+    const double fa = -invab * isinb2;
+    const double fb1 = (mgbc - (mgab * cosb)) * invabc * isinb2;
+    const double fb2 = cosc * invbc * isinc2;
+    const double fc1 = (mgbc - (mgcd * cosc)) * invbcd * isinc2;
+    const double fc2 = cosb * invbc * isinb2;
+    const double fd = -invcd * isinc2;
+    (void)fc1;
+    (void)fc2;
+    double *f[3] = {fx, fy, fz};
+    for (int m = 0; m < 3; m++) {
+      const double frc_i = crabbc[m] * fa;
+      const double frc_j = (fb1 * crabbc[m]) - (fb2 * crbccd[m]);
+      const double frc_l = -fd * crbccd[m];
+      f[m][i] += frc_i;
+      f[m][j] += frc_j;
+      f[m][k] -= frc_i + frc_j + frc_l;
+      f[m][l] += frc_l;
+    }
+  }
+  return amp * (1.0 + cos(sangle));
+}
+
+/* src/Potential/valence_potential.tpp:1288-1419 (standard branch, no clash damping) */
+static void attenuated_14(const orc_system *s, int i, int l, double scee, double scnb,
+                          const double *x, const double *y, const double *z, double *fx,
+                          double *fy, double *fz, int frc_elec, int frc_vdw, double *ele,
+                          double *vdw) {
+  const double dx = x[l] - x[i];
+  const double dy = y[l] - y[i];
+  const double dz = z[l] - z[i];
+  const int ilj = s->lj_idx[i];
+  const int jlj = s->lj_idx[l];
+  const size_t ij = ((size_t)ilj * s->n_lj_types) + jlj;
+  const double lja = s->lja14[ij] / scnb;
+  const double ljb = s->ljb14[ij] / scnb;
+  const double qiqj = (s->coulomb * s->charge[i] * s->charge[l]) / scee;
+  const double invr2 = 1.0 / ((dx * dx) + (dy * dy) + (dz * dz));
+  const double invr = sqrt(invr2);
+  const double invr4 = invr2 * invr2;
+  *ele = qiqj * invr;
+  *vdw = (lja * invr4 * invr4 * invr4) - (ljb * invr4 * invr2);
+  if (frc_elec || frc_vdw) {
+    double fmag = frc_elec ? -(qiqj * invr * invr2) : 0.0;
+    if (frc_vdw) {
+      fmag += ((6.0 * ljb) - (12.0 * lja * invr4 * invr2)) * invr4 * invr4;
+    }
+    fx[i] += fmag * dx; fy[i] += fmag * dy; fz[i] += fmag * dz;
+    fx[l] -= fmag * dx; fy[l] -= fmag * dy; fz[l] -= fmag * dz;
+  }
+}
+
+static void valence_impl(const orc_system *s, const double *x, const double *y,
+                         const double *z, double *fx, double *fy, double *fz, unsigned fmask,
+                         double nrg_scale, orc_energy *e) {
+  int eval_force = (fmask & ORC_TERM_BOND) != 0;
+  /* Bonds, src/Potential/valence_potential.tpp:100-140 (bond_leq is taken in absolute value). */
+  double sum = 0.0;
+  long long acc = 0;
+  for (int p = 0; p < s->nbond; p++) {
+    const double du = harmonic_stretch(s->bond_ij[2 * p], s->bond_ij[2 * p + 1], s->bond_keq[p],
+                                       fabs(s->bond_leq[p]), x, y, z, fx, fy, fz, eval_force);
+    sum += du;
+    acc += llround(du * nrg_scale);
+  }
+  e->nrg[ORC_BOND] += sum;
+  e->acc[ORC_BOND] += acc;
+  eval_force = (fmask & ORC_TERM_ANGLE) != 0;
+
+  /* Angles, lines 291-330. */
+  sum = 0.0;
+  acc = 0;
+  for (int p = 0; p < s->nangl; p++) {
+    const double du = harmonic_bend(s->angl_ijk[3 * p], s->angl_ijk[3 * p + 1],
+                                    s->angl_ijk[3 * p + 2], s->angl_keq[p], s->angl_theta[p], x,
+                                    y, z, fx, fy, fz, eval_force);
+    sum += du;
+    acc += llround(du * nrg_scale);
+  }
+  e->nrg[ORC_ANGLE] += sum;
+  e->acc[ORC_ANGLE] += acc;
+  eval_force = (fmask & ORC_TERM_DIHEDRAL) != 0;
+
+  /* Dihedrals, lines 582-627. */
+  double psum = 0.0, isum = 0.0;
+  long long pacc = 0, iacc = 0;
+  for (int p = 0; p < s->ndihe; p++) {
+    const int *q = &s->dihe_ijkl[4 * p];
+    const double du = dihedral_twist(q[0], q[1], q[2], q[3], s->dihe_amp[p], s->dihe_phase[p],
+                                     s->dihe_freq[p], x, y, z, fx, fy, fz, eval_force);
+    if (s->dihe_improper[p]) {
+      isum += du;
+      iacc += llround(du * nrg_scale);
+    }
+    else {
+      psum += du;
+      pacc += llround(du * nrg_scale);
+    }
+  }
+  e->nrg[ORC_PROPER_DIHEDRAL] += psum;
+  e->acc[ORC_PROPER_DIHEDRAL] += pacc;
+  e->nrg[ORC_IMPROPER_DIHEDRAL] += isum;
+  e->acc[ORC_IMPROPER_DIHEDRAL] += iacc;
+
+  /* Attenuated 1:4 pairs, lines 1447-1523. */
+  double esum = 0.0, vsum = 0.0;
+  long long eacc = 0, vacc = 0;
+  for (int p = 0; p < s->ndihe; p++) {
+    if (s->dihe_scee[p] <= 0.0) {
+      continue;
+    }
+    double ele, vdw;
+    attenuated_14(s, s->dihe_ijkl[4 * p], s->dihe_ijkl[4 * p + 3], s->dihe_scee[p],
+                  s->dihe_scnb[p], x, y, z, fx, fy, fz, (fmask & ORC_TERM_QQ14) != 0,
+                  (fmask & ORC_TERM_LJ14) != 0, &ele, &vdw);
+    esum += ele;
+    vsum += vdw;
+    eacc += llround(ele * nrg_scale);
+    vacc += llround(vdw * nrg_scale);
+  }
+  for (int p = 0; p < s->ninfr14; p++) {
+    if (s->infr14_scee[p] <= 0.0) {
+      continue;
+    }
+    double ele, vdw;
+    attenuated_14(s, s->infr14_il[2 * p], s->infr14_il[2 * p + 1], s->infr14_scee[p],
+                  s->infr14_scnb[p], x, y, z, fx, fy, fz, (fmask & ORC_TERM_QQ14) != 0,
+                  (fmask & ORC_TERM_LJ14) != 0, &ele, &vdw);
+    esum += ele;
+    vsum += vdw;
+    eacc += llround(ele * nrg_scale);
+    vacc += llround(vdw * nrg_scale);
+  }
+  e->nrg[ORC_ELEC_ONE_FOUR] += esum;
+  e->acc[ORC_ELEC_ONE_FOUR] += eacc;
+  e->nrg[ORC_VDW_ONE_FOUR] += vsum;
+  e->acc[ORC_VDW_ONE_FOUR] += vacc;
+}
+
+void orc_valence(const orc_system *s, const double *x, const double *y, const double *z,
+                 double *fx, double *fy, double *fz, int eval_force, double nrg_scale,
+                 orc_energy *e) {
+  valence_impl(s, x, y, z, fx, fy, fz, eval_force ? ORC_TERM_ALL : 0u, nrg_scale, e);
+}
+
+void orc_eval_terms(const orc_system *s, const double *x, const double *y, const double *z,
+                    double *fx, double *fy, double *fz, unsigned force_mask, int nrg_bits,
+                    orc_energy *e) {
+  const double nrg_scale = ldexp(1.0, nrg_bits);
+  memset(e, 0, sizeof(*e));
+  nonbonded_impl(s, x, y, z, fx, fy, fz, (force_mask & ORC_TERM_QQ) != 0,
+                 (force_mask & ORC_TERM_LJ) != 0, nrg_scale, e);
+  valence_impl(s, x, y, z, fx, fy, fz, force_mask, nrg_scale, e);
+  if ((force_mask & ORC_TERM_GB) || !(force_mask & ORC_TERM_ALL)) {
+    orc_gb(s, x, y, z, fx, fy, fz, (force_mask & ORC_TERM_GB) != 0, nrg_scale, e, NULL, NULL,
+           NULL);
+  }
+  else {
+    orc_gb(s, x, y, z, fx, fy, fz, 0, nrg_scale, e, NULL, NULL, NULL);
+  }
+}
+
+void orc_eval_mm(const orc_system *s, const double *x, const double *y, const double *z,
+                 double *fx, double *fy, double *fz, int eval_force, int nrg_bits,
+                 orc_energy *e) {
+  const double nrg_scale = ldexp(1.0, nrg_bits);
+  memset(e, 0, sizeof(*e));
+  orc_nonbonded(s, x, y, z, fx, fy, fz, eval_force, nrg_scale, e);
+  orc_valence(s, x, y, z, fx, fy, fz, eval_force, nrg_scale, e);
+  orc_gb(s, x, y, z, fx, fy, fz, eval_force, nrg_scale, e, NULL, NULL, NULL);
+}
+
+/* ----------------------------------------------------------------------------------------- */
+/* Threaded batch evaluation (the CPU baseline)                                                */
+/* ----------------------------------------------------------------------------------------- */
+typedef struct {
+  int first, last, eval_force, nrg_bits, repeats;
+  const orc_system *systems;
+  const long long *offsets;
+  const double *x, *y, *z;
+  double *fx, *fy, *fz;
+  double energy;
+} batch_task;
+
+static void *batch_worker(void *arg) {
+  batch_task *t = (batch_task *)arg;
+  double esum = 0.0;
+  for (int r = 0; r < t->repeats; r++) {
+    for (int i = t->first; i < t->last; i++) {
+      const long long o = t->offsets[i];
+      const int n = t->systems[i].natom;
+      if (t->eval_force) {
+        memset(t->fx + o, 0, sizeof(double) * n);
+        memset(t->fy + o, 0, sizeof(double) * n);
+        memset(t->fz + o, 0, sizeof(double) * n);
+      }
+      orc_energy e;
+      orc_eval_mm(&t->systems[i], t->x + o, t->y + o, t->z + o, t->fx + o, t->fy + o, t->fz + o,
+                  t->eval_force, t->nrg_bits, &e);
+      if (r == t->repeats - 1) {
+        for (int k = 0; k <= ORC_RESTRAINT; k++) {
+          esum += e.nrg[k];
+        }
+      }
+    }
+  }
+  t->energy = esum;
+  return NULL;
+}
+
+double orc_eval_batch(int nsys, const orc_system *systems, const long long *offsets,
+                      const double *x, const double *y, const double *z, double *fx, double *fy,
+                      double *fz, int eval_force, int nrg_bits, int repeats, int nthreads,
+                      double *total_energy) {
+  if (nthreads < 1) {
+    nthreads = 1;
+  }
+  if (nthreads > nsys) {
+    nthreads = nsys > 0 ? nsys : 1;
+  }
+  batch_task *tasks = (batch_task *)calloc(nthreads, sizeof(batch_task));
+  pthread_t *th = (pthread_t *)calloc(nthreads, sizeof(pthread_t));
+
+  /* Balance disjoint contiguous system ranges by pair work (N^2). */
+  double total = 0.0;
+  for (int i = 0; i < nsys; i++) {
+    total += (double)systems[i].natom * systems[i].natom;
+  }
+  int cursor = 0;
+  double running = 0.0;
+  for (int t = 0; t < nthreads; t++) {
+    tasks[t].first = cursor;
+    const double target = total * (double)(t + 1) / (double)nthreads;
+    while (cursor < nsys && (running < target || t == nthreads - 1)) {
+      running += (double)systems[cursor].natom * systems[cursor].natom;
+      cursor++;
+    }
+    tasks[t].last = cursor;
+    tasks[t].systems = systems;
+    tasks[t].offsets = offsets;
+    tasks[t].x = x; tasks[t].y = y; tasks[t].z = z;
+    tasks[t].fx = fx; tasks[t].fy = fy; tasks[t].fz = fz;
+    tasks[t].eval_force = eval_force;
+    tasks[t].nrg_bits = nrg_bits;
+    tasks[t].repeats = repeats;
+  }
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  for (int t = 0; t < nthreads; t++) {
+    pthread_create(&th[t], NULL, batch_worker, &tasks[t]);
+  }
+  double esum = 0.0;
+  for (int t = 0; t < nthreads; t++) {
+    pthread_join(th[t], NULL);
+    esum += tasks[t].energy;
+  }
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  if (total_energy != NULL) {
+    *total_energy = esum;
+  }
+  free(tasks);
+  free(th);
+  return (double)(t1.tv_sec - t0.tv_sec) + 1.0e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+/* ----------------------------------------------------------------------------------------- */
+/* GPU minimizer semantics                                                                     */
+/* ----------------------------------------------------------------------------------------- */
+
+/* Move 3 of src/MolecularMechanics/line_movement.cui:129-338: solve for the cubic through the
+ * four (move, energy) points by the same Gauss-Jordan sequence on the augmented [A | I] columns,
+ * then choose the move. */
+double orc_cubic_move(const double evec[4], const double mvec[4], double *new_baseline_move,
+                      double coefs_out[4]) {
+  double col[8][4];
+  for (int r = 0; r < 4; r++) {
+    const double xv = mvec[r];
+    const double x2 = xv * xv;
+    col[0][r] = x2 * xv;
+    col[1][r] = x2;
+    col[2][r] = xv;
+    col[3][r] = 1.0;
+    for (int c = 4; c < 8; c++) {
+      col[c][r] = (c - 4 == r) ? 1.0 : 0.0;
+    }
+  }
+  for (int i = 0; i < 4; i++) {
+    /* Pivot search within the pivot column (lane i). */
+    double maxval = fabs(col[i][i]);
+    int maxpos = i;
+    for (int j = i + 1; j < 4; j++) {
+      const double a = fabs(col[i][j]);
+      if (a > maxval) {
+        maxval = a;
+        maxpos = j;
+      }
+    }
+    for (int c = 0; c < 8; c++) {
+      const double tmp = col[c][maxpos];
+      col[c][maxpos] = col[c][i];
+      col[c][i] = tmp;
+    }
+    for (int j = i + 1; j < 4; j++) {
+      const double mult = col[i][j] / col[i][i];
+      for (int c = i; c < 8; c++) {
+        col[c][j] -= mult * col[c][i];
+      }
+    }
+  }
+  for (int i = 3; i >= 0; i--) {
+    const double div = col[i][i];
+    for (int c = 0; c < 8; c++) {
+      col[c][i] /= div;
+    }
+    for (int j = i - 1; j >= 0; j--) {
+      const double mult = col[i][j];
+      for (int c = i; c < 8; c++) {
+        col[c][j] -= mult * col[c][i];
+      }
+    }
+  }
+  double abcd[4];
+  for (int r = 0; r < 4; r++) {
+    abcd[r] = (col[4][r] * evec[0]) + (col[5][r] * evec[1]) + (col[6][r] * evec[2]) +
+              (col[7][r] * evec[3]);
+  }
+  if (coefs_out != NULL) {
+    for (int r = 0; r < 4; r++) {
+      coefs_out[r] = abcd[r];
+    }
+  }
+  double move_factor, nbm;
+  const double e0 = evec[0];
+  if (e0 < evec[1] && e0 < evec[2] && e0 < evec[3]) {
+    move_factor = -mvec[3];
+    nbm = 0.8;
+  }
+  else {
+    const double d0 = 3.0 * abcd[0];
+    const double d1 = 2.0 * abcd[1];
+    const double d2 = abcd[2];
+    const double sqrt_arg = (d1 * d1) - (4.0 * d0 * d2);
+    if (sqrt_arg < 0.0) {
+      if (evec[0] < evec[3]) {
+        move_factor = -mvec[3];
+        nbm = 0.8;
+      }
+      else {
+        move_factor = 0.0;
+        nbm = 1.2;
+      }
+    }
+    else {
+      const double ext_i = (-d1 + sqrt(sqrt_arg)) / (2.0 * d0);
+      const double ext_ii = (-d1 - sqrt(sqrt_arg)) / (2.0 * d0);
+      const double min_pos = ((2.0 * d0 * ext_i) + d1 > 0.0) ? ext_i : ext_ii;
+      if (min_pos <= 0.0) {
+        if (evec[3] >= evec[0]) {
+          move_factor = -mvec[3];
+          nbm = 0.8;
+        }
+        else {
+          move_factor = 0.0;
+          nbm = 1.2;
+        }
+      }
+      else if (min_pos < mvec[3]) {
+        const double epred = ((((abcd[0] * min_pos) + abcd[1]) * min_pos) + abcd[2]) * min_pos +
+                             abcd[3];
+        if (epred > evec[0]) {
+          move_factor = -mvec[3];
+          nbm = 0.8;
+        }
+        else if (epred < evec[3]) {
+          move_factor = min_pos - mvec[3];
+          nbm = (min_pos > 0.6) ? 1.05 : 0.95;
+        }
+        else {
+          move_factor = 0.0;
+          nbm = 1.05;
+        }
+      }
+      else {
+        move_factor = 0.0;
+        nbm = 1.05;
+      }
+    }
+  }
+  *new_baseline_move = nbm;
+  return move_factor;
+}
+
+/* Double to int64 with round-half-even, the semantics of __double2ll_rn. */
+static inline long long d2ll_rn(double v) { return (long long)nearbyint(v); }
+
+/* Energy the line minimizer reads: the sum of the fixed-precision accumulators of the 13
+ * potential-energy slots BOND..RESTRAINT (src/MolecularMechanics/line_movement.cui:25-91). */
+static double line_energy(const orc_energy *e, int nrg_bits) {
+  long long s = 0;
+  for (int k = 0; k <= ORC_RESTRAINT; k++) {
+    s += e->acc[k];
+  }
+  return (double)s * ldexp(1.0, -nrg_bits);
+}
+
+static void eval_fixed(const orc_system *s, const long long *xc, const long long *yc,
+                       const long long *zc, int gpos_bits, int frc_bits, int nrg_bits,
+                       int eval_force, long long *ifx, long long *ify, long long *ifz,
+                       double *work, orc_energy *e) {
+  const int n = s->natom;
+  double *x = work, *y = work + n, *z = work + 2 * n;
+  double *fx = work + 3 * n, *fy = work + 4 * n, *fz = work + 5 * n;
+  const double inv_gpos = ldexp(1.0, -gpos_bits);
+  for (int i = 0; i < n; i++) {
+    x[i] = (double)xc[i] * inv_gpos;
+    y[i] = (double)yc[i] * inv_gpos;
+    z[i] = (double)zc[i] * inv_gpos;
+    fx[i] = fy[i] = fz[i] = 0.0;
+  }
+  orc_eval_mm(s, x, y, z, fx, fy, fz, eval_force, nrg_bits, e);
+  if (eval_force) {
+    const double fscale = ldexp(1.0, frc_bits);
+    for (int i = 0; i < n; i++) {
+      ifx[i] = llround(fx[i] * fscale);
+      ify[i] = llround(fy[i] * fscale);
+      ifz[i] = llround(fz[i] * fscale);
+    }
+  }
+}
+
+void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long long *zcrd,
+                  int gpos_bits, int frc_bits, int nrg_bits, int nsteps, int sd_cycles,
+                  double dx0, int dp_mode, double *energy_trace) {
+  const int n = s->natom;
+  long long *buf = (long long *)calloc((size_t)9 * n, sizeof(long long));
+  long long *fx = buf, *fy = buf + n, *fz = buf + 2 * n;
+  long long *px = buf + 3 * n, *py = buf + 4 * n, *pz = buf + 5 * n;
+  long long *cx = buf + 6 * n, *cy = buf + 7 * n, *cz = buf + 8 * n;
+  double *work = (double *)calloc((size_t)6 * n, sizeof(double));
+  const double inv_frc_scale = ldexp(1.0, -frc_bits);
+  const double gpos_scale = ldexp(1.0, gpos_bits);
+  double l_move = dx0, s_move = dx0;
+  double mfac_a = 0.0, mfac_b = 0.0, mfac_c = 0.0;
+  double nrg_a = 0.0, nrg_b = 0.0, nrg_c = 0.0, nrg_d = 0.0;
+  int step_counter = 0;   /* MMControlKit::step of the force-evaluation controls */
+  orc_energy e;
+  long long *crd[3] = {xcrd, ycrd, zcrd};
+  long long *frc[3] = {fx, fy, fz};
+  long long *prv[3] = {px, py, pz};
+  long long *cgt[3] = {cx, cy, cz};
+
+  for (int it = 0; it < nsteps; it++) {
+    /* Force + energy evaluation (hpc_minimization.cu:264-287). */
+    eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 1, fx, fy, fz, work, &e);
+    step_counter += 1;
+
+    /* Conjugate gradient all-reduce (conjugate_gradient.cui:259-370, hpc_reduction.cu:39-244). */
+    double gg = 0.0, dgg = 0.0;
+    for (int d = 0; d < 3; d++) {
+      for (int i = 0; i < n; i++) {
+        const long long lp = prv[d][i];
+        const long long lf = frc[d][i];
+        const double dp = (double)lp;
+        const double df = (double)lf;
+        const double dd = (double)(lf - lp);
+        gg += dp * dp;
+        dgg += dd * df;
+      }
+    }
+    const double gam = (step_counter >= sd_cycles) ? (dgg / gg) : 0.0;
+    double msum = 0.0;
+    for (int d = 0; d < 3; d++) {
+      for (int i = 0; i < n; i++) {
+        const long long ifv = frc[d][i];
+        prv[d][i] = ifv;
+        if (fabs(gam) > 1.0e-12) {
+          const double dfcg = (double)ifv + (gam * (double)cgt[d][i]);
+          msum += dfcg * dfcg;
+          const long long cg = d2ll_rn(dfcg);
+          cgt[d][i] = cg;
+          frc[d][i] = cg;
+        }
+        else {
+          const double dfcg = (double)ifv;
+          msum += dfcg * dfcg;
+          cgt[d][i] = ifv;
+        }
+      }
+    }
+    const double inv_factor = 1.0 / (sqrt(msum) * inv_frc_scale);
+    for (int d = 0; d < 3; d++) {
+      for (int i = 0; i < n; i++) {
+        const double v = (double)frc[d][i] * inv_factor;
+        frc[d][i] = dp_mode ? d2ll_rn(v) : (long long)v;
+      }
+    }
+
+    /* Four line moves (line_movement.cui), with energy-only evaluations between them. */
+    for (int move = 0; move < 4; move++) {
+      if (move > 0) {
+        long long dummy = 0;
+        (void)dummy;
+        eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 0, NULL, NULL, NULL,
+                   work, &e);
+      }
+      const double cur = line_energy(&e, nrg_bits);
+      if (move == 0 && energy_trace != NULL) {
+        energy_trace[it] = cur;
+      }
+      double move_factor = 0.0;
+      if (move == 0) {
+        nrg_a = cur;
+        move_factor = 1.0;
+        mfac_a = move_factor;
+        s_move = l_move;
+      }
+      else if (move == 1) {
+        nrg_b = cur;
+        move_factor = (cur < nrg_a) ? 1.04 : 0.8;
+        mfac_b = move_factor;
+      }
+      else if (move == 2) {
+        nrg_c = cur;
+        move_factor = (cur < nrg_b) ? mfac_b * 1.03 : mfac_b * 0.85;
+        mfac_c = move_factor;
+      }
+      else {
+        nrg_d = cur;
+        const double evec[4] = {nrg_a, nrg_b, nrg_c, nrg_d};
+        const double mvec[4] = {0.0, mfac_a, mfac_b + mfac_a, mfac_c + mfac_b + mfac_a};
+        double nbm;
+        move_factor = orc_cubic_move(evec, mvec, &nbm, NULL);
+        l_move = s_move * nbm;
+      }
+      if (fabs(move_factor) > 1.0e-12) {
+        move_factor *= inv_frc_scale * gpos_scale;
+        move_factor *= (move == 0) ? l_move : s_move;
+        for (int d = 0; d < 3; d++) {
+          for (int i = 0; i < n; i++) {
+            crd[d][i] += d2ll_rn(move_factor * (double)frc[d][i]);
+          }
+        }
+      }
+    }
+  }
+  if (energy_trace != NULL) {
+    eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 0, NULL, NULL, NULL, work, &e);
+    energy_trace[nsteps] = line_energy(&e, nrg_bits);
+  }
+  (void)nrg_d;
+  (void)nrg_c;
+  free(buf);
+  free(work);
+}

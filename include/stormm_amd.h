@@ -1,0 +1,261 @@
+/*
+ * stormm_amd.h -- C-ABI of the MI355X batched molecular-mechanics energy/force path.
+ *
+ * The entry points replace the reference's launch surface for this path (paths relative to the
+ * reference repository root, suobset/stormm):
+ *
+ *   stormm_launch_nonbonded          launchNonbonded        src/Potential/hpc_nonbonded_potential.h:142-157
+ *   stormm_launch_valence            launchValence          src/Potential/hpc_valence_potential.h:356-378
+ *   stormm_launch_mm_evaluation      launchNonbonded + launchValence back to back, as driven by
+ *                                    test/Synthesis/test_hpc_synthesis.cu:106,143
+ *   stormm_launch_conjugate_gradient launchConjugateGradient src/Math/hpc_reduction.h:55-63
+ *   stormm_launch_line_advance       launchLineAdvance      src/MolecularMechanics/hpc_minimization.h:63-70
+ *   stormm_launch_minimization       launchMinimization     src/MolecularMechanics/hpc_minimization.h:122-151
+ *   stormm_phasespace_initialize_forces   PhaseSpaceSynthesis::initializeForces
+ *                                         src/Synthesis/phasespace_synthesis.h:787-797
+ *   stormm_phasespace_prime_conjugate_gradient  PhaseSpaceSynthesis::primeConjugateGradientCalculation
+ *                                         src/Synthesis/phasespace_synthesis.h:799-808
+ *   stormm_scorecard_initialize      ScoreCard::initialize  src/Potential/scorecard.h:212-222
+ *   stormm_scorecard_commit          ScoreCard::commit      src/Potential/scorecard.h:248-262
+ *
+ * Objects (AtomGraphSynthesis, PhaseSpaceSynthesis, ScoreCard, MolecularMechanicsControls,
+ * LineMinimization) are opaque handles owning device memory.  Their device layouts follow the
+ * reference abstracts: fixed-precision int64 coordinates and forces per atom with each system's
+ * atoms starting on a 32-atom boundary (PsSynthesisWriter, src/Synthesis/phasespace_synthesis.h:
+ * 87-195), ScoreCard int64 accumulators with a stride of 32 slots per system in StateVariable
+ * order (src/Potential/scorecard.h:64-95, src/Potential/energy_enumerators.h:36-75).
+ *
+ * Every function returns 0 on success and a negative STORMM_ERR_* code on failure;
+ * stormm_last_error() returns a description of the last failure on the calling thread.
+ * Streams are hipStream_t passed as void* (NULL = the default stream).
+ */
+#ifndef STORMM_AMD_H
+#define STORMM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STORMM_OK 0
+#define STORMM_ERR_INVALID_ARGUMENT (-1)
+#define STORMM_ERR_HIP (-2)
+#define STORMM_ERR_CAPACITY (-3)
+#define STORMM_ERR_STATE (-4)
+
+/* Precision models, src/Constants/behavior.h (PrecisionModel). */
+#define STORMM_PREC_DOUBLE 0
+#define STORMM_PREC_SINGLE 1
+
+/* Implicit solvent models (AMBER igb numbers). */
+#define STORMM_IGB_NONE 0
+#define STORMM_IGB_HCT 1
+#define STORMM_IGB_OBC 2
+#define STORMM_IGB_OBC2 5
+
+/* ScoreCard slots in StateVariable order, src/Potential/energy_enumerators.h:36-75. */
+#define STORMM_NRG_BOND 0
+#define STORMM_NRG_ANGLE 1
+#define STORMM_NRG_PROPER_DIHEDRAL 2
+#define STORMM_NRG_IMPROPER_DIHEDRAL 3
+#define STORMM_NRG_VDW 7
+#define STORMM_NRG_VDW_ONE_FOUR 8
+#define STORMM_NRG_ELECTROSTATIC 9
+#define STORMM_NRG_ELEC_ONE_FOUR 10
+#define STORMM_NRG_GENERALIZED_BORN 11
+#define STORMM_NRG_RESTRAINT 12
+#define STORMM_NRG_POTENTIAL_ENERGY 28
+#define STORMM_SCORECARD_STRIDE 32
+
+/* Many molecules' parameters, concatenated system after system.  Atom indices inside the term
+ * arrays are local to their system.  This is the input the reference assembles from a list of
+ * AtomGraph objects in the AtomGraphSynthesis constructor
+ * (src/Synthesis/ag_synthesis_constructors.cpp:11-468). */
+typedef struct {
+  int32_t system_count;
+  const int32_t *atom_counts;      /* [nsys]                                                    */
+  const double *charge;            /* [sum atoms] e                                             */
+  const int32_t *lj_type;          /* [sum atoms] type index local to the system's table        */
+  const int32_t *lj_type_counts;   /* [nsys] T                                                  */
+  const double *lja, *ljb;         /* [sum T^2] A / B, row-major (ti * T + tj) per system        */
+  const double *lja14, *ljb14;     /* [sum T^2] 1:4 A / B                                       */
+  const double *pb_radii;          /* [sum atoms] intrinsic radii (A)                           */
+  const double *gb_screen;         /* [sum atoms] screening factors                             */
+  const int32_t *bond_counts;      /* [nsys]                                                    */
+  const int32_t *bond_atoms;       /* [2 * sum bonds]                                           */
+  const double *bond_k, *bond_r0;
+  const int32_t *angle_counts;
+  const int32_t *angle_atoms;      /* [3 * sum angles]                                          */
+  const double *angle_k, *angle_theta;
+  const int32_t *dihedral_counts;
+  const int32_t *dihedral_atoms;   /* [4 * sum dihedrals]                                       */
+  const double *dihe_amp, *dihe_phase, *dihe_freq;
+  const int32_t *dihe_improper;    /* 0 proper, 1 improper                                      */
+  const double *dihe_scee;         /* 1:4 electrostatic divisor; <= 0: no 1:4 pair on this term */
+  const double *dihe_scnb;         /* 1:4 Lennard-Jones divisor                                 */
+  const int32_t *exclusion_counts; /* [nsys] number of excluded (i < j) pairs                   */
+  const int32_t *exclusion_pairs;  /* [2 * sum exclusions]                                      */
+  int32_t igb;                     /* one implicit-solvent model for the whole synthesis        */
+  double dielectric;
+  double salt_concentration;
+  double coulomb_constant;
+} stormm_topology_batch;
+
+/* Work-unit shaping, chosen by the autotuner (or fixed by the caller).  Zero = default. */
+typedef struct {
+  int32_t wu_atom_capacity;        /* atoms staged in LDS per work unit (<= 512)                */
+  int32_t target_wu_per_cu;        /* work units per compute unit to aim for when packing       */
+  int32_t block_threads;           /* threads per work-unit block (64..1024, multiple of 64)    */
+} stormm_launch_shape;
+
+typedef struct stormm_synthesis stormm_synthesis;
+typedef struct stormm_phasespace stormm_phasespace;
+typedef struct stormm_scorecard stormm_scorecard;
+typedef struct stormm_mm_controls stormm_mm_controls;
+typedef struct stormm_line_min stormm_line_min;
+
+typedef struct {
+  int32_t system_count;
+  int32_t padded_atom_count;       /* length of the per-atom device arrays                      */
+  int32_t work_unit_count;
+  int32_t task_count;
+  int32_t max_wu_atoms;
+  int32_t lj_table_systems;        /* systems using the stored LJ table (not a combining rule)  */
+  int64_t pair_count;              /* sum over systems of N (N - 1) / 2                         */
+  int64_t valence_term_count;
+} stormm_synthesis_info;
+
+/* Device pointers of a PhaseSpaceSynthesis, the PsSynthesisWriter layout
+ * (src/Synthesis/phasespace_synthesis.h:87-195). */
+typedef struct {
+  int32_t system_count;
+  int32_t gpos_bits, frc_bits;
+  double gpos_scale, inv_gpos_scale, frc_scale, inv_frc_scale;
+  const int32_t *atom_starts, *atom_counts;
+  int64_t *xcrd, *ycrd, *zcrd;
+  int64_t *xfrc, *yfrc, *zfrc;
+  int64_t *xprv, *yprv, *zprv;          /* prior-step forces for conjugate gradient            */
+  int64_t *x_cg_temp, *y_cg_temp, *z_cg_temp;
+} stormm_ps_writer;
+
+/* Minimization controls, the subset of MinimizeControls this path reads
+ * (src/Namelists/nml_minimize.h:21-30). */
+typedef struct {
+  int32_t total_cycles;            /* maxcyc                                                    */
+  int32_t steepest_descent_cycles; /* ncyc                                                      */
+  double initial_step;             /* dx0 (A)                                                   */
+  double convergence_rms_force;    /* stop a system once its RMS force <= this (0 = never)      */
+} stormm_minimize_controls;
+
+const char *stormm_last_error(void);
+int stormm_device_info(int32_t *cu_count, int32_t *lds_bytes_per_cu, char *name, int32_t name_len);
+
+/* ---- AtomGraphSynthesis ------------------------------------------------------------------- */
+/* src/Synthesis/atomgraph_synthesis.h, constructor ag_synthesis_constructors.cpp:11-468 */
+int stormm_synthesis_create(const stormm_topology_batch *batch, const stormm_launch_shape *shape,
+                            stormm_synthesis **out);
+int stormm_synthesis_destroy(stormm_synthesis *ag);
+int stormm_synthesis_get_info(const stormm_synthesis *ag, stormm_synthesis_info *info);
+int stormm_synthesis_atom_layout(const stormm_synthesis *ag, int32_t *atom_starts,
+                                 int32_t *atom_counts);
+/* Build the host-side plan (layout, work units, tile tasks) without touching the device. */
+int stormm_synthesis_plan(const stormm_topology_batch *batch, const stormm_launch_shape *shape,
+                          stormm_synthesis_info *info);
+/* Kernel status bits (1 = coordinates beyond the LDS coordinate range); clear != 0 resets. */
+int stormm_synthesis_status(const stormm_synthesis *ag, int32_t *status, int32_t clear);
+/* Autotuner: time candidate launch shapes on the device and return the fastest. */
+int stormm_autotune_launch_shape(const stormm_topology_batch *batch, int32_t precision,
+                                 stormm_launch_shape *best, double *best_ms);
+
+/* ---- PhaseSpaceSynthesis ------------------------------------------------------------------ */
+/* src/Synthesis/phasespace_synthesis.h:325-383 */
+int stormm_phasespace_create(const stormm_synthesis *ag, int32_t gpos_bits, int32_t frc_bits,
+                             stormm_phasespace **out);
+int stormm_phasespace_destroy(stormm_phasespace *ps);
+/* xyz: [sum atoms][3] double (unpadded, system after system); host <-> device, synchronous */
+int stormm_phasespace_upload_coordinates(stormm_phasespace *ps, const double *xyz);
+int stormm_phasespace_download_coordinates(const stormm_phasespace *ps, double *xyz);
+int stormm_phasespace_download_forces(const stormm_phasespace *ps, double *fxyz);
+/* Raw fixed-precision copies, [sum atoms][3] int64 */
+int stormm_phasespace_upload_fixed(stormm_phasespace *ps, const int64_t *xyz);
+int stormm_phasespace_download_fixed(const stormm_phasespace *ps, int64_t *xyz, int64_t *fxyz);
+int stormm_phasespace_writer(stormm_phasespace *ps, stormm_ps_writer *w);
+int stormm_phasespace_initialize_forces(stormm_phasespace *ps, void *stream);
+int stormm_phasespace_prime_conjugate_gradient(stormm_phasespace *ps, void *stream);
+
+/* ---- ScoreCard ---------------------------------------------------------------------------- */
+/* src/Potential/scorecard.h:101-550 */
+int stormm_scorecard_create(int32_t system_count, int32_t nrg_bits, stormm_scorecard **out);
+int stormm_scorecard_destroy(stormm_scorecard *sc);
+int stormm_scorecard_initialize(stormm_scorecard *sc, void *stream);
+int stormm_scorecard_commit(stormm_scorecard *sc, void *stream);
+/* instantaneous accumulators, [nsys][32] int64 */
+int stormm_scorecard_download(const stormm_scorecard *sc, int64_t *accumulators);
+/* running sums and sums of squares of committed samples, [nsys][32] double each */
+int stormm_scorecard_download_history(const stormm_scorecard *sc, double *running,
+                                      double *squared, int32_t *sample_count);
+int64_t *stormm_scorecard_device_accumulators(stormm_scorecard *sc);
+
+/* ---- MolecularMechanicsControls ----------------------------------------------------------- */
+/* src/MolecularMechanics/mm_controls.h:100-363 */
+int stormm_mm_controls_create(const stormm_minimize_controls *mincon,
+                              stormm_mm_controls **out);
+int stormm_mm_controls_destroy(stormm_mm_controls *ctrl);
+int stormm_mm_controls_get_step(const stormm_mm_controls *ctrl, int32_t *step);
+int stormm_mm_controls_set_step(stormm_mm_controls *ctrl, int32_t step);
+
+/* ---- LineMinimization --------------------------------------------------------------------- */
+/* src/MolecularMechanics/line_minimization.h:85-221 */
+int stormm_line_min_create(int32_t system_count, double dx0, stormm_line_min **out);
+int stormm_line_min_destroy(stormm_line_min *lm);
+/* [nsys] each: l_move, s_move, mfac_a, mfac_b, mfac_c, nrg_a, nrg_b, nrg_c, nrg_d */
+int stormm_line_min_download(const stormm_line_min *lm, double *values9xN);
+
+/* ---- Launch entry points ------------------------------------------------------------------ */
+/* Coulomb + Lennard-Jones + (if the synthesis has an implicit-solvent model) Born radii, GB pair
+ * energies and the radius-derivative pass.  Forces are added into the PhaseSpaceSynthesis force
+ * arrays and energies into the ScoreCard instantaneous accumulators. */
+int stormm_launch_nonbonded(int32_t precision, const stormm_synthesis *ag,
+                            stormm_mm_controls *ctrl, stormm_phasespace *ps,
+                            stormm_scorecard *sc, int32_t eval_force, int32_t eval_energy,
+                            void *stream);
+/* Bonds, angles, cosine dihedrals and attenuated 1:4 pairs. */
+int stormm_launch_valence(int32_t precision, const stormm_synthesis *ag,
+                          stormm_mm_controls *ctrl, stormm_phasespace *ps, stormm_scorecard *sc,
+                          int32_t eval_force, int32_t eval_energy, void *stream);
+/* All terms in one fused launch. */
+int stormm_launch_mm_evaluation(int32_t precision, const stormm_synthesis *ag,
+                                stormm_mm_controls *ctrl, stormm_phasespace *ps,
+                                stormm_scorecard *sc, int32_t eval_force, int32_t eval_energy,
+                                void *stream);
+/* Polak-Ribiere direction mixing and normalization of the force arrays per system. */
+int stormm_launch_conjugate_gradient(int32_t precision, const stormm_synthesis *ag,
+                                     stormm_phasespace *ps, stormm_mm_controls *ctrl,
+                                     void *stream);
+/* One of the four line moves (move_number 0..3). */
+int stormm_launch_line_advance(int32_t precision, const stormm_synthesis *ag,
+                               stormm_phasespace *ps, stormm_scorecard *sc, stormm_line_min *lm,
+                               int32_t move_number, void *stream);
+/* Whole minimization.  persistent != 0 runs every cycle of every system inside one launch with
+ * the systems' state held in LDS; persistent == 0 issues the reference's per-step launch
+ * sequence.  Final energies land in the ScoreCard. */
+int stormm_launch_minimization(int32_t precision, const stormm_synthesis *ag,
+                               stormm_phasespace *ps, const stormm_minimize_controls *mincon,
+                               stormm_scorecard *sc, stormm_line_min *lm, int32_t persistent,
+                               void *stream);
+/* Per-system status after a persistent minimization: cycles run and final RMS force. */
+int stormm_minimization_status(const stormm_synthesis *ag, int32_t *cycles_run,
+                               double *rms_force);
+
+/* ---- Timing helpers (HIP events on the launch stream) -------------------------------------- */
+/* Run `repeats` fused evaluations back to back on `stream` and return the mean kernel time in
+ * milliseconds measured with HIP events recorded on that stream. */
+int stormm_time_mm_evaluation(int32_t precision, const stormm_synthesis *ag,
+                              stormm_phasespace *ps, stormm_scorecard *sc, int32_t eval_force,
+                              int32_t repeats, void *stream, double *mean_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,104 @@
+// Internal device-side layouts shared by the host builder (synthesis.cpp) and the kernels
+// (mm_kernels.hip).  Not part of the public C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace stormm_amd {
+
+// Tile edge of the all-pairs work: one 16-lane DPP row of a wave64 owns a 16 x 16 tile of
+// atom pairs and rotates the j-atom data through its lanes.
+constexpr int kTile = 16;
+// Systems are laid out in the global per-atom arrays starting on 32-atom boundaries, the
+// reference's warp-size padding (src/Synthesis/phasespace_synthesis.cpp, atom_starts).
+constexpr int kGlobalPad = 32;
+// Largest number of atoms a work unit stages in LDS.
+constexpr int kMaxWuAtoms = 512;
+// Largest number of systems packed into one work unit.
+constexpr int kMaxWuSystems = 32;
+// Energy components tracked per system inside a work unit (StateVariable slots 0..12).
+constexpr int kNrgSlots = 13;
+// ScoreCard stride (slots per system), src/Potential/scorecard.cpp:66 with 32-wide warps.
+constexpr int kScoreStride = 32;
+// Fixed-precision scales of the LDS accumulators for the Born-radius integral and dE/dR.
+constexpr int kPsiBits = 40;
+constexpr int kDedrBits = 32;
+// LDS coordinate resolution (bits after the binary point) in single-precision mode.
+constexpr int kLdsCrdBits = 24;
+
+// StateVariable slots, src/Potential/energy_enumerators.h:36-75.
+enum NrgSlot : int {
+  kBond = 0, kAngle = 1, kProperDihedral = 2, kImproperDihedral = 3, kVdw = 7, kVdw14 = 8,
+  kElec = 9, kElec14 = 10, kGb = 11
+};
+
+// Term selection for the fused kernel.
+enum TermMask : int { kTermNonbonded = 1, kTermValence = 2 };
+
+// Valence term kinds packed in VTerm.x bits 0..3; the work-unit system slot sits in bits 8..15.
+enum VKind : int { kVBond = 0, kVAngle = 1, kVDihedral = 2 };
+
+// Work-unit descriptor, 8 ints.
+struct WuDesc {
+  int sys_first;     // index into the work-unit system list
+  int sys_count;
+  int task_first;    // index into the task list
+  int task_count;
+  int lds_atoms;     // atoms staged in LDS (each system padded to kTile)
+  int term_first;    // index into the valence term list
+  int term_count;
+  int lj_table;      // 1 = systems in this unit read their stored LJ tables
+};
+
+// One system inside a work unit.
+struct WuSys {
+  int sys_index;
+  int lds_base;
+  int global_start;
+  int natom;
+};
+
+// One 16 x 16 tile task: x = LDS base of the i tile | (LDS base of the j tile << 16);
+// y = system slot | (diagonal << 8) | (ni << 12) | (nj << 20); z = exclusion-mask offset (-1 when
+// the tile holds no excluded pair); w = unused.
+using Task = int4;
+
+// Everything the fused kernel reads, passed by value.
+struct MMKit {
+  int nsys, nwu, maxa, block;
+  int gpos_bits, frc_bits, nrg_bits;
+  int igb;
+  double dielectric, kappa;
+  const int *sys_start, *sys_natom;
+  const WuDesc *wu;
+  const WuSys *wu_sys;
+  const Task *tasks;
+  const unsigned *masks;
+  const int4 *vterms;
+  const double2 *bond_par;   // k, r0
+  const double2 *angl_par;   // k, theta0
+  const double4 *dihe_par;   // amp, phase, freq, improper (0 / 1)
+  const double2 *dihe14;     // 1 / scee (0 = none), 1 / scnb
+  const double *q;           // charge * sqrt(Coulomb constant)
+  const double *ljp1;        // sigma (combining-rule systems)
+  const double *ljp2;        // 2 sqrt(epsilon) (combining-rule systems)
+  const int *lj_type;        // type index (table systems)
+  const double2 *ljab;       // A, B per system table
+  const double2 *ljab14;     // 1:4 A, B per system table
+  const int *lj_off, *lj_nt; // per-system table offset and type count
+  const double *rho;         // intrinsic radius - offset
+  const double *scr;         // screening factor * rho
+  const double *pbr;         // intrinsic radius
+  const double *gb_alpha, *gb_beta, *gb_gamma;
+  long long *xcrd, *ycrd, *zcrd;
+  long long *xfrc, *yfrc, *zfrc;
+  long long *scorecard;
+  int *status;
+};
+
+// Status bits reported by kernels.
+enum StatusBit : int { kStatusCoordinateRange = 1, kStatusNonFinite = 2 };
+
+}  // namespace stormm_amd

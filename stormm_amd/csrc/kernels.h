@@ -1,0 +1,58 @@
+// Host-callable launchers of the device kernels (implemented in the .hip translation units).
+#pragma once
+
+#include "internal.h"
+
+namespace stormm_amd {
+
+// Threads per work-unit block of the evaluation and minimization kernels.
+constexpr int kEvalBlock = 256;
+
+size_t eval_lds_bytes(int precision, int maxa);
+hipError_t launch_mm_eval(int precision, const MMKit &k, int terms, int eval_force,
+                          int eval_energy, hipStream_t stream);
+hipError_t launch_fill(long long *a, long long *b, long long *c, size_t n, hipStream_t stream);
+hipError_t launch_scorecard_commit(const long long *inst, double *running, double *squared,
+                                   int nsys, double inv_scale, hipStream_t stream);
+
+// Minimizer pieces (mm_minimize.hip).
+struct CgKit {
+  int nsys;
+  const int *sys_start, *sys_natom;
+  long long *xfrc, *yfrc, *zfrc;
+  long long *xprv, *yprv, *zprv;
+  long long *xcg, *ycg, *zcg;
+  double inv_frc_scale;
+  int dp_mode;
+};
+struct LineKit {
+  int nsys;
+  const int *sys_start, *sys_natom;
+  long long *xcrd, *ycrd, *zcrd;
+  const long long *xfrc, *yfrc, *zfrc;
+  const long long *scorecard;
+  double inv_nrg_scale;
+  double inv_frc_scale, gpos_scale;
+  double *l_move, *s_move, *mfac_a, *mfac_b, *mfac_c, *nrg_a, *nrg_b, *nrg_c, *nrg_d;
+  int dp_mode;
+};
+hipError_t launch_conjugate_gradient(const CgKit &k, int step, int sd_cycles,
+                                     hipStream_t stream);
+hipError_t launch_line_advance(const LineKit &k, int move_number, hipStream_t stream);
+
+// Persistent minimization: every cycle for every system of a work unit inside one block.
+struct MinKit {
+  MMKit mm;
+  long long *xprv, *yprv, *zprv;
+  long long *xcg, *ycg, *zcg;
+  int total_cycles, sd_cycles;
+  double dx0;
+  double conv_rms;        // stop a system when its RMS force falls to this (0 = never)
+  double *l_move;         // per-system line-minimization state (optional, may be null)
+  int *cycles_run;        // per-system cycles executed
+  double *rms_force;      // per-system final RMS force
+};
+hipError_t launch_minimize_persistent(int precision, const MinKit &k, hipStream_t stream);
+size_t minimize_lds_bytes(int precision, int maxa);
+
+}  // namespace stormm_amd

@@ -1,0 +1,249 @@
+// Device-side arithmetic shared by the fused evaluation kernel and the persistent minimizer:
+// precision traits, DPP row rotation, the Born-radius integral and its radial derivative, the
+// work-unit LDS layout and the valence term evaluators.
+#pragma once
+
+#include "internal.h"
+
+namespace stormm_amd {
+
+// Generalized Born Taylor-series coefficients (far-field expansion of the HCT integral),
+// src/Constants/generalized_born.h:27-36.
+constexpr double kGTA = 0.33333333333333333333, kGTB = 0.4, kGTC = 0.42857142857142857143,
+                 kGTD = 0.44444444444444444444, kGTDD = 0.45454545454545454545;
+constexpr double kGTE = 1.33333333333333333333, kGTF = 2.4, kGTG = 3.42857142857142857143,
+                 kGTH = 4.44444444444444444444, kGTHH = 5.45454545454545454545;
+constexpr double kGbKscale = 0.73;      // src/Constants/generalized_born.h:120
+constexpr double kGbOffset = 0.09;      // src/Constants/generalized_born.h:112
+
+// ------------------------------------------------------------------------------------------
+// Precision traits.  Single precision uses the hardware transcendental units (v_rsq, v_rcp,
+// v_log, v_exp); double precision uses the correctly rounded library routines.
+// ------------------------------------------------------------------------------------------
+template <typename T> struct Prec;
+
+template <> struct Prec<float> {
+  using crd_t = int;
+  static __device__ __forceinline__ float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ __forceinline__ float sqrt(float x) { return __builtin_sqrtf(x); }
+  static __device__ __forceinline__ float log(float x) {
+    return __builtin_amdgcn_logf(x) * 0.693147180559945309417f;
+  }
+  static __device__ __forceinline__ float exp(float x) {
+    return __builtin_amdgcn_exp2f(x * 1.44269504088896340736f);
+  }
+  static __device__ __forceinline__ long long fixed(float x) {
+    return static_cast<long long>(__builtin_rintf(x));
+  }
+  // Threshold past which the dihedral angle is recovered from the cross products' separation,
+  // src/Constants/symbol_values.h:78 (near_to_one_f).
+  static constexpr float kNearOne = 0.99993896484375f;
+  static constexpr float kAsymptoticToOne = 0.99999904632568359375f;
+};
+
+template <> struct Prec<double> {
+  using crd_t = double;
+  static __device__ __forceinline__ double rsqrt(double x) { return 1.0 / ::sqrt(x); }
+  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+  static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+  static __device__ __forceinline__ double log(double x) { return ::log(x); }
+  static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+  static __device__ __forceinline__ long long fixed(double x) {
+    return static_cast<long long>(__builtin_rint(x));
+  }
+  // The reference declares its double-precision near-to-one threshold as a float, which rounds
+  // to exactly 1.0 (src/Constants/symbol_values.h:79).
+  static constexpr double kNearOne = 1.0;
+  static constexpr double kAsymptoticToOne = 0.99999904632568359375;
+};
+
+// ------------------------------------------------------------------------------------------
+// Rotation of a value one lane to the right inside each 16-lane DPP row (row_ror:1): after k
+// rotations lane L holds what lane (L - k) mod 16 of its row held.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ int row_rotate(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x121, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float row_rotate(float v) {
+  return __int_as_float(row_rotate(__float_as_int(v)));
+}
+__device__ __forceinline__ double row_rotate(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = row_rotate(static_cast<int>(bits & 0xffffffffLL));
+  const int hi = row_rotate(static_cast<int>(bits >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              (static_cast<long long>(lo) & 0xffffffffLL));
+}
+
+// Sum of a 64-bit integer over the 16 lanes of a row.
+__device__ __forceinline__ long long row_sum(long long v) {
+  v += __shfl_xor(v, 8, 16);
+  v += __shfl_xor(v, 4, 16);
+  v += __shfl_xor(v, 2, 16);
+  v += __shfl_xor(v, 1, 16);
+  return v;
+}
+
+__device__ __forceinline__ void lds_add(long long *addr, long long v) {
+  atomicAdd(reinterpret_cast<unsigned long long *>(addr), static_cast<unsigned long long>(v));
+}
+
+// ------------------------------------------------------------------------------------------
+// Born-radius integral contribution of a sphere of screened radius sj at distance r onto an
+// atom of offset radius ri (HCT pairwise descreening): the far-field Taylor series when
+// r > 4 sj, otherwise the analytic branches of src/Potential/nonbonded_potential.tpp:578-617.
+// The three near branches share one reciprocal and one logarithm:
+//   r > ri + sj          : -sj / (2 (r^2 - sj^2))             - ln(|r - sj| / (r + sj)) / (4 r)
+//   |ri - sj| < r <= ri+sj: -(1/ri (2 - theta) - 1/(r + sj)) / 4 - ln(ri / (r + sj)) / (4 r)
+//   r <= |ri - sj|, ri<sj: -sj / (2 (r^2 - sj^2)) - 1/ri       - ln(|r - sj| / (r + sj)) / (4 r)
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T psi_term(T r, T r2, T invr, T ri, T iri, T sj) {
+  using P = Prec<T>;
+  T res = T(0);
+  if (r > T(4) * sj) {
+    const T invr2 = invr * invr;
+    const T tmpsd = sj * sj * invr2;
+    const T dumbo = T(kGTA) + tmpsd * (T(kGTB) + tmpsd * (T(kGTC) + tmpsd * (T(kGTD) +
+                                                                            tmpsd * T(kGTDD))));
+    res = -(sj * tmpsd * invr2 * dumbo);
+  }
+  else {
+    const bool b2 = r > ri + sj;
+    const bool b3 = (!b2) && (r > fabs(ri - sj));
+    const bool b4 = (!b2) && (!b3) && (ri < sj);
+    if (b2 || b3 || b4) {
+      const T uij = P::rcp(r + sj);
+      const T lg = P::log((b3 ? ri : fabs(r - sj)) * uij);
+      T a;
+      if (b3) {
+        const T theta = T(0.5) * iri * invr * (r2 + (ri * ri) - (sj * sj));
+        a = -T(0.25) * ((iri * (T(2) - theta)) - uij);
+      }
+      else {
+        const T t1 = P::rcp(r2 - (sj * sj));
+        a = -(T(0.5) * sj * t1) - (b4 ? iri : T(0));
+      }
+      res = a - (T(0.25) * invr * lg);
+    }
+  }
+  return res;
+}
+
+// Radial derivative factor of the same integral (dpsi/dr divided by r, sign convention of
+// src/Potential/nonbonded_potential.tpp:1138-1188).  The near branches again share the term
+// ln(L) / (4 r^3) with L as in psi_term.
+template <typename T>
+__device__ __forceinline__ T dpsi_term(T r, T r2, T invr, T invr2, T ri, T iri, T sj) {
+  using P = Prec<T>;
+  T res = T(0);
+  if (r > T(4) * sj) {
+    const T tmpsd = sj * sj * invr2;
+    const T dumbo = T(kGTE) + tmpsd * (T(kGTF) + tmpsd * (T(kGTG) + tmpsd * (T(kGTH) +
+                                                                            tmpsd * T(kGTHH))));
+    res = tmpsd * sj * invr2 * invr2 * dumbo;
+  }
+  else {
+    const bool b2 = r > ri + sj;
+    const bool b3 = (!b2) && (r > fabs(ri - sj));
+    const bool b4 = (!b2) && (!b3) && (ri < sj);
+    if (b2 || b3 || b4) {
+      const T uij = P::rcp(r + sj);
+      const T lg = P::log((b3 ? ri : fabs(r - sj)) * uij);
+      const T invr3 = invr2 * invr;
+      T a;
+      if (b3) {
+        a = -T(0.25) * ((-T(0.5) * (r2 - (ri * ri) + (sj * sj)) * invr3 * iri * iri) +
+                        (invr * uij * (uij - invr)));
+      }
+      else {
+        const T t1 = P::rcp(r2 - (sj * sj));
+        a = sj * t1 * (t1 - (T(0.5) * invr2));
+      }
+      res = a + (T(0.25) * invr3 * lg);
+    }
+  }
+  return res;
+}
+
+// ------------------------------------------------------------------------------------------
+// Work-unit LDS layout.
+// ------------------------------------------------------------------------------------------
+template <typename T> struct WuLds {
+  using crd_t = typename Prec<T>::crd_t;
+  long long *psi, *dedr, *fx, *fy, *fz, *nrg, *org;
+  T *q, *p1, *p2, *rho, *irho, *scr, *rad, *sd;
+  crd_t *cx, *cy, *cz;
+  int *gidx, *slot;
+  WuSys *sys;
+  int2 *lj;        // per system slot: table offset, type count
+
+  static __host__ __device__ size_t bytes(int maxa) {
+    const size_t m = static_cast<size_t>(maxa);
+    return (5 * m * sizeof(long long)) + (kMaxWuSystems * 16 * sizeof(long long)) +
+           (3 * kMaxWuSystems * sizeof(long long)) + (8 * m * sizeof(T)) +
+           (3 * m * sizeof(crd_t)) + (2 * m * sizeof(int)) + (kMaxWuSystems * sizeof(WuSys)) +
+           (kMaxWuSystems * sizeof(int2));
+  }
+
+  __device__ WuLds(unsigned char *base, int maxa) {
+    unsigned char *p = base;
+    auto take = [&](size_t nbytes) {
+      unsigned char *r = p;
+      p += nbytes;
+      return r;
+    };
+    const size_t m = static_cast<size_t>(maxa);
+    psi = reinterpret_cast<long long *>(take(m * sizeof(long long)));
+    dedr = reinterpret_cast<long long *>(take(m * sizeof(long long)));
+    fx = reinterpret_cast<long long *>(take(m * sizeof(long long)));
+    fy = reinterpret_cast<long long *>(take(m * sizeof(long long)));
+    fz = reinterpret_cast<long long *>(take(m * sizeof(long long)));
+    nrg = reinterpret_cast<long long *>(take(kMaxWuSystems * 16 * sizeof(long long)));
+    org = reinterpret_cast<long long *>(take(3 * kMaxWuSystems * sizeof(long long)));
+    q = reinterpret_cast<T *>(take(m * sizeof(T)));
+    p1 = reinterpret_cast<T *>(take(m * sizeof(T)));
+    p2 = reinterpret_cast<T *>(take(m * sizeof(T)));
+    rho = reinterpret_cast<T *>(take(m * sizeof(T)));
+    irho = reinterpret_cast<T *>(take(m * sizeof(T)));
+    scr = reinterpret_cast<T *>(take(m * sizeof(T)));
+    rad = reinterpret_cast<T *>(take(m * sizeof(T)));
+    sd = reinterpret_cast<T *>(take(m * sizeof(T)));
+    cx = reinterpret_cast<crd_t *>(take(m * sizeof(crd_t)));
+    cy = reinterpret_cast<crd_t *>(take(m * sizeof(crd_t)));
+    cz = reinterpret_cast<crd_t *>(take(m * sizeof(crd_t)));
+    gidx = reinterpret_cast<int *>(take(m * sizeof(int)));
+    slot = reinterpret_cast<int *>(take(m * sizeof(int)));
+    sys = reinterpret_cast<WuSys *>(take(kMaxWuSystems * sizeof(WuSys)));
+    lj = reinterpret_cast<int2 *>(take(kMaxWuSystems * sizeof(int2)));
+  }
+};
+
+// Displacement between two LDS coordinates in Angstroms.
+__device__ __forceinline__ float crd_delta(int a, int b, float scale) {
+  return static_cast<float>(a - b) * scale;
+}
+__device__ __forceinline__ double crd_delta(double a, double b, double) { return a - b; }
+
+// Conversion of a fixed-precision global coordinate to the LDS representation.
+template <typename T> struct CrdConv;
+template <> struct CrdConv<float> {
+  static __device__ __forceinline__ int to_lds(long long v, long long origin, int shift) {
+    const long long d = v - origin;
+    const long long half = (shift > 0) ? (1LL << (shift - 1)) : 0LL;
+    return static_cast<int>((d + half) >> shift);
+  }
+  static __device__ __forceinline__ bool in_range(long long v, long long origin, int shift) {
+    const long long d = (v - origin) >> shift;
+    return d > -2147483000LL && d < 2147483000LL;
+  }
+};
+template <> struct CrdConv<double> {
+  static __device__ __forceinline__ double to_lds(long long v, long long origin, int shift) {
+    return static_cast<double>(v - origin) * ldexp(1.0, -shift);
+  }
+  static __device__ __forceinline__ bool in_range(long long, long long, int) { return true; }
+};
+
+}  // namespace stormm_amd

@@ -1,0 +1,830 @@
+// Work-unit evaluation stages (device functions).  A work unit is one thread block holding one
+// or more whole systems in LDS.  The stages run in order, separated by block barriers:
+//   stage_load      global coordinates / parameters -> LDS, accumulators zeroed
+//   stage_psi       Born-radius integrals over 16 x 16 pair tiles (both directions per pair)
+//   stage_radii     effective Born radii, GB self energies, self part of dE/dR (per atom, FP64)
+//   stage_pairs     Coulomb + Lennard-Jones (exclusion masked) and GB pair energy / force / dE/dR
+//   stage_chain     OBC chain rule on dE/dR (per atom, FP64)
+//   stage_deriv     forces from the radius derivatives over the pair tiles
+//   stage_valence   bonds, angles, cosine dihedrals and attenuated 1:4 pairs
+// Forces, psi and dE/dR are accumulated in LDS as 64-bit fixed-precision integers, so the
+// result does not depend on which lanes or work units processed which tiles.
+#pragma once
+
+#include "mm_device.h"
+
+namespace stormm_amd {
+
+template <typename T> struct StageCtx {
+  WuLds<T> L;
+  WuDesc wu;
+  int tid, nthr;
+  int crd_shift;     // right shift from global fixed precision to LDS coordinates
+  T crd_scale;       // LDS coordinate unit in Angstroms (single precision)
+  bool gb;
+  __device__ StageCtx(unsigned char *smem, int maxa) : L(smem, maxa) {}
+};
+
+// ------------------------------------------------------------------------------------------
+// Work-unit set-up: system table, per-atom parameters, and the coordinate origins.
+template <typename T>
+__device__ void stage_load_params(StageCtx<T> &c, const MMKit &k) {
+  WuLds<T> &L = c.L;
+  const WuDesc &wu = c.wu;
+  if (c.tid < wu.sys_count) {
+    const WuSys ws = k.wu_sys[wu.sys_first + c.tid];
+    L.sys[c.tid] = ws;
+    L.lj[c.tid] = make_int2(k.lj_off[ws.sys_index], k.lj_nt[ws.sys_index]);
+  }
+  __syncthreads();
+  for (int a = c.tid; a < wu.lds_atoms; a += c.nthr) {
+    int s = 0;
+    for (int t = 1; t < wu.sys_count; t++) {
+      s = (a >= L.sys[t].lds_base) ? t : s;
+    }
+    const WuSys ws = L.sys[s];
+    const int local = a - ws.lds_base;
+    const bool valid = local < ws.natom;
+    const int g = ws.global_start + local;
+    L.gidx[a] = valid ? g : -1;
+    L.slot[a] = s;
+    if (valid) {
+      L.q[a] = static_cast<T>(k.q[g]);
+      if (c.wu.lj_table) {
+        L.p1[a] = static_cast<T>(k.lj_type[g]);
+        L.p2[a] = T(0);
+      }
+      else {
+        L.p1[a] = static_cast<T>(k.ljp1[g]);
+        L.p2[a] = static_cast<T>(k.ljp2[g]);
+      }
+      const double rho = k.rho[g];
+      L.rho[a] = static_cast<T>(rho);
+      L.irho[a] = (rho > 0.0) ? static_cast<T>(1.0 / rho) : T(0);
+      L.scr[a] = static_cast<T>(k.scr[g]);
+    }
+    else {
+      L.q[a] = T(0);
+      L.p1[a] = T(0);
+      L.p2[a] = T(0);
+      L.rho[a] = T(1);
+      L.irho[a] = T(1);
+      L.scr[a] = T(0);
+    }
+  }
+  __syncthreads();
+}
+
+// Coordinates into LDS (relative to a per-system origin) and all accumulators zeroed.
+template <typename T>
+__device__ void stage_load_coords(StageCtx<T> &c, const MMKit &k) {
+  WuLds<T> &L = c.L;
+  const WuDesc &wu = c.wu;
+  if (c.tid < wu.sys_count) {
+    const int gs = L.sys[c.tid].global_start;
+    const long long mask = ~((1LL << c.crd_shift) - 1LL);
+    L.org[3 * c.tid + 0] = k.xcrd[gs] & mask;
+    L.org[3 * c.tid + 1] = k.ycrd[gs] & mask;
+    L.org[3 * c.tid + 2] = k.zcrd[gs] & mask;
+  }
+  for (int e = c.tid; e < wu.sys_count * 16; e += c.nthr) {
+    L.nrg[e] = 0;
+  }
+  __syncthreads();
+  for (int a = c.tid; a < wu.lds_atoms; a += c.nthr) {
+    const int g = L.gidx[a];
+    const int s = L.slot[a];
+    L.psi[a] = 0;
+    L.dedr[a] = 0;
+    L.fx[a] = 0;
+    L.fy[a] = 0;
+    L.fz[a] = 0;
+    L.rad[a] = T(1);
+    L.sd[a] = T(0);
+    if (g >= 0) {
+      const long long gx = k.xcrd[g], gy = k.ycrd[g], gz = k.zcrd[g];
+      const long long ox = L.org[3 * s], oy = L.org[3 * s + 1], oz = L.org[3 * s + 2];
+      if (!(CrdConv<T>::in_range(gx, ox, c.crd_shift) &&
+            CrdConv<T>::in_range(gy, oy, c.crd_shift) &&
+            CrdConv<T>::in_range(gz, oz, c.crd_shift))) {
+        atomicOr(k.status, static_cast<int>(kStatusCoordinateRange));
+      }
+      L.cx[a] = CrdConv<T>::to_lds(gx, ox, c.crd_shift);
+      L.cy[a] = CrdConv<T>::to_lds(gy, oy, c.crd_shift);
+      L.cz[a] = CrdConv<T>::to_lds(gz, oz, c.crd_shift);
+    }
+    else {
+      L.cx[a] = 0;
+      L.cy[a] = 0;
+      L.cz[a] = 0;
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ void init_ctx(StageCtx<T> &c, const MMKit &k, int wu_index, int terms) {
+  c.wu = k.wu[wu_index];
+  c.tid = threadIdx.x;
+  c.nthr = blockDim.x;
+  if (sizeof(T) == 4) {
+    c.crd_shift = (k.gpos_bits > kLdsCrdBits) ? (k.gpos_bits - kLdsCrdBits) : 0;
+    c.crd_scale = T(ldexp(1.0, -(k.gpos_bits - c.crd_shift)));
+  }
+  else {
+    c.crd_shift = k.gpos_bits;
+    c.crd_scale = T(1);
+  }
+  c.gb = (k.igb != 0) && ((terms & kTermNonbonded) != 0);
+}
+
+// Pair-validity inside a tile task at rotation step kstep for row lane il.
+__device__ __forceinline__ bool pair_valid(bool ivalid, int jl, int nj, bool diag, int kstep,
+                                           int il) {
+  const bool dok = (kstep >= 1) && ((kstep < 8) || ((kstep == 8) && (il < 8)));
+  return ivalid && (jl < nj) && (diag ? dok : true);
+}
+
+// Number of rotation steps the wave runs for its current tasks: 16 when any of its rows holds
+// an off-diagonal tile, 9 when all hold diagonal tiles.
+__device__ __forceinline__ int wave_steps(bool diag) {
+  const unsigned long long offd = __ballot(!diag);
+  return (offd != 0ULL) ? 16 : 9;
+}
+
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__device__ void stage_psi(StageCtx<T> &c, const MMKit &k) {
+  using P = Prec<T>;
+  WuLds<T> &L = c.L;
+  const int row = c.tid >> 4;
+  const int nrows = c.nthr >> 4;
+  const int il = c.tid & 15;
+  const double psi_scale = ldexp(1.0, kPsiBits);
+  for (int t = row; t < c.wu.task_count; t += nrows) {
+    const Task tk = k.tasks[c.wu.task_first + t];
+    const int ib = tk.x & 0xffff, jb = tk.x >> 16;
+    const bool diag = (tk.y >> 8) & 1;
+    const int ni = (tk.y >> 12) & 0xff, nj = (tk.y >> 20) & 0xff;
+    const bool ivalid = il < ni;
+    const int i = ib + il;
+    const int j0 = jb + il;
+    const auto xi = L.cx[i], yi = L.cy[i], zi = L.cz[i];
+    const T ri = L.rho[i], iri = L.irho[i], si = L.scr[i];
+    auto xj = L.cx[j0], yj = L.cy[j0], zj = L.cz[j0];
+    T rj = L.rho[j0], irj = L.irho[j0], sj = L.scr[j0];
+    // The Born-radius integral is summed in double precision.
+    double acci = 0.0, accj = 0.0;
+    const int nsteps = wave_steps(diag);
+    for (int ks = 0; ks < nsteps; ks++) {
+      const int jl = (il - ks) & 15;
+      if (pair_valid(ivalid, jl, nj, diag, ks, il)) {
+        const T dx = crd_delta(xj, xi, c.crd_scale);
+        const T dy = crd_delta(yj, yi, c.crd_scale);
+        const T dz = crd_delta(zj, zi, c.crd_scale);
+        const T r2 = (dx * dx) + (dy * dy) + (dz * dz);
+        const T invr = P::rsqrt(r2);
+        const T r = r2 * invr;
+        acci += static_cast<double>(psi_term(r, r2, invr, ri, iri, sj));
+        accj += static_cast<double>(psi_term(r, r2, invr, rj, irj, si));
+      }
+      xj = row_rotate(xj);
+      yj = row_rotate(yj);
+      zj = row_rotate(zj);
+      rj = row_rotate(rj);
+      irj = row_rotate(irj);
+      sj = row_rotate(sj);
+      accj = row_rotate(accj);
+    }
+    if (ivalid) {
+      lds_add(&L.psi[i], Prec<double>::fixed(acci * psi_scale));
+    }
+    const int jl_home = (il - nsteps) & 15;
+    if (jl_home < nj) {
+      lds_add(&L.psi[jb + jl_home], Prec<double>::fixed(accj * psi_scale));
+    }
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------
+// Effective radii (src/Potential/nonbonded_potential.tpp:731-781), self energies and the self
+// part of the radius derivative accumulator (:784-806), all in double precision.
+template <typename T, bool FORCES>
+__device__ void stage_radii(StageCtx<T> &c, const MMKit &k, double coulomb_inv_scaled) {
+  WuLds<T> &L = c.L;
+  const double nrg_scale = ldexp(1.0, k.nrg_bits);
+  const double inv_diel = 1.0 / k.dielectric;
+  for (int a = c.tid; a < c.wu.lds_atoms; a += c.nthr) {
+    const int g = L.gidx[a];
+    if (g < 0) {
+      continue;
+    }
+    const double psi = static_cast<double>(L.psi[a]) * ldexp(1.0, -kPsiBits);
+    const double rho = k.rho[g];
+    const double pbr = k.pbr[g];
+    double R;
+    if (k.igb == 1) {
+      R = 1.0 / ((1.0 / rho) + psi);
+      R = (R < 0.0) ? 30.0 : R;
+    }
+    else {
+      const double fipsi = psi * (-rho);
+      R = 1.0 / ((1.0 / rho) - tanh((k.gb_alpha[g] - (k.gb_beta[g] * fipsi) +
+                                     (k.gb_gamma[g] * fipsi * fipsi)) * fipsi) / pbr);
+    }
+    L.rad[a] = static_cast<T>(R);
+    // q holds charge * sqrt(Coulomb), so q^2 = Coulomb * charge^2.
+    const double qg = k.q[g];
+    const double expmkf = exp(-kGbKscale * k.kappa * R) * inv_diel;
+    const double dielfac = 1.0 - expmkf;
+    const double atmq2h = 0.5 * qg * qg;
+    const double atmqd2h = atmq2h * dielfac;
+    const double contrib = -atmqd2h / R;
+    lds_add(&L.nrg[L.slot[a] * 16 + kGb], static_cast<long long>(__builtin_rint(contrib *
+                                                                                  nrg_scale)));
+    if (FORCES) {
+      const double sdi = atmqd2h - (kGbKscale * k.kappa * atmq2h * expmkf * R);
+      L.dedr[a] = static_cast<long long>(__builtin_rint(sdi * ldexp(1.0, kDedrBits)));
+    }
+  }
+  (void)coulomb_inv_scaled;
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------
+// Coulomb + Lennard-Jones (src/Potential/nonbonded_potential.tpp:174-231,312-373) and the GB
+// pair energy, force and radius derivative (:811-972).
+template <typename T, bool FORCES, bool LJTABLE>
+__device__ void stage_pairs(StageCtx<T> &c, const MMKit &k, bool do_nb) {
+  using P = Prec<T>;
+  WuLds<T> &L = c.L;
+  const int row = c.tid >> 4;
+  const int nrows = c.nthr >> 4;
+  const int il = c.tid & 15;
+  const T frc_scale = T(ldexp(1.0, k.frc_bits));
+  const T dedr_scale = T(ldexp(1.0, kDedrBits));
+  const double nrg_scale = ldexp(1.0, k.nrg_bits);
+  const T inv_diel = T(1.0 / k.dielectric);
+  const T kap = T(k.kappa * kGbKscale);
+  const bool use_kappa = (k.kappa != 0.0);
+  const bool gb = c.gb;
+  for (int t = row; t < c.wu.task_count; t += nrows) {
+    const Task tk = k.tasks[c.wu.task_first + t];
+    const int ib = tk.x & 0xffff, jb = tk.x >> 16;
+    const int slot = tk.y & 0xff;
+    const bool diag = (tk.y >> 8) & 1;
+    const int ni = (tk.y >> 12) & 0xff, nj = (tk.y >> 20) & 0xff;
+    const bool ivalid = il < ni;
+    const int i = ib + il;
+    const int j0 = jb + il;
+    const unsigned emask = (tk.z >= 0 && ivalid) ? k.masks[tk.z + il] : 0u;
+    const auto xi = L.cx[i], yi = L.cy[i], zi = L.cz[i];
+    const T qi = L.q[i], p1i = L.p1[i], p2i = L.p2[i], Ri = L.rad[i];
+    auto xj = L.cx[j0], yj = L.cy[j0], zj = L.cz[j0];
+    T qj = L.q[j0], p1j = L.p1[j0], p2j = L.p2[j0], Rj = L.rad[j0];
+    T fxi = T(0), fyi = T(0), fzi = T(0), dri = T(0);
+    T fxj = T(0), fyj = T(0), fzj = T(0), drj = T(0);
+    // Energies are summed in double precision (one conversion and add per term and pair).
+    double e_qq = 0.0, e_lj = 0.0, e_gb = 0.0;
+    int lj_off = 0, lj_nt = 0;
+    if (LJTABLE) {
+      lj_off = L.lj[slot].x;
+      lj_nt = L.lj[slot].y;
+    }
+    const int nsteps = wave_steps(diag);
+    for (int ks = 0; ks < nsteps; ks++) {
+      const int jl = (il - ks) & 15;
+      if (pair_valid(ivalid, jl, nj, diag, ks, il)) {
+        const T dx = crd_delta(xj, xi, c.crd_scale);
+        const T dy = crd_delta(yj, yi, c.crd_scale);
+        const T dz = crd_delta(zj, zi, c.crd_scale);
+        const T r2 = (dx * dx) + (dy * dy) + (dz * dz);
+        const T qiqj = qi * qj;
+        T fmag = T(0);
+        if (gb) {
+          const T rr = Ri * Rj;
+          const T efac = P::exp(-T(0.25) * r2 * P::rcp(rr));
+          const T fgb2 = r2 + (rr * efac);
+          const T fgbi = P::rsqrt(fgb2);
+          T dielfac, temp6;
+          if (use_kappa) {
+            const T fgbk = -kap * fgb2 * fgbi;
+            const T expmkf = P::exp(fgbk) * inv_diel;
+            dielfac = T(1) - expmkf;
+            temp6 = qiqj * fgbi * fgbi * fgbi * (dielfac + (fgbk * expmkf));
+          }
+          else {
+            dielfac = T(1) - inv_diel;
+            temp6 = qiqj * fgbi * fgbi * fgbi * dielfac;
+          }
+          e_gb -= static_cast<double>(qiqj * dielfac * fgbi);
+          if (FORCES) {
+            fmag += temp6 * (T(1) - (T(0.25) * efac));
+            const T temp5 = T(0.5) * efac * temp6 * (rr + (T(0.25) * r2));
+            dri += Ri * temp5;
+            drj += Rj * temp5;
+          }
+        }
+        if (do_nb && !((emask >> jl) & 1u)) {
+          const T invr = P::rsqrt(r2);
+          const T invr2 = invr * invr;
+          e_qq += static_cast<double>(qiqj * invr);
+          if (LJTABLE) {
+            const double2 ab = k.ljab[lj_off + (static_cast<int>(p1i) * lj_nt) +
+                                      static_cast<int>(p1j)];
+            const T lja = static_cast<T>(ab.x), ljb = static_cast<T>(ab.y);
+            const T invr4 = invr2 * invr2;
+            e_lj += static_cast<double>(((lja * invr4 * invr4) - (ljb * invr2)) * invr4);
+            if (FORCES) {
+              fmag += ((T(6) * ljb) - (T(12) * lja * invr4 * invr2)) * invr4 * invr4;
+            }
+          }
+          else {
+            const T sig = T(0.5) * (p1i + p1j);
+            const T s2 = sig * sig * invr2;
+            const T s6 = s2 * s2 * s2;
+            const T e4 = p2i * p2j;
+            e_lj += static_cast<double>(e4 * ((s6 * s6) - s6));
+            if (FORCES) {
+              fmag += e4 * ((T(6) * s6) - (T(12) * s6 * s6)) * invr2;
+            }
+          }
+          if (FORCES) {
+            fmag -= qiqj * invr * invr2;
+          }
+        }
+        if (FORCES) {
+          fxi += fmag * dx;
+          fyi += fmag * dy;
+          fzi += fmag * dz;
+          fxj -= fmag * dx;
+          fyj -= fmag * dy;
+          fzj -= fmag * dz;
+        }
+      }
+      xj = row_rotate(xj);
+      yj = row_rotate(yj);
+      zj = row_rotate(zj);
+      qj = row_rotate(qj);
+      p1j = row_rotate(p1j);
+      p2j = row_rotate(p2j);
+      Rj = row_rotate(Rj);
+      if (FORCES) {
+        fxj = row_rotate(fxj);
+        fyj = row_rotate(fyj);
+        fzj = row_rotate(fzj);
+        drj = row_rotate(drj);
+      }
+    }
+    if (FORCES) {
+      if (ivalid) {
+        lds_add(&L.fx[i], P::fixed(fxi * frc_scale));
+        lds_add(&L.fy[i], P::fixed(fyi * frc_scale));
+        lds_add(&L.fz[i], P::fixed(fzi * frc_scale));
+        if (gb) {
+          lds_add(&L.dedr[i], P::fixed(dri * dedr_scale));
+        }
+      }
+      const int jl_home = (il - nsteps) & 15;
+      if (jl_home < nj) {
+        const int j = jb + jl_home;
+        lds_add(&L.fx[j], P::fixed(fxj * frc_scale));
+        lds_add(&L.fy[j], P::fixed(fyj * frc_scale));
+        lds_add(&L.fz[j], P::fixed(fzj * frc_scale));
+        if (gb) {
+          lds_add(&L.dedr[j], P::fixed(drj * dedr_scale));
+        }
+      }
+    }
+    const long long vqq = row_sum(Prec<double>::fixed(e_qq * nrg_scale));
+    const long long vlj = row_sum(Prec<double>::fixed(e_lj * nrg_scale));
+    const long long vgb = row_sum(Prec<double>::fixed(e_gb * nrg_scale));
+    if (il == 0) {
+      if (do_nb) {
+        lds_add(&L.nrg[slot * 16 + kElec], vqq);
+        lds_add(&L.nrg[slot * 16 + kVdw], vlj);
+      }
+      if (gb) {
+        lds_add(&L.nrg[slot * 16 + kGb], vgb);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------
+// OBC chain rule on the accumulated radius derivatives, src/Potential/nonbonded_potential.tpp:
+// 985-1017.  The HCT model uses the accumulated values as they stand.
+template <typename T>
+__device__ void stage_chain(StageCtx<T> &c, const MMKit &k) {
+  WuLds<T> &L = c.L;
+  for (int a = c.tid; a < c.wu.lds_atoms; a += c.nthr) {
+    const int g = L.gidx[a];
+    if (g < 0) {
+      L.sd[a] = T(0);
+      continue;
+    }
+    double sdi = static_cast<double>(L.dedr[a]) * ldexp(1.0, -kDedrBits);
+    if (k.igb != 1) {
+      const double psi = static_cast<double>(L.psi[a]) * ldexp(1.0, -kPsiBits);
+      const double rho = k.rho[g];
+      const double fipsi = psi * (-rho);
+      const double al = k.gb_alpha[g], be = k.gb_beta[g], ga = k.gb_gamma[g];
+      const double thi = tanh((al - (be - (ga * fipsi)) * fipsi) * fipsi);
+      sdi *= (al - ((2.0 * be) - (3.0 * ga * fipsi)) * fipsi) * (1.0 - thi * thi) * rho /
+             k.pbr[g];
+    }
+    L.sd[a] = static_cast<T>(sdi);
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------
+// Forces from the Born-radius derivatives, src/Potential/nonbonded_potential.tpp:1018-1292.
+template <typename T>
+__device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
+  using P = Prec<T>;
+  WuLds<T> &L = c.L;
+  const int row = c.tid >> 4;
+  const int nrows = c.nthr >> 4;
+  const int il = c.tid & 15;
+  const T frc_scale = T(ldexp(1.0, k.frc_bits));
+  for (int t = row; t < c.wu.task_count; t += nrows) {
+    const Task tk = k.tasks[c.wu.task_first + t];
+    const int ib = tk.x & 0xffff, jb = tk.x >> 16;
+    const bool diag = (tk.y >> 8) & 1;
+    const int ni = (tk.y >> 12) & 0xff, nj = (tk.y >> 20) & 0xff;
+    const bool ivalid = il < ni;
+    const int i = ib + il;
+    const int j0 = jb + il;
+    const auto xi = L.cx[i], yi = L.cy[i], zi = L.cz[i];
+    const T ri = L.rho[i], iri = L.irho[i], si = L.scr[i], sdi = L.sd[i];
+    auto xj = L.cx[j0], yj = L.cy[j0], zj = L.cz[j0];
+    T rj = L.rho[j0], irj = L.irho[j0], sj = L.scr[j0], sdj = L.sd[j0];
+    T fxi = T(0), fyi = T(0), fzi = T(0);
+    T fxj = T(0), fyj = T(0), fzj = T(0);
+    const int nsteps = wave_steps(diag);
+    for (int ks = 0; ks < nsteps; ks++) {
+      const int jl = (il - ks) & 15;
+      if (pair_valid(ivalid, jl, nj, diag, ks, il)) {
+        const T dx = crd_delta(xj, xi, c.crd_scale);
+        const T dy = crd_delta(yj, yi, c.crd_scale);
+        const T dz = crd_delta(zj, zi, c.crd_scale);
+        const T r2 = (dx * dx) + (dy * dy) + (dz * dz);
+        const T invr = P::rsqrt(r2);
+        const T invr2 = invr * invr;
+        const T r = r2 * invr;
+        const T datmpi = dpsi_term(r, r2, invr, invr2, ri, iri, sj);
+        const T datmpj = dpsi_term(r, r2, invr, invr2, rj, irj, si);
+        const T fmag = (datmpi * sdi) + (datmpj * sdj);
+        fxi -= fmag * dx;
+        fyi -= fmag * dy;
+        fzi -= fmag * dz;
+        fxj += fmag * dx;
+        fyj += fmag * dy;
+        fzj += fmag * dz;
+      }
+      xj = row_rotate(xj);
+      yj = row_rotate(yj);
+      zj = row_rotate(zj);
+      rj = row_rotate(rj);
+      irj = row_rotate(irj);
+      sj = row_rotate(sj);
+      sdj = row_rotate(sdj);
+      fxj = row_rotate(fxj);
+      fyj = row_rotate(fyj);
+      fzj = row_rotate(fzj);
+    }
+    if (ivalid) {
+      lds_add(&L.fx[i], P::fixed(fxi * frc_scale));
+      lds_add(&L.fy[i], P::fixed(fyi * frc_scale));
+      lds_add(&L.fz[i], P::fixed(fzi * frc_scale));
+    }
+    const int jl_home = (il - nsteps) & 15;
+    if (jl_home < nj) {
+      const int j = jb + jl_home;
+      lds_add(&L.fx[j], P::fixed(fxj * frc_scale));
+      lds_add(&L.fy[j], P::fixed(fyj * frc_scale));
+      lds_add(&L.fz[j], P::fixed(fzj * frc_scale));
+    }
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------
+// Valence terms.  They are O(N) work, so both precision models evaluate them in double
+// precision from the full fixed-precision global coordinates: a single-precision r - r0 or
+// theta - theta0 near equilibrium loses most of its significant bits.
+// ------------------------------------------------------------------------------------------
+struct Vec3 {
+  double x, y, z;
+};
+__device__ __forceinline__ Vec3 cross(const Vec3 &a, const Vec3 &b) {
+  return {(a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x)};
+}
+__device__ __forceinline__ double dot(const Vec3 &a, const Vec3 &b) {
+  return (a.x * b.x) + (a.y * b.y) + (a.z * b.z);
+}
+
+struct ValenceCtx {
+  const long long *xc, *yc, *zc;
+  const int *gidx;
+  double inv_gpos;
+  long long *fx, *fy, *fz;
+  __device__ __forceinline__ Vec3 disp(int from, int to) const {
+    const int gf = gidx[from], gt = gidx[to];
+    return {static_cast<double>(xc[gt] - xc[gf]) * inv_gpos,
+            static_cast<double>(yc[gt] - yc[gf]) * inv_gpos,
+            static_cast<double>(zc[gt] - zc[gf]) * inv_gpos};
+  }
+  __device__ __forceinline__ void add(int a, long long vx, long long vy, long long vz) const {
+    lds_add(&fx[a], vx);
+    lds_add(&fy[a], vy);
+    lds_add(&fz[a], vz);
+  }
+};
+
+__device__ __forceinline__ long long fix64(double v) {
+  return static_cast<long long>(__builtin_rint(v));
+}
+
+// Harmonic bond, src/Potential/valence_potential.tpp:9-96.
+template <bool FORCES>
+__device__ __forceinline__ double eval_bond(const ValenceCtx &v, int i, int j, double kb,
+                                            double r0, double fscale) {
+  const Vec3 d = v.disp(i, j);
+  const double dr = sqrt(dot(d, d));
+  const double dl = dr - r0;
+  if (FORCES) {
+    const double fmag = 2.0 * kb * dl / dr;
+    const long long ix = fix64(fmag * d.x * fscale);
+    const long long iy = fix64(fmag * d.y * fscale);
+    const long long iz = fix64(fmag * d.z * fscale);
+    v.add(i, ix, iy, iz);
+    v.add(j, -ix, -iy, -iz);
+  }
+  return kb * dl * dl;
+}
+
+// Harmonic angle, src/Potential/valence_potential.tpp:170-287.
+template <bool FORCES>
+__device__ __forceinline__ double eval_angle(const ValenceCtx &v, int i, int j, int kk,
+                                             double ka, double th0, double fscale) {
+  const Vec3 ba = v.disp(j, i);
+  const Vec3 bc = v.disp(j, kk);
+  const double mgba = dot(ba, ba);
+  const double mgbc = dot(bc, bc);
+  const double invbabc = 1.0 / sqrt(mgba * mgbc);
+  double costheta = dot(ba, bc) * invbabc;
+  costheta = (costheta < -1.0) ? -1.0 : ((costheta > 1.0) ? 1.0 : costheta);
+  const double theta = acos(costheta);
+  const double dtheta = theta - th0;
+  if (FORCES) {
+    const double dA = -2.0 * ka * dtheta / sqrt(1.0 - (costheta * costheta));
+    const double sqba = dA / mgba;
+    const double sqbc = dA / mgbc;
+    const double mbabc = dA * invbabc;
+    const long long iax = fix64(((bc.x * mbabc) - (costheta * ba.x * sqba)) * fscale);
+    const long long iay = fix64(((bc.y * mbabc) - (costheta * ba.y * sqba)) * fscale);
+    const long long iaz = fix64(((bc.z * mbabc) - (costheta * ba.z * sqba)) * fscale);
+    const long long icx = fix64(((ba.x * mbabc) - (costheta * bc.x * sqbc)) * fscale);
+    const long long icy = fix64(((ba.y * mbabc) - (costheta * bc.y * sqbc)) * fscale);
+    const long long icz = fix64(((ba.z * mbabc) - (costheta * bc.z * sqbc)) * fscale);
+    v.add(i, -iax, -iay, -iaz);
+    v.add(j, iax + icx, iay + icy, iaz + icz);
+    v.add(kk, -icx, -icy, -icz);
+  }
+  return ka * dtheta * dtheta;
+}
+
+// Dihedral angle from the two plane normals: the arccosine, with the reference's fallback for
+// (anti-)parallel normals (src/Math/rounding.tpp:24-59; the double-precision threshold is 1.0,
+// the reference's float-typed constant), signed by the normals' cross product along the
+// central bond.
+__device__ __forceinline__ double dihedral_angle(double costheta, const Vec3 &crabbc,
+                                                 const Vec3 &crbccd, const Vec3 &bc,
+                                                 const Vec3 &scr) {
+  const double sgn = dot(scr, bc);
+  if (fabs(costheta) >= 1.0) {
+    const double m1 = 1.0 / sqrt(dot(crabbc, crabbc));
+    const double m2 = 1.0 / sqrt(dot(crbccd, crbccd));
+    Vec3 rd = {(crbccd.x * m2) - (crabbc.x * m1), (crbccd.y * m2) - (crabbc.y * m1),
+               (crbccd.z * m2) - (crabbc.z * m1)};
+    double rs = sqrt(dot(rd, rd));
+    if (fabs(rs) > 1.0) {
+      rd = {(crbccd.x * m2) + (crabbc.x * m1), (crbccd.y * m2) + (crabbc.y * m1),
+            (crbccd.z * m2) + (crabbc.z * m1)};
+      rs = 3.14159265358979323846 - sqrt(dot(rd, rd));
+    }
+    return (sgn > 0.0) ? rs : -rs;
+  }
+  const double th = acos(costheta);
+  return (sgn > 0.0) ? th : -th;
+}
+
+// Cosine dihedral, src/Potential/valence_potential.tpp:359-578.
+template <bool FORCES>
+__device__ __forceinline__ double eval_dihedral(const ValenceCtx &v, int i, int j, int kk, int l,
+                                                double amp, double phase, double freq,
+                                                double fscale) {
+  const Vec3 ab = v.disp(i, j);
+  const Vec3 bc = v.disp(j, kk);
+  const Vec3 cd = v.disp(kk, l);
+  Vec3 crabbc = cross(ab, bc);
+  Vec3 crbccd = cross(bc, cd);
+  double costheta = dot(crabbc, crbccd) / sqrt(dot(crabbc, crabbc) * dot(crbccd, crbccd));
+  const Vec3 scr = cross(crabbc, crbccd);
+  costheta = (costheta < -1.0) ? -1.0 : ((costheta > 1.0) ? 1.0 : costheta);
+  const double theta = dihedral_angle(costheta, crabbc, crbccd, bc, scr);
+  const double sangle = (freq * theta) - phase;
+  if (FORCES) {
+    const double fr = amp * freq * sin(sangle);
+    const double mgab = sqrt(dot(ab, ab));
+    const double mgbc = sqrt(dot(bc, bc));
+    const double mgcd = sqrt(dot(cd, cd));
+    const double invab = 1.0 / mgab;
+    const double invbc = 1.0 / mgbc;
+    const double invcd = 1.0 / mgcd;
+    const double cosb = -dot(ab, bc) * invab * invbc;
+    const double cosc = -dot(bc, cd) * invbc * invcd;
+    // src/Constants/symbol_values.h:72-82 (asymptotic_to_one_lf, inverse_one_minus_asymptote)
+    const double isinb2 = (cosb * cosb < 0.99999904632568359375) ? fr / (1.0 - (cosb * cosb)) :
+                                                                   fr * 1048576.0;
+    const double isinc2 = (cosc * cosc < 0.99999904632568359375) ? fr / (1.0 - (cosc * cosc)) :
+                                                                   fr * 1048576.0;
+    const double invabc = invab * invbc;
+    const double invbcd = invbc * invcd;
+    crabbc = {crabbc.x * invabc, crabbc.y * invabc, crabbc.z * invabc};
+    crbccd = {crbccd.x * invbcd, crbccd.y * invbcd, crbccd.z * invbcd};
+    const double fa = -invab * isinb2;
+    const double fb1 = (mgbc - (mgab * cosb)) * invabc * isinb2;
+    const double fb2 = cosc * invbc * isinc2;
+    const double fd = -invcd * isinc2;
+    const long long fix = fix64(crabbc.x * fa * fscale);
+    const long long fjx = fix64(((fb1 * crabbc.x) - (fb2 * crbccd.x)) * fscale);
+    const long long flx = fix64(-fd * crbccd.x * fscale);
+    const long long fiy = fix64(crabbc.y * fa * fscale);
+    const long long fjy = fix64(((fb1 * crabbc.y) - (fb2 * crbccd.y)) * fscale);
+    const long long fly = fix64(-fd * crbccd.y * fscale);
+    const long long fiz = fix64(crabbc.z * fa * fscale);
+    const long long fjz = fix64(((fb1 * crabbc.z) - (fb2 * crbccd.z)) * fscale);
+    const long long flz = fix64(-fd * crbccd.z * fscale);
+    v.add(i, fix, fiy, fiz);
+    v.add(j, fjx, fjy, fjz);
+    v.add(kk, -(fix + fjx + flx), -(fiy + fjy + fly), -(fiz + fjz + flz));
+    v.add(l, flx, fly, flz);
+  }
+  return amp * (1.0 + cos(sangle));
+}
+
+// Attenuated 1:4 pair, src/Potential/valence_potential.tpp:1288-1419.
+template <bool FORCES, bool LJTABLE>
+__device__ __forceinline__ void eval_14(const ValenceCtx &v, const MMKit &k, int2 lj, int i,
+                                        int l, double inv_scee, double inv_scnb, double fscale,
+                                        double &ele, double &vdw) {
+  const int gi = v.gidx[i], gl = v.gidx[l];
+  const Vec3 d = v.disp(i, l);
+  const double r2 = dot(d, d);
+  const double invr2 = 1.0 / r2;
+  const double invr = sqrt(invr2);
+  const double qiqj = k.q[gi] * k.q[gl] * inv_scee;
+  ele = qiqj * invr;
+  double fmag = 0.0;
+  if (LJTABLE) {
+    const double2 ab = k.ljab14[lj.x + (k.lj_type[gi] * lj.y) + k.lj_type[gl]];
+    const double lja = ab.x * inv_scnb;
+    const double ljb = ab.y * inv_scnb;
+    const double invr4 = invr2 * invr2;
+    vdw = (lja * invr4 * invr4 * invr4) - (ljb * invr4 * invr2);
+    if (FORCES) {
+      fmag = ((6.0 * ljb) - (12.0 * lja * invr4 * invr2)) * invr4 * invr4;
+    }
+  }
+  else {
+    const double sig = 0.5 * (k.ljp1[gi] + k.ljp1[gl]);
+    const double s2 = sig * sig * invr2;
+    const double s6 = s2 * s2 * s2;
+    const double e4 = k.ljp2[gi] * k.ljp2[gl] * inv_scnb;
+    vdw = e4 * ((s6 * s6) - s6);
+    if (FORCES) {
+      fmag = e4 * ((6.0 * s6) - (12.0 * s6 * s6)) * invr2;
+    }
+  }
+  if (FORCES) {
+    fmag -= qiqj * invr * invr2;
+    const long long ix = fix64(fmag * d.x * fscale);
+    const long long iy = fix64(fmag * d.y * fscale);
+    const long long iz = fix64(fmag * d.z * fscale);
+    v.add(i, ix, iy, iz);
+    v.add(l, -ix, -iy, -iz);
+  }
+}
+
+template <typename T, bool FORCES, bool LJTABLE>
+__device__ void stage_valence(StageCtx<T> &c, const MMKit &k) {
+  const double fscale = ldexp(1.0, k.frc_bits);
+  const double nrg_scale = ldexp(1.0, k.nrg_bits);
+  ValenceCtx v;
+  v.xc = k.xcrd;
+  v.yc = k.ycrd;
+  v.zc = k.zcrd;
+  v.gidx = c.L.gidx;
+  v.inv_gpos = ldexp(1.0, -k.gpos_bits);
+  v.fx = c.L.fx;
+  v.fy = c.L.fy;
+  v.fz = c.L.fz;
+  for (int t = c.tid; t < c.wu.term_count; t += c.nthr) {
+    const int4 vt = k.vterms[c.wu.term_first + t];
+    const int kind = vt.x & 0xf;
+    const int slot = (vt.x >> 8) & 0xff;
+    const int i = vt.y & 0xffff, j = vt.y >> 16;
+    const int kk = vt.z & 0xffff, l = vt.z >> 16;
+    long long *nrg = &c.L.nrg[slot * 16];
+    if (kind == kVBond) {
+      const double2 p = k.bond_par[vt.w];
+      lds_add(&nrg[kBond], fix64(eval_bond<FORCES>(v, i, j, p.x, p.y, fscale) * nrg_scale));
+    }
+    else if (kind == kVAngle) {
+      const double2 p = k.angl_par[vt.w];
+      lds_add(&nrg[kAngle],
+              fix64(eval_angle<FORCES>(v, i, j, kk, p.x, p.y, fscale) * nrg_scale));
+    }
+    else {
+      const double4 p = k.dihe_par[vt.w];
+      const double e = eval_dihedral<FORCES>(v, i, j, kk, l, p.x, p.y, p.z, fscale);
+      lds_add(&nrg[(p.w != 0.0) ? kImproperDihedral : kProperDihedral], fix64(e * nrg_scale));
+      const double2 a14 = k.dihe14[vt.w];
+      if (a14.x > 0.0) {
+        double ele, vdw;
+        eval_14<FORCES, LJTABLE>(v, k, c.L.lj[slot], i, l, a14.x, a14.y, fscale, ele, vdw);
+        lds_add(&nrg[kElec14], fix64(ele * nrg_scale));
+        lds_add(&nrg[kVdw14], fix64(vdw * nrg_scale));
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------
+// One evaluation of the selected terms on the coordinates currently in global memory, leaving
+// forces (FORCES) and energies in the LDS accumulators.
+template <typename T, bool FORCES>
+__device__ void evaluate_wu(StageCtx<T> &c, const MMKit &k, int terms) {
+  const bool lj_table = (c.wu.lj_table != 0);
+  stage_load_coords<T>(c, k);
+  if (c.gb) {
+    stage_psi<T>(c, k);
+    stage_radii<T, FORCES>(c, k, 0.0);
+  }
+  if (terms & kTermNonbonded) {
+    if (lj_table) {
+      stage_pairs<T, FORCES, true>(c, k, true);
+    }
+    else {
+      stage_pairs<T, FORCES, false>(c, k, true);
+    }
+  }
+  if (FORCES && c.gb) {
+    stage_chain<T>(c, k);
+    stage_deriv<T>(c, k);
+  }
+  if (terms & kTermValence) {
+    if (lj_table) {
+      stage_valence<T, FORCES, true>(c, k);
+    }
+    else {
+      stage_valence<T, FORCES, false>(c, k);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Results out: forces added to the global fixed-precision arrays (one owning block per atom)
+// and energies added to the ScoreCard rows of the block's systems.
+template <typename T, bool FORCES>
+__device__ void stage_store(StageCtx<T> &c, const MMKit &k, bool eval_energy) {
+  WuLds<T> &L = c.L;
+  if (FORCES) {
+    for (int a = c.tid; a < c.wu.lds_atoms; a += c.nthr) {
+      const int g = L.gidx[a];
+      if (g >= 0) {
+        k.xfrc[g] += L.fx[a];
+        k.yfrc[g] += L.fy[a];
+        k.zfrc[g] += L.fz[a];
+      }
+    }
+  }
+  if (eval_energy) {
+    for (int e = c.tid; e < c.wu.sys_count * kNrgSlots; e += c.nthr) {
+      const int s = e / kNrgSlots;
+      const int comp = e - (s * kNrgSlots);
+      const long long v = L.nrg[s * 16 + comp];
+      if (v != 0) {
+        k.scorecard[(static_cast<size_t>(L.sys[s].sys_index) * kScoreStride) + comp] += v;
+      }
+    }
+  }
+}
+
+}  // namespace stormm_amd

@@ -1,0 +1,108 @@
+// Fused batched energy / force evaluation kernels and their launchers.
+#include "mm_eval.h"
+#include "kernels.h"
+
+namespace stormm_amd {
+
+template <typename T, bool FORCES>
+__global__ void __launch_bounds__(kEvalBlock) k_mm_eval(MMKit k, int terms, int eval_energy) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  StageCtx<T> c(smem, k.maxa);
+  init_ctx<T>(c, k, blockIdx.x, terms);
+  stage_load_params<T>(c, k);
+  evaluate_wu<T, FORCES>(c, k, terms);
+  stage_store<T, FORCES>(c, k, eval_energy != 0);
+}
+
+size_t eval_lds_bytes(int precision, int maxa) {
+  return (precision == 0) ? WuLds<double>::bytes(maxa) : WuLds<float>::bytes(maxa);
+}
+
+template <typename T, bool FORCES>
+static hipError_t launch_eval_t(const MMKit &k, int terms, int eval_energy, hipStream_t stream) {
+  const size_t lds = WuLds<T>::bytes(k.maxa);
+  auto fn = k_mm_eval<T, FORCES>;
+  hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(lds));
+  if (err != hipSuccess) {
+    return err;
+  }
+  hipLaunchKernelGGL(fn, dim3(k.nwu), dim3(k.block), lds, stream, k, terms, eval_energy);
+  return hipGetLastError();
+}
+
+hipError_t launch_mm_eval(int precision, const MMKit &k, int terms, int eval_force,
+                          int eval_energy, hipStream_t stream) {
+  if (k.nwu == 0) {
+    return hipSuccess;
+  }
+  if (precision == 0) {
+    return eval_force ? launch_eval_t<double, true>(k, terms, eval_energy, stream) :
+                        launch_eval_t<double, false>(k, terms, eval_energy, stream);
+  }
+  return eval_force ? launch_eval_t<float, true>(k, terms, eval_energy, stream) :
+                      launch_eval_t<float, false>(k, terms, eval_energy, stream);
+}
+
+// ------------------------------------------------------------------------------------------
+// Zeroing kernels: PhaseSpaceSynthesis::initializeForces (kPsyInitializeForces,
+// src/Synthesis/hpc_phasespace_synthesis.cu:243) and ScoreCard::initialize
+// (kInitializeScoreCard, src/Potential/hpc_scorecard.cu:17).  Plain fill kernels keep these
+// graph-capturable.
+__global__ void k_fill_ll(long long *a, long long *b, long long *c, size_t n, long long v) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    a[i] = v;
+    if (b != nullptr) {
+      b[i] = v;
+    }
+    if (c != nullptr) {
+      c[i] = v;
+    }
+  }
+}
+
+hipError_t launch_fill(long long *a, long long *b, long long *c, size_t n, hipStream_t stream) {
+  if (n == 0) {
+    return hipSuccess;
+  }
+  const int block = 256;
+  const size_t want = (n + block - 1) / block;
+  const int grid = static_cast<int>(want < 4096 ? want : 4096);
+  hipLaunchKernelGGL(k_fill_ll, dim3(grid), dim3(block), 0, stream, a, b, c, n, 0LL);
+  return hipGetLastError();
+}
+
+// ScoreCard::commit (kScoreCardCommit, src/Potential/hpc_scorecard.cu:72): fold the
+// instantaneous accumulators into running sums and sums of squares, and store the potential
+// energy total in its slot.
+__global__ void k_scorecard_commit(const long long *inst, double *running, double *squared,
+                                   int nsys, double inv_scale) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) {
+    return;
+  }
+  long long pe = 0;
+  for (int k = 0; k <= 12; k++) {
+    pe += inst[static_cast<size_t>(s) * kScoreStride + k];
+  }
+  for (int k = 0; k < kScoreStride; k++) {
+    const long long v = (k == 28) ? pe : inst[static_cast<size_t>(s) * kScoreStride + k];
+    const double d = static_cast<double>(v) * inv_scale;
+    running[static_cast<size_t>(s) * kScoreStride + k] += d;
+    squared[static_cast<size_t>(s) * kScoreStride + k] += d * d;
+  }
+}
+
+hipError_t launch_scorecard_commit(const long long *inst, double *running, double *squared,
+                                   int nsys, double inv_scale, hipStream_t stream) {
+  if (nsys == 0) {
+    return hipSuccess;
+  }
+  hipLaunchKernelGGL(k_scorecard_commit, dim3((nsys + 127) / 128), dim3(128), 0, stream, inst,
+                     running, squared, nsys, inv_scale);
+  return hipGetLastError();
+}
+
+}  // namespace stormm_amd

@@ -1,0 +1,390 @@
+// Line-minimization kernels: the conjugate-gradient direction update and the four line moves of
+// the reference's launch-per-step minimizer, plus the persistent minimizer that runs every cycle
+// of a work unit's systems inside one block.
+#include "minimize_device.h"
+#include "mm_eval.h"
+#include "kernels.h"
+
+namespace stormm_amd {
+
+__device__ __forceinline__ double wave_sum64(double v) {
+  for (int m = 32; m >= 1; m >>= 1) {
+    v += __shfl_xor(v, m, 64);
+  }
+  return v;
+}
+
+// Polak-Ribiere direction and normalization, one 64-lane wave per system:
+// src/Math/hpc_reduction.cu:39-244 (gather / scatter cores) and
+// src/Math/conjugate_gradient.cui:259-370 (all-reduce kernel).  The per-lane accumulation order
+// and the butterfly reductions match the persistent minimizer, so both paths produce the same
+// directions bit for bit.
+__global__ void __launch_bounds__(64) k_conjugate_gradient(CgKit k, int step, int sd_cycles) {
+  const int s = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int start = k.sys_start[s];
+  const int n = k.sys_natom[s];
+  long long *frc[3] = {k.xfrc, k.yfrc, k.zfrc};
+  long long *prv[3] = {k.xprv, k.yprv, k.zprv};
+  long long *cg[3] = {k.xcg, k.ycg, k.zcg};
+  double gg = 0.0, dgg = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    for (int d = 0; d < 3; d++) {
+      const long long lf = frc[d][start + i];
+      const long long lp = prv[d][start + i];
+      const double df = static_cast<double>(lf);
+      const double dp = static_cast<double>(lp);
+      gg += dp * dp;
+      dgg += static_cast<double>(lf - lp) * df;
+    }
+  }
+  gg = wave_sum64(gg);
+  dgg = wave_sum64(dgg);
+  const double gam = (step >= sd_cycles) ? (dgg / gg) : 0.0;
+  double msum = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const int g = start + i;
+    for (int d = 0; d < 3; d++) {
+      const long long ifv = frc[d][g];
+      prv[d][g] = ifv;
+      if (fabs(gam) > 1.0e-12) {
+        const double dfcg = static_cast<double>(ifv) + (gam * static_cast<double>(cg[d][g]));
+        msum += dfcg * dfcg;
+        const long long v = static_cast<long long>(__builtin_rint(dfcg));
+        cg[d][g] = v;
+        frc[d][g] = v;
+      }
+      else {
+        msum += static_cast<double>(ifv) * static_cast<double>(ifv);
+        cg[d][g] = ifv;
+      }
+    }
+  }
+  msum = wave_sum64(msum);
+  const double inv_factor = 1.0 / (sqrt(msum) * k.inv_frc_scale);
+  for (int i = lane; i < n; i += 64) {
+    const int g = start + i;
+    for (int d = 0; d < 3; d++) {
+      const double v = static_cast<double>(frc[d][g]) * inv_factor;
+      frc[d][g] = k.dp_mode ? static_cast<long long>(__builtin_rint(v)) : static_cast<long long>(v);
+    }
+  }
+}
+
+hipError_t launch_conjugate_gradient(const CgKit &k, int step, int sd_cycles,
+                                     hipStream_t stream) {
+  if (k.nsys == 0) {
+    return hipSuccess;
+  }
+  hipLaunchKernelGGL(k_conjugate_gradient, dim3(k.nsys), dim3(64), 0, stream, k, step,
+                     sd_cycles);
+  return hipGetLastError();
+}
+
+// One line move per system, src/MolecularMechanics/line_movement.cui:4-415.
+__global__ void __launch_bounds__(256) k_line_advance(LineKit k, int move_number) {
+  __shared__ double sh_move;
+  const int s = blockIdx.x;
+  if (threadIdx.x == 0) {
+    long long acc = 0;
+    for (int c = 0; c <= 12; c++) {
+      acc += k.scorecard[static_cast<size_t>(s) * kScoreStride + c];
+    }
+    const double cur = static_cast<double>(acc) * k.inv_nrg_scale;
+    LineState ls;
+    ls.l_move = k.l_move[s];
+    ls.s_move = k.s_move[s];
+    ls.mfac_a = k.mfac_a[s];
+    ls.mfac_b = k.mfac_b[s];
+    ls.mfac_c = k.mfac_c[s];
+    ls.nrg_a = k.nrg_a[s];
+    ls.nrg_b = k.nrg_b[s];
+    ls.nrg_c = k.nrg_c[s];
+    ls.nrg_d = k.nrg_d[s];
+    double mf = line_move_factor(ls, move_number, cur);
+    k.l_move[s] = ls.l_move;
+    k.s_move[s] = ls.s_move;
+    k.mfac_a[s] = ls.mfac_a;
+    k.mfac_b[s] = ls.mfac_b;
+    k.mfac_c[s] = ls.mfac_c;
+    k.nrg_a[s] = ls.nrg_a;
+    k.nrg_b[s] = ls.nrg_b;
+    k.nrg_c[s] = ls.nrg_c;
+    k.nrg_d[s] = ls.nrg_d;
+    if (fabs(mf) > 1.0e-12) {
+      mf *= (k.inv_frc_scale * k.gpos_scale) * ((move_number == 0) ? ls.l_move : ls.s_move);
+    }
+    else {
+      mf = 0.0;
+    }
+    sh_move = mf;
+  }
+  __syncthreads();
+  const double mf = sh_move;
+  if (mf == 0.0) {
+    return;
+  }
+  const int start = k.sys_start[s];
+  const int n = k.sys_natom[s];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int g = start + i;
+    k.xcrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.xfrc[g])));
+    k.ycrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.yfrc[g])));
+    k.zcrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.zfrc[g])));
+  }
+}
+
+hipError_t launch_line_advance(const LineKit &k, int move_number, hipStream_t stream) {
+  if (k.nsys == 0) {
+    return hipSuccess;
+  }
+  hipLaunchKernelGGL(k_line_advance, dim3(k.nsys), dim3(256), 0, stream, k, move_number);
+  return hipGetLastError();
+}
+
+}  // namespace stormm_amd
+
+namespace stormm_amd {
+
+// ------------------------------------------------------------------------------------------
+// Persistent minimizer.  One block owns one work unit for the whole minimization: parameters are
+// staged in LDS once, and every cycle runs the force + energy evaluation, the conjugate-gradient
+// direction update (one wave per system, deterministic butterfly reductions), and the four line
+// moves with three energy-only evaluations, exactly the sequence of the reference's host loop
+// (src/MolecularMechanics/hpc_minimization.cu:262-405) without returning to the host.
+// Coordinates stay in the global fixed-precision arrays (read back into LDS at each evaluation);
+// the normalized move direction is kept in the global force arrays as the reference does.
+// ------------------------------------------------------------------------------------------
+struct MinLds {
+  LineState ls[kMaxWuSystems];
+  double mf[kMaxWuSystems];
+  double rms[kMaxWuSystems];
+  int done[kMaxWuSystems];
+  int cycles[kMaxWuSystems];
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int m = 32; m >= 1; m >>= 1) {
+    v += __shfl_xor(v, m, 64);
+  }
+  return v;
+}
+
+template <typename T>
+__device__ void apply_moves(StageCtx<T> &c, const MMKit &k, const MinLds *M) {
+  for (int a = c.tid; a < c.wu.lds_atoms; a += c.nthr) {
+    const int g = c.L.gidx[a];
+    if (g < 0) {
+      continue;
+    }
+    const double mf = M->mf[c.L.slot[a]];
+    if (mf != 0.0) {
+      k.xcrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.xfrc[g])));
+      k.ycrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.yfrc[g])));
+      k.zcrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.zfrc[g])));
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ double wu_energy(const StageCtx<T> &c, int s) {
+  long long acc = 0;
+  for (int e = 0; e <= 12; e++) {
+    acc += c.L.nrg[s * 16 + e];
+  }
+  return static_cast<double>(acc);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kEvalBlock) k_minimize(MinKit mk) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const MMKit &k = mk.mm;
+  StageCtx<T> c(smem, k.maxa);
+  MinLds *M = reinterpret_cast<MinLds *>(smem + WuLds<T>::bytes(k.maxa));
+  const int terms = kTermNonbonded | kTermValence;
+  init_ctx<T>(c, k, blockIdx.x, terms);
+  stage_load_params<T>(c, k);
+  const int nsys = c.wu.sys_count;
+  const double inv_frc = ldexp(1.0, -k.frc_bits);
+  const double inv_nrg = ldexp(1.0, -k.nrg_bits);
+  const double move_scale = inv_frc * ldexp(1.0, k.gpos_bits);
+  const bool dp_mode = (sizeof(T) == 8);
+  if (c.tid < nsys) {
+    LineState ls;
+    ls.l_move = mk.dx0;
+    ls.s_move = mk.dx0;
+    ls.mfac_a = ls.mfac_b = ls.mfac_c = 0.0;
+    ls.nrg_a = ls.nrg_b = ls.nrg_c = ls.nrg_d = 0.0;
+    M->ls[c.tid] = ls;
+    M->mf[c.tid] = 0.0;
+    M->rms[c.tid] = 0.0;
+    M->done[c.tid] = 0;
+    M->cycles[c.tid] = 0;
+  }
+  // PhaseSpaceSynthesis::primeConjugateGradientCalculation for this unit's atoms.
+  for (int a = c.tid; a < c.wu.lds_atoms; a += c.nthr) {
+    const int g = c.L.gidx[a];
+    if (g >= 0) {
+      mk.xprv[g] = 0;
+      mk.yprv[g] = 0;
+      mk.zprv[g] = 0;
+      mk.xcg[g] = 0;
+      mk.ycg[g] = 0;
+      mk.zcg[g] = 0;
+    }
+  }
+  __syncthreads();
+  const int wave = c.tid >> 6;
+  const int lane = c.tid & 63;
+  const int nwaves = c.nthr >> 6;
+  for (int cyc = 0; cyc < mk.total_cycles; cyc++) {
+    bool all_done = true;
+    for (int s = 0; s < nsys; s++) {
+      all_done = all_done && (M->done[s] != 0);
+    }
+    if (all_done) {
+      break;
+    }
+    evaluate_wu<T, true>(c, k, terms);
+
+    // Convergence test and conjugate-gradient direction, one wave per system.
+    for (int s = wave; s < nsys; s += nwaves) {
+      const WuSys ws = c.L.sys[s];
+      long long *lf[3] = {c.L.fx, c.L.fy, c.L.fz};
+      long long *gf[3] = {k.xfrc, k.yfrc, k.zfrc};
+      long long *pv[3] = {mk.xprv, mk.yprv, mk.zprv};
+      long long *cg[3] = {mk.xcg, mk.ycg, mk.zcg};
+      double ff = 0.0, gg = 0.0, dgg = 0.0;
+      for (int i = lane; i < ws.natom; i += 64) {
+        const int a = ws.lds_base + i;
+        const int g = ws.global_start + i;
+        for (int d = 0; d < 3; d++) {
+          const long long lfv = lf[d][a];
+          const long long lpv = pv[d][g];
+          const double df = static_cast<double>(lfv);
+          const double dp = static_cast<double>(lpv);
+          ff += df * df;
+          gg += dp * dp;
+          dgg += static_cast<double>(lfv - lpv) * df;
+        }
+      }
+      ff = wave_sum(ff);
+      gg = wave_sum(gg);
+      dgg = wave_sum(dgg);
+      const double rms = sqrt(ff / (3.0 * ws.natom)) * inv_frc;
+      const bool done = (M->done[s] != 0) || (mk.conv_rms > 0.0 && rms <= mk.conv_rms);
+      double mf0 = 0.0;
+      if (!done) {
+        const int step = cyc + 1;
+        const double gam = (step >= mk.sd_cycles) ? (dgg / gg) : 0.0;
+        double msum = 0.0;
+        for (int i = lane; i < ws.natom; i += 64) {
+          const int a = ws.lds_base + i;
+          const int g = ws.global_start + i;
+          for (int d = 0; d < 3; d++) {
+            const long long ifv = lf[d][a];
+            pv[d][g] = ifv;
+            if (fabs(gam) > 1.0e-12) {
+              const double dfcg = static_cast<double>(ifv) + (gam * static_cast<double>(cg[d][g]));
+              msum += dfcg * dfcg;
+              const long long v = static_cast<long long>(__builtin_rint(dfcg));
+              cg[d][g] = v;
+              gf[d][g] = v;
+            }
+            else {
+              msum += static_cast<double>(ifv) * static_cast<double>(ifv);
+              cg[d][g] = ifv;
+              gf[d][g] = ifv;
+            }
+          }
+        }
+        msum = wave_sum(msum);
+        const double inv_factor = 1.0 / (sqrt(msum) * inv_frc);
+        for (int i = lane; i < ws.natom; i += 64) {
+          const int g = ws.global_start + i;
+          for (int d = 0; d < 3; d++) {
+            const double v = static_cast<double>(gf[d][g]) * inv_factor;
+            gf[d][g] = dp_mode ? static_cast<long long>(__builtin_rint(v)) :
+                                 static_cast<long long>(v);
+          }
+        }
+      }
+      if (lane == 0) {
+        M->rms[s] = rms;
+        if (!done) {
+          LineState ls = M->ls[s];
+          mf0 = line_move_factor(ls, 0, wu_energy(c, s) * inv_nrg);
+          M->ls[s] = ls;
+          M->mf[s] = (fabs(mf0) > 1.0e-12) ? mf0 * (move_scale * ls.l_move) : 0.0;
+          M->cycles[s] = cyc + 1;
+        }
+        else {
+          M->done[s] = 1;
+          M->mf[s] = 0.0;
+        }
+      }
+    }
+    __syncthreads();
+    apply_moves<T>(c, k, M);
+    for (int move = 1; move < 4; move++) {
+      evaluate_wu<T, false>(c, k, terms);
+      for (int s = c.tid; s < nsys; s += c.nthr) {
+        if (M->done[s] == 0) {
+          LineState ls = M->ls[s];
+          const double mf = line_move_factor(ls, move, wu_energy(c, s) * inv_nrg);
+          M->ls[s] = ls;
+          M->mf[s] = (fabs(mf) > 1.0e-12) ? mf * (move_scale * ls.s_move) : 0.0;
+        }
+        else {
+          M->mf[s] = 0.0;
+        }
+      }
+      __syncthreads();
+      apply_moves<T>(c, k, M);
+    }
+  }
+  // Final energies of every system (hpc_minimization.cu:395-405).
+  evaluate_wu<T, false>(c, k, terms);
+  stage_store<T, false>(c, k, true);
+  for (int s = c.tid; s < nsys; s += c.nthr) {
+    const int si = c.L.sys[s].sys_index;
+    if (mk.cycles_run != nullptr) {
+      mk.cycles_run[si] = M->cycles[s];
+    }
+    if (mk.rms_force != nullptr) {
+      mk.rms_force[si] = M->rms[s];
+    }
+    if (mk.l_move != nullptr) {
+      mk.l_move[si] = M->ls[s].l_move;
+    }
+  }
+}
+
+size_t minimize_lds_bytes(int precision, int maxa) {
+  return ((precision == 0) ? WuLds<double>::bytes(maxa) : WuLds<float>::bytes(maxa)) +
+         sizeof(MinLds);
+}
+
+template <typename T>
+static hipError_t launch_min_t(const MinKit &k, hipStream_t stream) {
+  const size_t lds = WuLds<T>::bytes(k.mm.maxa) + sizeof(MinLds);
+  auto fn = k_minimize<T>;
+  hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(lds));
+  if (err != hipSuccess) {
+    return err;
+  }
+  hipLaunchKernelGGL(fn, dim3(k.mm.nwu), dim3(k.mm.block), lds, stream, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_minimize_persistent(int precision, const MinKit &k, hipStream_t stream) {
+  if (k.mm.nwu == 0) {
+    return hipSuccess;
+  }
+  return (precision == 0) ? launch_min_t<double>(k, stream) : launch_min_t<float>(k, stream);
+}
+
+}  // namespace stormm_amd

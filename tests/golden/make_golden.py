@@ -5,10 +5,11 @@ runs this.  It copies four input data files the reference's tests read (AMBER to
 coordinates of Trp-cage and alanine dipeptide) and extracts the per-atom force snapshots that the
 reference's tests compare against from the Octave-format ``.m`` files, writing them to
 ``reference_snapshots.npz``.  The known-answer energies are transcribed in ``KNOWN_ANSWERS`` with
-the test file and line each comes from.
+the test file and line each comes from, and written to ``known_answers.json``.
 
 Usage: python tests/golden/make_golden.py [/path/to/reference]
 """
+import json
 import os
 import re
 import shutil
@@ -86,6 +87,8 @@ def main(ref_root):
     for src, var in SNAPSHOTS:
         arrays[var] = read_snapshot(os.path.join(ref_root, src), var)
     np.savez_compressed(os.path.join(HERE, "reference_snapshots.npz"), **arrays)
+    with open(os.path.join(HERE, "known_answers.json"), "w") as fh:
+        json.dump(KNOWN_ANSWERS, fh, indent=1, sort_keys=True)
     print(f"wrote {len(arrays)} snapshot arrays and {len(INPUT_FILES)} input files")
 
 

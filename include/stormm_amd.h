@@ -150,6 +150,9 @@ typedef struct {
 
 const char *stormm_last_error(void);
 int stormm_device_info(int32_t *cu_count, int32_t *lds_bytes_per_cu, char *name, int32_t name_len);
+/* Wait for all work on the device (cudaDeviceSynchronize in the reference's timers,
+ * src/MolecularMechanics/hpc_minimization.cu:593). */
+int stormm_device_synchronize(void);
 
 /* ---- AtomGraphSynthesis ------------------------------------------------------------------- */
 /* src/Synthesis/atomgraph_synthesis.h, constructor ag_synthesis_constructors.cpp:11-468 */
@@ -254,6 +257,13 @@ int stormm_minimization_status(const stormm_synthesis *ag, int32_t *cycles_run,
 int stormm_time_mm_evaluation(int32_t precision, const stormm_synthesis *ag,
                               stormm_phasespace *ps, stormm_scorecard *sc, int32_t eval_force,
                               int32_t repeats, void *stream, double *mean_ms);
+
+/* ---- Device microbenchmarks (roofline calibration) ------------------------------------------ */
+/* Stream-copy `bytes` bytes `repeats` times; returns read+write GB/s.  No reference counterpart:
+ * used to confirm the HBM ceiling on the device and to scale profiler memory counters. */
+int stormm_copy_bandwidth(int64_t bytes, int32_t repeats, double *gbps);
+/* Achievable non-packed FP32 FMA rate in TFLOP/s.  No reference counterpart. */
+int stormm_fma_throughput(int32_t iters, double *tflops);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,7 @@ class MinimizeControls(ctypes.Structure):
 _SIGS = {
     "stormm_last_error": (ctypes.c_char_p, []),
     "stormm_device_info": (c_int32, [P(c_int32), P(c_int32), ctypes.c_char_p, c_int32]),
+    "stormm_device_synchronize": (c_int32, []),
     "stormm_synthesis_create": (c_int32, [P(TopologyBatch), P(LaunchShape), P(vp)]),
     "stormm_synthesis_destroy": (c_int32, [vp]),
     "stormm_synthesis_get_info": (c_int32, [vp, P(SynthesisInfo)]),
@@ -116,6 +117,8 @@ _SIGS = {
     "stormm_launch_minimization": (c_int32, [c_int32, vp, vp, P(MinimizeControls), vp, vp,
                                              c_int32, vp]),
     "stormm_minimization_status": (c_int32, [vp, vp, vp]),
+    "stormm_copy_bandwidth": (c_int32, [c_int64, c_int32, P(c_double)]),
+    "stormm_fma_throughput": (c_int32, [c_int32, P(c_double)]),
     "stormm_time_mm_evaluation": (c_int32, [c_int32, vp, vp, vp, c_int32, c_int32, vp,
                                             P(c_double)]),
 }

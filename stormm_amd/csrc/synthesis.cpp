@@ -540,6 +540,11 @@ extern "C" int stormm_device_info(int32_t *cu_count, int32_t *lds_bytes_per_cu, 
   return STORMM_OK;
 }
 
+extern "C" int stormm_device_synchronize(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return STORMM_OK;
+}
+
 extern "C" int stormm_synthesis_plan(const stormm_topology_batch *batch,
                                      const stormm_launch_shape *shape,
                                      stormm_synthesis_info *info) {

@@ -453,6 +453,24 @@ def time_mm_evaluation(prec, ag, ps, sc, eval_force=True, repeats=10, stream=Non
     return ms.value
 
 
+def copy_bandwidth(nbytes: int = 1 << 30, repeats: int = 20) -> float:
+    """Measured stream-copy bandwidth (read + write) in GB/s."""
+    v = ctypes.c_double(0.0)
+    check(_lib.load().stormm_copy_bandwidth(int(nbytes), int(repeats), ctypes.byref(v)))
+    return v.value
+
+
+def fma_throughput(iters: int = 65536) -> float:
+    """Measured non-packed FP32 FMA rate in TFLOP/s."""
+    v = ctypes.c_double(0.0)
+    check(_lib.load().stormm_fma_throughput(int(iters), ctypes.byref(v)))
+    return v.value
+
+
+def synchronize() -> None:
+    check(_lib.load().stormm_device_synchronize())
+
+
 def device_info():
     cu = ctypes.c_int32(0)
     lds = ctypes.c_int32(0)

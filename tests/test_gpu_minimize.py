@@ -73,18 +73,27 @@ def test_minimization_lowers_energy_and_copies_stay_identical(prec):
 
 
 def test_result_is_a_local_minimum():
-    topos, coords = synthetic.config2(nsys=3, seed=15)
-    res = minimize(topos, coords, "double", 500, persistent=True)
-    rng = np.random.default_rng(4)
+    """test/MolecularMechanics/test_minimization.cpp:53-125,193-217: every central-difference
+    gradient component (displacement 0.0008 A) of the result is below 1e-2 kcal/mol/A.  The
+    reference runs its CPU minimizer for 1000 cycles; the GPU line search used here needs 3000 on
+    one of these molecules (the oracle's restatement of that line search agrees)."""
+    topos, coords = synthetic.uniform_batch(3, 14, 24, seed=15, gb=True, perturb=0.1)
+    res = minimize(topos, coords, "double", 3000, persistent=True)
+    h = 0.0008
     for s, t in enumerate(topos):
         x = res["X"][s].astype(np.float64) * 2.0 ** -32
-        e0 = oracle.total_potential(oracle.eval_mm(t, x, False)[0])
-        for _ in range(10):
-            a, d = rng.integers(t.natom), rng.integers(3)
-            for step in (0.01, -0.01):
-                moved = x.copy()
-                moved[a, d] += step
-                assert oracle.total_potential(oracle.eval_mm(t, moved, False)[0]) > e0 - 1e-4
+        handle = oracle.SystemHandle(t)
+        worst = 0.0
+        for a in range(t.natom):
+            for d in range(3):
+                xp, xm = x.copy(), x.copy()
+                xp[a, d] += h
+                xm[a, d] -= h
+                ep = oracle.total_potential(oracle.eval_mm(t, xp, False, handle=handle)[0])
+                em = oracle.total_potential(oracle.eval_mm(t, xm, False, handle=handle)[0])
+                worst = max(worst, abs(ep - em) / (2 * h))
+        print(f"system {s} ({t.natom} atoms): largest gradient component {worst:.2e}")
+        assert worst < 1.0e-2
 
 
 def test_convergence_threshold_stops_systems():

@@ -106,7 +106,7 @@ typedef struct {
 typedef struct {
   int32_t wu_atom_capacity;        /* atoms staged in LDS per work unit (<= 512)                */
   int32_t target_wu_per_cu;        /* work units per compute unit to aim for when packing       */
-  int32_t block_threads;           /* threads per work-unit block (64..1024, multiple of 64)    */
+  int32_t block_threads;           /* threads per work-unit block: 256, 512 (default) or 1024   */
 } stormm_launch_shape;
 
 typedef struct stormm_synthesis stormm_synthesis;

@@ -27,6 +27,17 @@ constexpr int kPsiBits = 40;
 constexpr int kDedrBits = 32;
 // LDS coordinate resolution (bits after the binary point) in single-precision mode.
 constexpr int kLdsCrdBits = 24;
+// Threads of an evaluation block: the default, and the largest a single-precision launch may
+// use; double-precision launches use at most kMaxEvalBlockDouble threads (their kernels need
+// more than the 128 registers per lane that 16-wavefront blocks allow).
+constexpr int kEvalBlock = 512;
+constexpr int kMaxEvalBlock = 1024;
+constexpr int kMaxEvalBlockDouble = 512;
+// The persistent minimizer (evaluation plus line search in one kernel) runs at most 512 threads.
+constexpr int kMaxMinimizeBlock = 512;
+// Entries of each wavefront's LDS queue of near-field Born-integral pairs.  A wave drains 64
+// entries whenever the queue holds at least 64, and one rotation step adds at most 64.
+constexpr int kNearQueue = 128;
 
 // StateVariable slots, src/Potential/energy_enumerators.h:36-75.
 enum NrgSlot : int {
@@ -66,6 +77,16 @@ struct WuSys {
 using Task = int4;
 
 // Everything the fused kernel reads, passed by value.
+// Cosine dihedral parameters: amplitude, cosine and sine of the phase, integer periodicity and
+// the improper flag (which energy slot the term reports to).
+struct DiheParam {
+  double amp, cos_phase, sin_phase;
+  int period, improper;
+};
+
+// Largest dihedral periodicity evaluated by the multiple-angle expansion.
+constexpr int kMaxDihePeriod = 32;
+
 struct MMKit {
   int nsys, nwu, maxa, block;
   int gpos_bits, frc_bits, nrg_bits;
@@ -79,7 +100,7 @@ struct MMKit {
   const int4 *vterms;
   const double2 *bond_par;   // k, r0
   const double2 *angl_par;   // k, theta0
-  const double4 *dihe_par;   // amp, phase, freq, improper (0 / 1)
+  const DiheParam *dihe_par;
   const double2 *dihe14;     // 1 / scee (0 = none), 1 / scnb
   const double *q;           // charge * sqrt(Coulomb constant)
   const double *ljp1;        // sigma (combining-rule systems)

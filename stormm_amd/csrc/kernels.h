@@ -5,10 +5,6 @@
 
 namespace stormm_amd {
 
-// Threads per work-unit block of the evaluation and minimization kernels.
-constexpr int kEvalBlock = 256;
-
-size_t eval_lds_bytes(int precision, int maxa);
 hipError_t launch_mm_eval(int precision, const MMKit &k, int terms, int eval_force,
                           int eval_energy, hipStream_t stream);
 hipError_t launch_fill(long long *a, long long *b, long long *c, size_t n, hipStream_t stream);
@@ -53,6 +49,5 @@ struct MinKit {
   double *rms_force;      // per-system final RMS force
 };
 hipError_t launch_minimize_persistent(int precision, const MinKit &k, hipStream_t stream);
-size_t minimize_lds_bytes(int precision, int maxa);
 
 }  // namespace stormm_amd

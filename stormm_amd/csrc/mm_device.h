@@ -36,10 +36,6 @@ template <> struct Prec<float> {
   static __device__ __forceinline__ long long fixed(float x) {
     return static_cast<long long>(__builtin_rintf(x));
   }
-  // Threshold past which the dihedral angle is recovered from the cross products' separation,
-  // src/Constants/symbol_values.h:78 (near_to_one_f).
-  static constexpr float kNearOne = 0.99993896484375f;
-  static constexpr float kAsymptoticToOne = 0.99999904632568359375f;
 };
 
 template <> struct Prec<double> {
@@ -52,10 +48,6 @@ template <> struct Prec<double> {
   static __device__ __forceinline__ long long fixed(double x) {
     return static_cast<long long>(__builtin_rint(x));
   }
-  // The reference declares its double-precision near-to-one threshold as a float, which rounds
-  // to exactly 1.0 (src/Constants/symbol_values.h:79).
-  static constexpr double kNearOne = 1.0;
-  static constexpr double kAsymptoticToOne = 0.99999904632568359375;
 };
 
 // ------------------------------------------------------------------------------------------
@@ -63,7 +55,10 @@ template <> struct Prec<double> {
 // rotations lane L holds what lane (L - k) mod 16 of its row held.
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ int row_rotate(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x121, 0xF, 0xF, false);
+  // Every lane of a row has a source under row_ror, so the bound-control and old-value
+  // operands never take effect; mov_dpp leaves the old value undefined, which lets the compiler
+  // rotate a register in place instead of zero-filling a fresh one each step.
+  return __builtin_amdgcn_mov_dpp(v, 0x121, 0xF, 0xF, true);
 }
 __device__ __forceinline__ float row_rotate(float v) {
   return __int_as_float(row_rotate(__float_as_int(v)));
@@ -91,81 +86,99 @@ __device__ __forceinline__ void lds_add(long long *addr, long long v) {
 
 // ------------------------------------------------------------------------------------------
 // Born-radius integral contribution of a sphere of screened radius sj at distance r onto an
-// atom of offset radius ri (HCT pairwise descreening): the far-field Taylor series when
-// r > 4 sj, otherwise the analytic branches of src/Potential/nonbonded_potential.tpp:578-617.
-// The three near branches share one reciprocal and one logarithm:
-//   r > ri + sj          : -sj / (2 (r^2 - sj^2))             - ln(|r - sj| / (r + sj)) / (4 r)
-//   |ri - sj| < r <= ri+sj: -(1/ri (2 - theta) - 1/(r + sj)) / 4 - ln(ri / (r + sj)) / (4 r)
-//   r <= |ri - sj|, ri<sj: -sj / (2 (r^2 - sj^2)) - 1/ri       - ln(|r - sj| / (r + sj)) / (4 r)
+// atom of offset radius ri (HCT pairwise descreening), src/Potential/nonbonded_potential.tpp:
+// 578-617, and its radial derivative divided by r (sign convention of :1138-1188).  The
+// reference takes a far-field Taylor series in (sj / r)^2 when r > 4 sj and one of three
+// analytic branches otherwise:
+//   b2  r > ri + sj           : -sj / (2 (r^2 - sj^2))             - ln(|r - sj| / (r + sj)) / (4 r)
+//   b3  |ri - sj| < r <= ri+sj: -(1/ri (2 - theta) - 1/(r + sj)) / 4 - ln(ri / (r + sj)) / (4 r)
+//   b4  r <= |ri - sj|, ri<sj : -sj / (2 (r^2 - sj^2)) - 1/ri       - ln(|r - sj| / (r + sj)) / (4 r)
+// (zero when none applies).  The far and near forms are separate functions so that the tile
+// loops can evaluate the cheap series for every lane and hand the rarer near pairs to a
+// compacted pass.  The near branches share one reciprocal and one logarithm: b3 needs
+// 1/(r + sj), b2 and b4 need 1/(r + sj) and 1/(r^2 - sj^2), both taken from 1/((r + sj)(r - sj))
+// since |r - sj| >= ri keeps that product away from zero in those branches.
 // ------------------------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ T psi_term(T r, T r2, T invr, T ri, T iri, T sj) {
-  using P = Prec<T>;
-  T res = T(0);
-  if (r > T(4) * sj) {
-    const T invr2 = invr * invr;
-    const T tmpsd = sj * sj * invr2;
-    const T dumbo = T(kGTA) + tmpsd * (T(kGTB) + tmpsd * (T(kGTC) + tmpsd * (T(kGTD) +
-                                                                            tmpsd * T(kGTDD))));
-    res = -(sj * tmpsd * invr2 * dumbo);
-  }
-  else {
-    const bool b2 = r > ri + sj;
-    const bool b3 = (!b2) && (r > fabs(ri - sj));
-    const bool b4 = (!b2) && (!b3) && (ri < sj);
-    if (b2 || b3 || b4) {
-      const T uij = P::rcp(r + sj);
-      const T lg = P::log((b3 ? ri : fabs(r - sj)) * uij);
-      T a;
-      if (b3) {
-        const T theta = T(0.5) * iri * invr * (r2 + (ri * ri) - (sj * sj));
-        a = -T(0.25) * ((iri * (T(2) - theta)) - uij);
-      }
-      else {
-        const T t1 = P::rcp(r2 - (sj * sj));
-        a = -(T(0.5) * sj * t1) - (b4 ? iri : T(0));
-      }
-      res = a - (T(0.25) * invr * lg);
-    }
-  }
-  return res;
+__device__ __forceinline__ T psi_far(T sj, T invr2) {
+  const T tmpsd = sj * sj * invr2;
+  const T dumbo = T(kGTA) + tmpsd * (T(kGTB) + tmpsd * (T(kGTC) + tmpsd * (T(kGTD) +
+                                                                          tmpsd * T(kGTDD))));
+  return -(sj * tmpsd * invr2 * dumbo);
 }
 
-// Radial derivative factor of the same integral (dpsi/dr divided by r, sign convention of
-// src/Potential/nonbonded_potential.tpp:1138-1188).  The near branches again share the term
-// ln(L) / (4 r^3) with L as in psi_term.
 template <typename T>
-__device__ __forceinline__ T dpsi_term(T r, T r2, T invr, T invr2, T ri, T iri, T sj) {
-  using P = Prec<T>;
-  T res = T(0);
-  if (r > T(4) * sj) {
-    const T tmpsd = sj * sj * invr2;
-    const T dumbo = T(kGTE) + tmpsd * (T(kGTF) + tmpsd * (T(kGTG) + tmpsd * (T(kGTH) +
-                                                                            tmpsd * T(kGTHH))));
-    res = tmpsd * sj * invr2 * invr2 * dumbo;
-  }
-  else {
-    const bool b2 = r > ri + sj;
-    const bool b3 = (!b2) && (r > fabs(ri - sj));
-    const bool b4 = (!b2) && (!b3) && (ri < sj);
-    if (b2 || b3 || b4) {
-      const T uij = P::rcp(r + sj);
-      const T lg = P::log((b3 ? ri : fabs(r - sj)) * uij);
-      const T invr3 = invr2 * invr;
-      T a;
-      if (b3) {
-        a = -T(0.25) * ((-T(0.5) * (r2 - (ri * ri) + (sj * sj)) * invr3 * iri * iri) +
-                        (invr * uij * (uij - invr)));
-      }
-      else {
-        const T t1 = P::rcp(r2 - (sj * sj));
-        a = sj * t1 * (t1 - (T(0.5) * invr2));
-      }
-      res = a + (T(0.25) * invr3 * lg);
-    }
-  }
-  return res;
+__device__ __forceinline__ T dpsi_far(T sj, T invr2) {
+  const T tmpsd = sj * sj * invr2;
+  const T dumbo = T(kGTE) + tmpsd * (T(kGTF) + tmpsd * (T(kGTG) + tmpsd * (T(kGTH) +
+                                                                          tmpsd * T(kGTHH))));
+  return tmpsd * sj * invr2 * invr2 * dumbo;
 }
+
+// Shared set-up of the near branches: branch flags, 1/(r + sj), 1/(r^2 - sj^2) (b2, b4) and
+// the logarithm's argument.
+template <typename T> struct NearBranch {
+  bool b3, any, b4;
+  T uij, t1, lg;
+  __device__ __forceinline__ NearBranch(T r, T ri, T sj) {
+    using P = Prec<T>;
+    const bool b2 = r > ri + sj;
+    b3 = (!b2) && (r > fabs(ri - sj));
+    b4 = (!b2) && (!b3) && (ri < sj);
+    any = b2 || b3 || b4;
+    const T rps = r + sj;
+    const T rms = r - sj;
+    const T inv = P::rcp(b3 ? rps : rps * rms);
+    uij = b3 ? inv : rms * inv;
+    t1 = inv;
+    lg = P::log((b3 ? ri : fabs(rms)) * uij);
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ T psi_near(T r, T r2, T invr, T ri, T iri, T sj) {
+  const NearBranch<T> nb(r, ri, sj);
+  const T theta = T(0.5) * iri * invr * (r2 + (ri * ri) - (sj * sj));
+  const T a3 = -T(0.25) * ((iri * (T(2) - theta)) - nb.uij);
+  const T a24 = -(T(0.5) * sj * nb.t1) - (nb.b4 ? iri : T(0));
+  const T res = (nb.b3 ? a3 : a24) - (T(0.25) * invr * nb.lg);
+  return nb.any ? res : T(0);
+}
+
+template <typename T>
+__device__ __forceinline__ T dpsi_near(T r, T r2, T invr, T invr2, T ri, T iri, T sj) {
+  const NearBranch<T> nb(r, ri, sj);
+  const T invr3 = invr2 * invr;
+  const T a3 = -T(0.25) * ((-T(0.5) * (r2 - (ri * ri) + (sj * sj)) * invr3 * iri * iri) +
+                           (invr * nb.uij * (nb.uij - invr)));
+  const T a24 = sj * nb.t1 * (nb.t1 - (T(0.5) * invr2));
+  const T res = (nb.b3 ? a3 : a24) + (T(0.25) * invr3 * nb.lg);
+  return nb.any ? res : T(0);
+}
+
+// Fixed-precision image round(x * 2^bits) of a pair contribution added to an LDS accumulator.
+// Single precision converts through 32 bits (rounding at 2^-min(bits, 28), then shifting) when
+// the value fits, and through double precision otherwise.
+__device__ __forceinline__ long long fixed_contrib(float x, int bits) {
+  const int lo = (bits < 28) ? bits : 28;
+  const float v = x * __builtin_ldexpf(1.0f, lo);
+  if (fabsf(v) < 2147483520.0f) {
+    return static_cast<long long>(__float2int_rn(v)) << (bits - lo);
+  }
+  return static_cast<long long>(__builtin_rint(ldexp(static_cast<double>(x), bits)));
+}
+__device__ __forceinline__ long long fixed_contrib(double x, int bits) {
+  return static_cast<long long>(__builtin_rint(ldexp(x, bits)));
+}
+
+// Largest block a kernel of each precision is compiled for.
+template <typename T> struct EvalBlockLimit;
+template <> struct EvalBlockLimit<float> {
+  static constexpr int value = kMaxEvalBlock;
+};
+template <> struct EvalBlockLimit<double> {
+  static constexpr int value = kMaxEvalBlockDouble;
+};
 
 // ------------------------------------------------------------------------------------------
 // Work-unit LDS layout.
@@ -178,16 +191,18 @@ template <typename T> struct WuLds {
   int *gidx, *slot;
   WuSys *sys;
   int2 *lj;        // per system slot: table offset, type count
+  unsigned *nearq; // kNearQueue packed pair entries per wavefront
 
-  static __host__ __device__ size_t bytes(int maxa) {
+  static __host__ __device__ size_t bytes(int maxa, int waves) {
     const size_t m = static_cast<size_t>(maxa);
     return (5 * m * sizeof(long long)) + (kMaxWuSystems * 16 * sizeof(long long)) +
            (3 * kMaxWuSystems * sizeof(long long)) + (8 * m * sizeof(T)) +
            (3 * m * sizeof(crd_t)) + (2 * m * sizeof(int)) + (kMaxWuSystems * sizeof(WuSys)) +
-           (kMaxWuSystems * sizeof(int2));
+           (kMaxWuSystems * sizeof(int2)) + (static_cast<size_t>(waves) * kNearQueue *
+                                              sizeof(unsigned));
   }
 
-  __device__ WuLds(unsigned char *base, int maxa) {
+  __device__ WuLds(unsigned char *base, int maxa, int waves) {
     unsigned char *p = base;
     auto take = [&](size_t nbytes) {
       unsigned char *r = p;
@@ -217,6 +232,8 @@ template <typename T> struct WuLds {
     slot = reinterpret_cast<int *>(take(m * sizeof(int)));
     sys = reinterpret_cast<WuSys *>(take(kMaxWuSystems * sizeof(WuSys)));
     lj = reinterpret_cast<int2 *>(take(kMaxWuSystems * sizeof(int2)));
+    nearq = reinterpret_cast<unsigned *>(take(static_cast<size_t>(waves) * kNearQueue *
+                                              sizeof(unsigned)));
   }
 };
 

@@ -22,7 +22,7 @@ template <typename T> struct StageCtx {
   int crd_shift;     // right shift from global fixed precision to LDS coordinates
   T crd_scale;       // LDS coordinate unit in Angstroms (single precision)
   bool gb;
-  __device__ StageCtx(unsigned char *smem, int maxa) : L(smem, maxa) {}
+  __device__ StageCtx(unsigned char *smem, int maxa, int waves) : L(smem, maxa, waves) {}
 };
 
 // ------------------------------------------------------------------------------------------
@@ -153,16 +153,106 @@ __device__ __forceinline__ int wave_steps(bool diag) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Task dealing for the Born-integral stages: pass tb gives the four 16-lane rows of a
+// wavefront four consecutive tasks; rows past the end of the list get an empty diagonal task,
+// so every lane of a wavefront runs the same passes and rotation steps (the near queue below
+// needs the whole wavefront at each step).
+__device__ __forceinline__ Task wave_task(const MMKit &k, const WuDesc &wu, int t) {
+  if (t < wu.task_count) {
+    return k.tasks[wu.task_first + t];
+  }
+  return make_int4(0, 1 << 8, -1, 0);
+}
+
+// Near-field pair queue of a wavefront.  Entries pack the LDS indices of the pair (9 bits each)
+// and which of the two directions (onto i, onto j) lies within 4 screened radii.
+constexpr unsigned kNearI = 1u << 18;
+constexpr unsigned kNearJ = 1u << 19;
+
+__device__ __forceinline__ unsigned near_entry(int i, int j, bool near_i, bool near_j) {
+  return static_cast<unsigned>(i) | (static_cast<unsigned>(j) << 9) | (near_i ? kNearI : 0u) |
+         (near_j ? kNearJ : 0u);
+}
+
+// Appends the entries of the lanes with `need` set, in lane order; n is wave-uniform.
+__device__ __forceinline__ void nearq_push(unsigned *q, int &n, bool need, unsigned entry) {
+  const unsigned long long m = __ballot(need);
+  const int below = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(m >> 32),
+                                              __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(m),
+                                                                        0u));
+  if (need) {
+    q[n + below] = entry;
+  }
+  n += __popcll(m);
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Moves the entries past the first 64 (already drained) to the front of the queue.
+__device__ __forceinline__ void nearq_shift(unsigned *q, int &n, int lane) {
+  const int rem = n - 64;
+  __builtin_amdgcn_wave_barrier();
+  if (lane < rem) {
+    const unsigned e = q[64 + lane];
+    q[lane] = e;
+  }
+  n = rem;
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Geometry of a queued pair, recomputed from the LDS coordinates.
+template <typename T> struct NearPair {
+  int i, j;
+  bool near_i, near_j;
+  T dx, dy, dz, r2, invr, r;
+  __device__ __forceinline__ NearPair(const StageCtx<T> &c, unsigned e) {
+    using P = Prec<T>;
+    i = static_cast<int>(e & 0x1ffu);
+    j = static_cast<int>((e >> 9) & 0x1ffu);
+    near_i = (e & kNearI) != 0u;
+    near_j = (e & kNearJ) != 0u;
+    dx = crd_delta(c.L.cx[j], c.L.cx[i], c.crd_scale);
+    dy = crd_delta(c.L.cy[j], c.L.cy[i], c.crd_scale);
+    dz = crd_delta(c.L.cz[j], c.L.cz[i], c.crd_scale);
+    r2 = (dx * dx) + (dy * dy) + (dz * dz);
+    invr = P::rsqrt(r2);
+    r = r2 * invr;
+  }
+};
+
+// Near-field Born-integral terms of the first `count` queued pairs, one pair per lane.
+template <typename T>
+__device__ __forceinline__ void psi_drain(StageCtx<T> &c, const unsigned *q, int count,
+                                          int lane) {
+  WuLds<T> &L = c.L;
+  if (lane < count) {
+    const NearPair<T> p(c, q[lane]);
+    if (p.near_i) {
+      const T t = psi_near(p.r, p.r2, p.invr, L.rho[p.i], L.irho[p.i], L.scr[p.j]);
+      lds_add(&L.psi[p.i], fixed_contrib(t, kPsiBits));
+    }
+    if (p.near_j) {
+      const T t = psi_near(p.r, p.r2, p.invr, L.rho[p.j], L.irho[p.j], L.scr[p.i]);
+      lds_add(&L.psi[p.j], fixed_contrib(t, kPsiBits));
+    }
+  }
+}
+
+// Born-radius integrals (src/Potential/nonbonded_potential.tpp:560-650).  Each lane evaluates
+// the far-field series of its pair at every rotation step and queues the pair when either
+// direction is within 4 screened radii; a full wavefront of queued pairs is evaluated at once.
 template <typename T>
 __device__ void stage_psi(StageCtx<T> &c, const MMKit &k) {
   using P = Prec<T>;
   WuLds<T> &L = c.L;
-  const int row = c.tid >> 4;
-  const int nrows = c.nthr >> 4;
   const int il = c.tid & 15;
+  const int lane = c.tid & 63;
+  const int wave = c.tid >> 6;
+  const int nwaves = c.nthr >> 6;
+  unsigned *q = L.nearq + (wave * kNearQueue);
+  int qn = 0;
   const double psi_scale = ldexp(1.0, kPsiBits);
-  for (int t = row; t < c.wu.task_count; t += nrows) {
-    const Task tk = k.tasks[c.wu.task_first + t];
+  for (int tb = 4 * wave; tb < c.wu.task_count; tb += 4 * nwaves) {
+    const Task tk = wave_task(k, c.wu, tb + ((c.tid >> 4) & 3));
     const int ib = tk.x & 0xffff, jb = tk.x >> 16;
     const bool diag = (tk.y >> 8) & 1;
     const int ni = (tk.y >> 12) & 0xff, nj = (tk.y >> 20) & 0xff;
@@ -170,29 +260,37 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k) {
     const int i = ib + il;
     const int j0 = jb + il;
     const auto xi = L.cx[i], yi = L.cy[i], zi = L.cz[i];
-    const T ri = L.rho[i], iri = L.irho[i], si = L.scr[i];
+    const T si = L.scr[i];
+    const T si4 = T(4) * si;
     auto xj = L.cx[j0], yj = L.cy[j0], zj = L.cz[j0];
-    T rj = L.rho[j0], irj = L.irho[j0], sj = L.scr[j0];
-    // The Born-radius integral is summed in double precision.
+    T sj = L.scr[j0];
+    // The far-field series is summed in double precision.
     double acci = 0.0, accj = 0.0;
     const int nsteps = wave_steps(diag);
     for (int ks = 0; ks < nsteps; ks++) {
       const int jl = (il - ks) & 15;
-      if (pair_valid(ivalid, jl, nj, diag, ks, il)) {
-        const T dx = crd_delta(xj, xi, c.crd_scale);
-        const T dy = crd_delta(yj, yi, c.crd_scale);
-        const T dz = crd_delta(zj, zi, c.crd_scale);
-        const T r2 = (dx * dx) + (dy * dy) + (dz * dz);
-        const T invr = P::rsqrt(r2);
-        const T r = r2 * invr;
-        acci += static_cast<double>(psi_term(r, r2, invr, ri, iri, sj));
-        accj += static_cast<double>(psi_term(r, r2, invr, rj, irj, si));
+      const bool valid = pair_valid(ivalid, jl, nj, diag, ks, il);
+      const T dx = crd_delta(xj, xi, c.crd_scale);
+      const T dy = crd_delta(yj, yi, c.crd_scale);
+      const T dz = crd_delta(zj, zi, c.crd_scale);
+      const T r2 = (dx * dx) + (dy * dy) + (dz * dz);
+      const T invr = P::rsqrt(r2);
+      const T invr2 = invr * invr;
+      const T r = r2 * invr;
+      const bool far_i = r > T(4) * sj;
+      const bool far_j = r > si4;
+      acci += (valid && far_i) ? static_cast<double>(psi_far(sj, invr2)) : 0.0;
+      accj += (valid && far_j) ? static_cast<double>(psi_far(si, invr2)) : 0.0;
+      const bool near_i = valid && !far_i;
+      const bool near_j = valid && !far_j;
+      nearq_push(q, qn, near_i || near_j, near_entry(i, jb + jl, near_i, near_j));
+      if (qn >= 64) {
+        psi_drain(c, q, 64, lane);
+        nearq_shift(q, qn, lane);
       }
       xj = row_rotate(xj);
       yj = row_rotate(yj);
       zj = row_rotate(zj);
-      rj = row_rotate(rj);
-      irj = row_rotate(irj);
       sj = row_rotate(sj);
       accj = row_rotate(accj);
     }
@@ -204,6 +302,7 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k) {
       lds_add(&L.psi[jb + jl_home], Prec<double>::fixed(accj * psi_scale));
     }
   }
+  psi_drain(c, q, qn, lane);
   __syncthreads();
 }
 
@@ -441,17 +540,50 @@ __device__ void stage_chain(StageCtx<T> &c, const MMKit &k) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Forces from the Born-radius derivatives, src/Potential/nonbonded_potential.tpp:1018-1292.
+// Near-field radius-derivative forces of the first `count` queued pairs, one pair per lane.
+template <typename T>
+__device__ __forceinline__ void deriv_drain(StageCtx<T> &c, const unsigned *q, int count,
+                                            int lane, int frc_bits) {
+  WuLds<T> &L = c.L;
+  if (lane < count) {
+    const NearPair<T> p(c, q[lane]);
+    const T invr2 = p.invr * p.invr;
+    T fmag = T(0);
+    if (p.near_i) {
+      fmag += dpsi_near(p.r, p.r2, p.invr, invr2, L.rho[p.i], L.irho[p.i], L.scr[p.j]) *
+              L.sd[p.i];
+    }
+    if (p.near_j) {
+      fmag += dpsi_near(p.r, p.r2, p.invr, invr2, L.rho[p.j], L.irho[p.j], L.scr[p.i]) *
+              L.sd[p.j];
+    }
+    const long long fx = fixed_contrib(fmag * p.dx, frc_bits);
+    const long long fy = fixed_contrib(fmag * p.dy, frc_bits);
+    const long long fz = fixed_contrib(fmag * p.dz, frc_bits);
+    lds_add(&L.fx[p.i], -fx);
+    lds_add(&L.fy[p.i], -fy);
+    lds_add(&L.fz[p.i], -fz);
+    lds_add(&L.fx[p.j], fx);
+    lds_add(&L.fy[p.j], fy);
+    lds_add(&L.fz[p.j], fz);
+  }
+}
+
+// Forces from the Born-radius derivatives, src/Potential/nonbonded_potential.tpp:1018-1292,
+// with the same far-field / queued near-field split as stage_psi.
 template <typename T>
 __device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
   using P = Prec<T>;
   WuLds<T> &L = c.L;
-  const int row = c.tid >> 4;
-  const int nrows = c.nthr >> 4;
   const int il = c.tid & 15;
+  const int lane = c.tid & 63;
+  const int wave = c.tid >> 6;
+  const int nwaves = c.nthr >> 6;
+  unsigned *q = L.nearq + (wave * kNearQueue);
+  int qn = 0;
   const T frc_scale = T(ldexp(1.0, k.frc_bits));
-  for (int t = row; t < c.wu.task_count; t += nrows) {
-    const Task tk = k.tasks[c.wu.task_first + t];
+  for (int tb = 4 * wave; tb < c.wu.task_count; tb += 4 * nwaves) {
+    const Task tk = wave_task(k, c.wu, tb + ((c.tid >> 4) & 3));
     const int ib = tk.x & 0xffff, jb = tk.x >> 16;
     const bool diag = (tk.y >> 8) & 1;
     const int ni = (tk.y >> 12) & 0xff, nj = (tk.y >> 20) & 0xff;
@@ -459,37 +591,44 @@ __device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
     const int i = ib + il;
     const int j0 = jb + il;
     const auto xi = L.cx[i], yi = L.cy[i], zi = L.cz[i];
-    const T ri = L.rho[i], iri = L.irho[i], si = L.scr[i], sdi = L.sd[i];
+    const T si = L.scr[i], sdi = L.sd[i];
+    const T si4 = T(4) * si;
     auto xj = L.cx[j0], yj = L.cy[j0], zj = L.cz[j0];
-    T rj = L.rho[j0], irj = L.irho[j0], sj = L.scr[j0], sdj = L.sd[j0];
+    T sj = L.scr[j0], sdj = L.sd[j0];
     T fxi = T(0), fyi = T(0), fzi = T(0);
     T fxj = T(0), fyj = T(0), fzj = T(0);
     const int nsteps = wave_steps(diag);
     for (int ks = 0; ks < nsteps; ks++) {
       const int jl = (il - ks) & 15;
-      if (pair_valid(ivalid, jl, nj, diag, ks, il)) {
-        const T dx = crd_delta(xj, xi, c.crd_scale);
-        const T dy = crd_delta(yj, yi, c.crd_scale);
-        const T dz = crd_delta(zj, zi, c.crd_scale);
-        const T r2 = (dx * dx) + (dy * dy) + (dz * dz);
-        const T invr = P::rsqrt(r2);
-        const T invr2 = invr * invr;
-        const T r = r2 * invr;
-        const T datmpi = dpsi_term(r, r2, invr, invr2, ri, iri, sj);
-        const T datmpj = dpsi_term(r, r2, invr, invr2, rj, irj, si);
-        const T fmag = (datmpi * sdi) + (datmpj * sdj);
-        fxi -= fmag * dx;
-        fyi -= fmag * dy;
-        fzi -= fmag * dz;
-        fxj += fmag * dx;
-        fyj += fmag * dy;
-        fzj += fmag * dz;
+      const bool valid = pair_valid(ivalid, jl, nj, diag, ks, il);
+      const T dx = crd_delta(xj, xi, c.crd_scale);
+      const T dy = crd_delta(yj, yi, c.crd_scale);
+      const T dz = crd_delta(zj, zi, c.crd_scale);
+      const T r2 = (dx * dx) + (dy * dy) + (dz * dz);
+      const T invr = P::rsqrt(r2);
+      const T invr2 = invr * invr;
+      const T r = r2 * invr;
+      const bool far_i = r > T(4) * sj;
+      const bool far_j = r > si4;
+      const T ti = (valid && far_i) ? dpsi_far(sj, invr2) * sdi : T(0);
+      const T tj = (valid && far_j) ? dpsi_far(si, invr2) * sdj : T(0);
+      const T fmag = ti + tj;
+      fxi -= fmag * dx;
+      fyi -= fmag * dy;
+      fzi -= fmag * dz;
+      fxj += fmag * dx;
+      fyj += fmag * dy;
+      fzj += fmag * dz;
+      const bool near_i = valid && !far_i;
+      const bool near_j = valid && !far_j;
+      nearq_push(q, qn, near_i || near_j, near_entry(i, jb + jl, near_i, near_j));
+      if (qn >= 64) {
+        deriv_drain(c, q, 64, lane, k.frc_bits);
+        nearq_shift(q, qn, lane);
       }
       xj = row_rotate(xj);
       yj = row_rotate(yj);
       zj = row_rotate(zj);
-      rj = row_rotate(rj);
-      irj = row_rotate(irj);
       sj = row_rotate(sj);
       sdj = row_rotate(sdj);
       fxj = row_rotate(fxj);
@@ -509,6 +648,7 @@ __device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
       lds_add(&L.fz[j], P::fixed(fzj * frc_scale));
     }
   }
+  deriv_drain(c, q, qn, lane, k.frc_bits);
   __syncthreads();
 }
 
@@ -598,50 +738,39 @@ __device__ __forceinline__ double eval_angle(const ValenceCtx &v, int i, int j, 
   return ka * dtheta * dtheta;
 }
 
-// Dihedral angle from the two plane normals: the arccosine, with the reference's fallback for
-// (anti-)parallel normals (src/Math/rounding.tpp:24-59; the double-precision threshold is 1.0,
-// the reference's float-typed constant), signed by the normals' cross product along the
-// central bond.
-__device__ __forceinline__ double dihedral_angle(double costheta, const Vec3 &crabbc,
-                                                 const Vec3 &crbccd, const Vec3 &bc,
-                                                 const Vec3 &scr) {
-  const double sgn = dot(scr, bc);
-  if (fabs(costheta) >= 1.0) {
-    const double m1 = 1.0 / sqrt(dot(crabbc, crabbc));
-    const double m2 = 1.0 / sqrt(dot(crbccd, crbccd));
-    Vec3 rd = {(crbccd.x * m2) - (crabbc.x * m1), (crbccd.y * m2) - (crabbc.y * m1),
-               (crbccd.z * m2) - (crabbc.z * m1)};
-    double rs = sqrt(dot(rd, rd));
-    if (fabs(rs) > 1.0) {
-      rd = {(crbccd.x * m2) + (crabbc.x * m1), (crbccd.y * m2) + (crabbc.y * m1),
-            (crbccd.z * m2) + (crabbc.z * m1)};
-      rs = 3.14159265358979323846 - sqrt(dot(rd, rd));
-    }
-    return (sgn > 0.0) ? rs : -rs;
-  }
-  const double th = acos(costheta);
-  return (sgn > 0.0) ? th : -th;
-}
-
-// Cosine dihedral, src/Potential/valence_potential.tpp:359-578.
+// Cosine dihedral, src/Potential/valence_potential.tpp:359-578: E = A (1 + cos(n theta - phase))
+// with the force factor A n sin(n theta - phase).  The reference recovers the signed angle
+// theta with an arccosine (src/Math/rounding.tpp:24-59) and then takes the sine and cosine of
+// n theta - phase.  For the integer periodicities used here both functions follow exactly from
+// cos(theta) and sin(theta) by the multiple-angle expansion (cos theta + i sin theta)^n, and the
+// phase enters through its precomputed cosine and sine.  cos(theta) is the normalized dot
+// product of the two plane normals and the signed sin(theta) their cross product projected on
+// the central bond (the reference's sign convention for theta), so no inverse or forward
+// trigonometric function is evaluated.
 template <bool FORCES>
 __device__ __forceinline__ double eval_dihedral(const ValenceCtx &v, int i, int j, int kk, int l,
-                                                double amp, double phase, double freq,
-                                                double fscale) {
+                                                const DiheParam &dp, double fscale) {
   const Vec3 ab = v.disp(i, j);
   const Vec3 bc = v.disp(j, kk);
   const Vec3 cd = v.disp(kk, l);
   Vec3 crabbc = cross(ab, bc);
   Vec3 crbccd = cross(bc, cd);
-  double costheta = dot(crabbc, crbccd) / sqrt(dot(crabbc, crabbc) * dot(crbccd, crbccd));
+  const double inv12 = 1.0 / sqrt(dot(crabbc, crabbc) * dot(crbccd, crbccd));
+  const double mgbc = sqrt(dot(bc, bc));
   const Vec3 scr = cross(crabbc, crbccd);
-  costheta = (costheta < -1.0) ? -1.0 : ((costheta > 1.0) ? 1.0 : costheta);
-  const double theta = dihedral_angle(costheta, crabbc, crbccd, bc, scr);
-  const double sangle = (freq * theta) - phase;
+  const double cth = dot(crabbc, crbccd) * inv12;
+  const double sth = dot(scr, bc) * inv12 / mgbc;
+  double cn = 1.0, sn = 0.0;
+  for (int p = 0; p < dp.period; p++) {
+    const double t = (cn * cth) - (sn * sth);
+    sn = (sn * cth) + (cn * sth);
+    cn = t;
+  }
+  const double cos_sangle = (cn * dp.cos_phase) + (sn * dp.sin_phase);
   if (FORCES) {
-    const double fr = amp * freq * sin(sangle);
+    const double sin_sangle = (sn * dp.cos_phase) - (cn * dp.sin_phase);
+    const double fr = dp.amp * static_cast<double>(dp.period) * sin_sangle;
     const double mgab = sqrt(dot(ab, ab));
-    const double mgbc = sqrt(dot(bc, bc));
     const double mgcd = sqrt(dot(cd, cd));
     const double invab = 1.0 / mgab;
     const double invbc = 1.0 / mgbc;
@@ -662,20 +791,20 @@ __device__ __forceinline__ double eval_dihedral(const ValenceCtx &v, int i, int 
     const double fb2 = cosc * invbc * isinc2;
     const double fd = -invcd * isinc2;
     const long long fix = fix64(crabbc.x * fa * fscale);
-    const long long fjx = fix64(((fb1 * crabbc.x) - (fb2 * crbccd.x)) * fscale);
-    const long long flx = fix64(-fd * crbccd.x * fscale);
     const long long fiy = fix64(crabbc.y * fa * fscale);
-    const long long fjy = fix64(((fb1 * crabbc.y) - (fb2 * crbccd.y)) * fscale);
-    const long long fly = fix64(-fd * crbccd.y * fscale);
     const long long fiz = fix64(crabbc.z * fa * fscale);
-    const long long fjz = fix64(((fb1 * crabbc.z) - (fb2 * crbccd.z)) * fscale);
-    const long long flz = fix64(-fd * crbccd.z * fscale);
     v.add(i, fix, fiy, fiz);
+    const long long flx = fix64(-fd * crbccd.x * fscale);
+    const long long fly = fix64(-fd * crbccd.y * fscale);
+    const long long flz = fix64(-fd * crbccd.z * fscale);
+    v.add(l, flx, fly, flz);
+    const long long fjx = fix64(((fb1 * crabbc.x) - (fb2 * crbccd.x)) * fscale);
+    const long long fjy = fix64(((fb1 * crabbc.y) - (fb2 * crbccd.y)) * fscale);
+    const long long fjz = fix64(((fb1 * crabbc.z) - (fb2 * crbccd.z)) * fscale);
     v.add(j, fjx, fjy, fjz);
     v.add(kk, -(fix + fjx + flx), -(fiy + fjy + fly), -(fiz + fjz + flz));
-    v.add(l, flx, fly, flz);
   }
-  return amp * (1.0 + cos(sangle));
+  return dp.amp * (1.0 + cos_sangle);
 }
 
 // Attenuated 1:4 pair, src/Potential/valence_potential.tpp:1288-1419.
@@ -751,9 +880,9 @@ __device__ void stage_valence(StageCtx<T> &c, const MMKit &k) {
               fix64(eval_angle<FORCES>(v, i, j, kk, p.x, p.y, fscale) * nrg_scale));
     }
     else {
-      const double4 p = k.dihe_par[vt.w];
-      const double e = eval_dihedral<FORCES>(v, i, j, kk, l, p.x, p.y, p.z, fscale);
-      lds_add(&nrg[(p.w != 0.0) ? kImproperDihedral : kProperDihedral], fix64(e * nrg_scale));
+      const DiheParam p = k.dihe_par[vt.w];
+      const double e = eval_dihedral<FORCES>(v, i, j, kk, l, p, fscale);
+      lds_add(&nrg[p.improper ? kImproperDihedral : kProperDihedral], fix64(e * nrg_scale));
       const double2 a14 = k.dihe14[vt.w];
       if (a14.x > 0.0) {
         double ele, vdw;

@@ -1,26 +1,26 @@
 // Fused batched energy / force evaluation kernels and their launchers.
+#include <algorithm>
+
 #include "mm_eval.h"
 #include "kernels.h"
 
 namespace stormm_amd {
 
 template <typename T, bool FORCES>
-__global__ void __launch_bounds__(kEvalBlock) k_mm_eval(MMKit k, int terms, int eval_energy) {
+__global__ void __launch_bounds__(EvalBlockLimit<T>::value)
+    k_mm_eval(MMKit k, int terms, int eval_energy) {
   extern __shared__ __align__(16) unsigned char smem[];
-  StageCtx<T> c(smem, k.maxa);
+  StageCtx<T> c(smem, k.maxa, static_cast<int>(blockDim.x >> 6));
   init_ctx<T>(c, k, blockIdx.x, terms);
   stage_load_params<T>(c, k);
   evaluate_wu<T, FORCES>(c, k, terms);
   stage_store<T, FORCES>(c, k, eval_energy != 0);
 }
 
-size_t eval_lds_bytes(int precision, int maxa) {
-  return (precision == 0) ? WuLds<double>::bytes(maxa) : WuLds<float>::bytes(maxa);
-}
-
 template <typename T, bool FORCES>
 static hipError_t launch_eval_t(const MMKit &k, int terms, int eval_energy, hipStream_t stream) {
-  const size_t lds = WuLds<T>::bytes(k.maxa);
+  const int block = std::min(k.block, EvalBlockLimit<T>::value);
+  const size_t lds = WuLds<T>::bytes(k.maxa, block / 64);
   auto fn = k_mm_eval<T, FORCES>;
   hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -28,7 +28,7 @@ static hipError_t launch_eval_t(const MMKit &k, int terms, int eval_energy, hipS
   if (err != hipSuccess) {
     return err;
   }
-  hipLaunchKernelGGL(fn, dim3(k.nwu), dim3(k.block), lds, stream, k, terms, eval_energy);
+  hipLaunchKernelGGL(fn, dim3(k.nwu), dim3(block), lds, stream, k, terms, eval_energy);
   return hipGetLastError();
 }
 

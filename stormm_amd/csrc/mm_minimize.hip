@@ -1,6 +1,8 @@
 // Line-minimization kernels: the conjugate-gradient direction update and the four line moves of
 // the reference's launch-per-step minimizer, plus the persistent minimizer that runs every cycle
 // of a work unit's systems inside one block.
+#include <algorithm>
+
 #include "minimize_device.h"
 #include "mm_eval.h"
 #include "kernels.h"
@@ -197,11 +199,12 @@ __device__ double wu_energy(const StageCtx<T> &c, int s) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kEvalBlock) k_minimize(MinKit mk) {
+__global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
   extern __shared__ __align__(16) unsigned char smem[];
   const MMKit &k = mk.mm;
-  StageCtx<T> c(smem, k.maxa);
-  MinLds *M = reinterpret_cast<MinLds *>(smem + WuLds<T>::bytes(k.maxa));
+  const int waves = static_cast<int>(blockDim.x >> 6);
+  StageCtx<T> c(smem, k.maxa, waves);
+  MinLds *M = reinterpret_cast<MinLds *>(smem + WuLds<T>::bytes(k.maxa, waves));
   const int terms = kTermNonbonded | kTermValence;
   init_ctx<T>(c, k, blockIdx.x, terms);
   stage_load_params<T>(c, k);
@@ -361,14 +364,10 @@ __global__ void __launch_bounds__(kEvalBlock) k_minimize(MinKit mk) {
   }
 }
 
-size_t minimize_lds_bytes(int precision, int maxa) {
-  return ((precision == 0) ? WuLds<double>::bytes(maxa) : WuLds<float>::bytes(maxa)) +
-         sizeof(MinLds);
-}
-
 template <typename T>
 static hipError_t launch_min_t(const MinKit &k, hipStream_t stream) {
-  const size_t lds = WuLds<T>::bytes(k.mm.maxa) + sizeof(MinLds);
+  const int block = std::min(k.mm.block, kMaxMinimizeBlock);
+  const size_t lds = WuLds<T>::bytes(k.mm.maxa, block / 64) + sizeof(MinLds);
   auto fn = k_minimize<T>;
   hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -376,7 +375,7 @@ static hipError_t launch_min_t(const MinKit &k, hipStream_t stream) {
   if (err != hipSuccess) {
     return err;
   }
-  hipLaunchKernelGGL(fn, dim3(k.mm.nwu), dim3(k.mm.block), lds, stream, k);
+  hipLaunchKernelGGL(fn, dim3(k.mm.nwu), dim3(block), lds, stream, k);
   return hipGetLastError();
 }
 

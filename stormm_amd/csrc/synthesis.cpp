@@ -92,7 +92,7 @@ struct SynthesisPlan {
   std::vector<unsigned> masks;
   std::vector<int4> vterms;
   std::vector<double2> bond_par, angl_par, dihe14;
-  std::vector<double4> dihe_par;
+  std::vector<DiheParam> dihe_par;
   long long pairs = 0;
   long long vterm_count = 0;
   int lj_table_systems = 0;
@@ -462,8 +462,18 @@ static int build_plan(const stormm_topology_batch *b, const stormm_launch_shape 
         P.vterms.push_back(make_int4(kVDihedral | (slot << 8), lidx(q4[0]) | (lidx(q4[1]) << 16),
                                      lidx(q4[2]) | (lidx(q4[3]) << 16),
                                      static_cast<int>(P.dihe_par.size())));
-        P.dihe_par.push_back(make_double4(b->dihe_amp[k], b->dihe_phase[k], b->dihe_freq[k],
-                                          b->dihe_improper[k] ? 1.0 : 0.0));
+        const double freq = b->dihe_freq[k];
+        if (!(freq >= 0.0 && freq <= kMaxDihePeriod && freq == std::rint(freq))) {
+          return fail(STORMM_ERR_INVALID_ARGUMENT,
+                      "dihedral periodicity must be an integer between 0 and 32");
+        }
+        DiheParam dp;
+        dp.amp = b->dihe_amp[k];
+        dp.cos_phase = std::cos(b->dihe_phase[k]);
+        dp.sin_phase = std::sin(b->dihe_phase[k]);
+        dp.period = static_cast<int>(freq);
+        dp.improper = b->dihe_improper[k] ? 1 : 0;
+        P.dihe_par.push_back(dp);
         const double scee = b->dihe_scee[k], scnb = b->dihe_scnb[k];
         P.dihe14.push_back(make_double2(scee > 0.0 ? 1.0 / scee : 0.0,
                                         (scee > 0.0 && scnb > 0.0) ? 1.0 / scnb : 0.0));
@@ -484,8 +494,8 @@ static int build_plan(const stormm_topology_batch *b, const stormm_launch_shape 
   }
   P.vterm_count = static_cast<long long>(P.vterms.size());
   P.block = (shape.block_threads > 0) ? shape.block_threads : kEvalBlock;
-  if (P.block != kEvalBlock) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "block_threads must be 256 in this build");
+  if (P.block != 256 && P.block != 512 && P.block != 1024) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "block_threads must be 256, 512 or 1024");
   }
   return STORMM_OK;
 }
@@ -627,7 +637,7 @@ extern "C" int stormm_synthesis_create(const stormm_topology_batch *batch,
   k.vterms = ag->vterms.as<int4>();
   k.bond_par = ag->bond_par.as<double2>();
   k.angl_par = ag->angl_par.as<double2>();
-  k.dihe_par = ag->dihe_par.as<double4>();
+  k.dihe_par = ag->dihe_par.as<DiheParam>();
   k.dihe14 = ag->dihe14.as<double2>();
   k.q = ag->q.as<double>();
   k.ljp1 = ag->ljp1.as<double>();
@@ -1352,6 +1362,7 @@ extern "C" int stormm_autotune_launch_shape(const stormm_topology_batch *batch,
   }
   const int caps[4] = {128, 256, 384, 512};
   const int per_cu[3] = {1, 2, 4};
+  const int blocks[3] = {256, 512, 1024};
   double best_t = 1.0e30;
   stormm_launch_shape best_shape = {512, 2, kEvalBlock};
   // Coordinates for timing: a loose cubic lattice per system (1.6 A spacing) keeps every pair
@@ -1369,7 +1380,8 @@ extern "C" int stormm_autotune_launch_shape(const stormm_topology_batch *batch,
       continue;
     }
     for (int pc : per_cu) {
-      stormm_launch_shape shape = {cap, pc, kEvalBlock};
+     for (int bt : blocks) {
+      stormm_launch_shape shape = {cap, pc, bt};
       stormm_synthesis *ag = nullptr;
       int rc = stormm_synthesis_create(batch, &shape, &ag);
       if (rc != STORMM_OK) {
@@ -1401,6 +1413,7 @@ extern "C" int stormm_autotune_launch_shape(const stormm_topology_batch *batch,
         best_t = ms;
         best_shape = shape;
       }
+     }
     }
   }
   *best = best_shape;

@@ -159,6 +159,20 @@ def test_results_do_not_depend_on_batch_composition(prec):
 
 
 @pytest.mark.parametrize("prec", ["double", "single"])
+@pytest.mark.parametrize("shape", [dict(wu_atom_capacity=256, target_wu_per_cu=4, block_threads=256),
+                                   dict(wu_atom_capacity=512, target_wu_per_cu=1, block_threads=1024),
+                                   dict(wu_atom_capacity=384, target_wu_per_cu=2, block_threads=512)])
+def test_results_do_not_depend_on_block_size(prec, shape):
+    # Every accumulation is a fixed-precision integer sum or is made inside one tile task, so the
+    # threads per block (and with them the task-to-wavefront assignment) cannot change a bit.
+    topos, coords = synthetic.config1(nsys=48, seed=9)
+    ref = evaluate(topos, coords, prec)
+    other = evaluate(topos, coords, prec, shape=shape)
+    assert np.array_equal(other["E"], ref["E"])
+    assert all(np.array_equal(x, y) for x, y in zip(other["Ffix"], ref["Ffix"]))
+
+
+@pytest.mark.parametrize("prec", ["double", "single"])
 def test_energy_only_evaluation_matches_force_evaluation_energies(prec):
     topos, coords = synthetic.config1(nsys=16, seed=7)
     a = evaluate(topos, coords, prec, eval_force=True)

@@ -38,6 +38,16 @@ def test_persistent_launch_equals_per_step_launches(prec):
     assert np.array_equal(a["E"], b["E"])
 
 
+@pytest.mark.parametrize("prec", ["double", "single"])
+def test_persistent_minimizer_does_not_depend_on_block_size(prec):
+    topos, coords = synthetic.config2(nsys=12, seed=15)
+    a = minimize(topos, coords, prec, 40, persistent=True)
+    b = minimize(topos, coords, prec, 40, persistent=True,
+                 shape=dict(wu_atom_capacity=256, target_wu_per_cu=4, block_threads=256))
+    assert all(np.array_equal(x, y) for x, y in zip(a["X"], b["X"]))
+    assert np.array_equal(a["E"], b["E"])
+
+
 def test_double_precision_minimizer_tracks_the_oracle():
     topos, coords = synthetic.config2(nsys=6, seed=13)
     gpos, frc, nrg = MODES["double"]

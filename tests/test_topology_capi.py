@@ -1,5 +1,6 @@
 """Host-side logic: topology intake and output, the synthetic generator, the C-ABI library's
 exports and the work-unit planner (no device calls)."""
+import dataclasses
 import math
 import os
 
@@ -115,6 +116,19 @@ def test_plan_rejects_oversize_systems_and_bad_models():
         t2, _ = synthetic.branched_molecule(20, seed=3, gb=False)
         t3, _ = synthetic.branched_molecule(20, seed=4, gb=True)
         S.plan_synthesis([t2, t3])
+
+
+def test_plan_rejects_fractional_dihedral_periodicity_and_bad_blocks():
+    topo, _ = synthetic.branched_molecule(30, seed=2)
+    assert S.plan_synthesis([topo]).valence_term_count > 0
+    frac = dataclasses.replace(topo, dihe_freq=np.where(np.arange(len(topo.dihe_freq)) == 0, 2.5,
+                                                        topo.dihe_freq))
+    with pytest.raises(_lib.StormmError, match="periodicity"):
+        S.plan_synthesis([frac])
+    with pytest.raises(_lib.StormmError, match="block_threads"):
+        S.plan_synthesis([topo], {"block_threads": 384})
+    for bt in (256, 512, 1024):
+        assert S.plan_synthesis([topo], {"block_threads": bt}).system_count == 1
 
 
 def test_table_and_combining_rule_systems_plan_together():

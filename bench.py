@@ -13,7 +13,16 @@ against the FP64 CPU oracle are measured on a 16-system subsample.  The CPU base
 oracle (the restatement of the reference's CPU path) on a bounded sample of the same systems with
 one thread per host core (up to 16).
 
+Two further workloads ride on the same line (skip them with --no-extra):
+  "minimization": 4096 systems, N ~ U[16, 128], starts perturbed by N(0, 0.3 A), 500 cycles
+    (50 steepest descent, then conjugate gradient) in FP32: the persistent single-launch
+    minimizer, one persistent launch per size class, and the reference's per-step launch
+    sequence, with the oracle's minimizer on host threads as the CPU rate.
+  "large_batch": 16384 systems, N ~ U[32, 96]: FP32 and FP64 evaluation throughput and the
+    errors of both modes against the FP64 oracle on a 64-system subsample.
+
   python bench.py [--gpus 1] [--steps 500] [--warmup 50] [--cpu-seconds 12] [--no-cpu]
+                  [--no-extra]
 """
 import argparse
 import json
@@ -56,6 +65,96 @@ def parity_errors(S, oracle, topos, coords, prec, gpos, frc, nrg):
             "force_rel_rms": worst_f}
 
 
+def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu):
+    """FP32 minimization of the 4096-system batch: three launch strategies and the CPU rate."""
+    topos, coords = synthetic.config2(nsys=nsys, seed=2)
+    sd, dx0 = 50, 0.01
+    mc = S.minimize_controls(cycles, sd, dx0, 0.0)
+
+    def setup(sub_t, sub_c):
+        ag = S.AtomGraphSynthesis(sub_t)
+        ps = S.PhaseSpaceSynthesis(ag, 32, 23)
+        sc = S.ScoreCard(ag.system_count, 26)
+        lm = S.LineMinimization(ag.system_count, dx0)
+        ps.upload(sub_c)
+        return ag, ps, sc, lm
+
+    def run(parts, persistent):
+        S.synchronize()
+        t0 = time.perf_counter()
+        for ag, ps, sc, lm in parts:
+            S.launch_minimization("single", ag, ps, mc, sc, lm, persistent=persistent)
+        S.synchronize()
+        return time.perf_counter() - t0
+
+    # Warm-up on a few systems (kernel loading).
+    wag, wps, wsc, wlm = setup(topos[:8], coords[:8])
+    for persistent in (True, False):
+        S.launch_minimization("single", wag, wps, S.minimize_controls(3, 1, dx0, 0.0), wsc, wlm,
+                              persistent=persistent)
+    whole = [setup(topos, coords)]
+    t_pers = run(whole, True)
+    e_pers = whole[0][2].potential_energies()
+    whole = [setup(topos, coords)]
+    t_step = run(whole, False)
+    e_step = whole[0][2].potential_energies()
+    # One persistent launch per size class (atom-count quartiles of the range).
+    edges = [16, 44, 72, 100, 129]
+    parts = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        idx = [i for i, t in enumerate(topos) if lo <= t.natom < hi]
+        if idx:
+            parts.append(setup([topos[i] for i in idx], [coords[i] for i in idx]))
+    t_class = run(parts, True)
+    out = {"systems": nsys, "cycles": cycles, "sd_cycles": sd, "precision": "fp32",
+           "persistent_s": t_pers, "per_size_class_s": t_class, "per_step_launches_s": t_step,
+           "minimizations_per_s": nsys / t_pers,
+           "persistent_vs_per_step": t_step / t_pers,
+           "persistent_equals_per_step": bool(np.array_equal(e_pers, e_step)),
+           "size_classes": len(parts)}
+    if with_cpu:
+        from concurrent.futures import ThreadPoolExecutor
+        nsample = 128
+        threads = max(1, min(16, os.cpu_count() or 1))
+        fixed = [np.rint(np.asarray(c) * 2.0 ** 32).astype(np.int64) for c in coords[:nsample]]
+
+        def one(i):
+            oracle.minimize(topos[i], fixed[i], 32, 23, 26, cycles, sd, dx0, dp_mode=False)
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(one, range(nsample)))
+        secs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": nsample / secs, "unit": "minimizations/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"first {nsample} of the {nsys} systems, {cycles} "
+                                         f"cycles each, {threads} threads, {secs:.1f} s"}
+    return out
+
+
+def large_batch_workload(S, synthetic, oracle, nsys, reps):
+    """16384-system evaluation throughput in both precision modes and their oracle errors."""
+    topos, coords = synthetic.config4(nsys=nsys, seed=4)
+    out = {"systems": nsys}
+    for prec, bits in (("single", (32, 23, 26)), ("double", (32, 40, 32))):
+        ag = S.AtomGraphSynthesis(topos)
+        ps = S.PhaseSpaceSynthesis(ag, bits[0], bits[1])
+        sc = S.ScoreCard(ag.system_count, bits[2])
+        ps.upload(coords)
+        S.time_mm_evaluation(prec, ag, ps, sc, True, 3)
+        ms = S.time_mm_evaluation(prec, ag, ps, sc, True, reps)
+        out["fp32" if prec == "single" else "fp64"] = {
+            "kernel_ms": ms, "system_evals_per_s": nsys / (ms * 1.0e-3),
+            "force_bits": bits[1], "energy_bits": bits[2]}
+        del ag, ps, sc
+    sub = slice(0, 64)
+    out["errors"] = {
+        "fp32": parity_errors(S, oracle, topos[sub], coords[sub], "single", 32, 23, 26),
+        "fp64": parity_errors(S, oracle, topos[sub], coords[sub], "double", 32, 40, 32),
+        "sample": "first 64 of the 16384 systems, oracle at the device coordinates"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +164,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-parity", action="store_true", help="skip the oracle error check")
     ap.add_argument("--systems", type=int, default=1024)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the minimization and large-batch workloads")
+    ap.add_argument("--min-cycles", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this path runs every system of a batch on a single "
@@ -151,6 +253,16 @@ def main():
                          f"{secs:.1f} s"}
         log(f"CPU oracle: {cpu_value:.4e} system-evals/s on {threads} threads")
 
+    extra = {}
+    if not args.no_extra:
+        import oracle
+        extra["minimization"] = minimization_workload(S, synthetic, oracle, 4096,
+                                                      args.min_cycles, not args.no_cpu)
+        log(f"minimization: {extra['minimization']}")
+        extra["large_batch"] = large_batch_workload(S, synthetic, oracle, 16384,
+                                                    max(20, args.steps // 5))
+        log(f"large batch: {extra['large_batch']}")
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -197,6 +309,7 @@ def main():
         "parity": parity,
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

@@ -78,8 +78,8 @@ typedef struct {
   const double *charge;            /* [sum atoms] e                                             */
   const int32_t *lj_type;          /* [sum atoms] type index local to the system's table        */
   const int32_t *lj_type_counts;   /* [nsys] T                                                  */
-  const double *lja, *ljb;         /* [sum T^2] A / B, row-major (ti * T + tj) per system        */
-  const double *lja14, *ljb14;     /* [sum T^2] 1:4 A / B                                       */
+  const double *lja, *ljb;         /* [sum T^2 over table systems] A / B, row-major (ti*T + tj)  */
+  const double *lja14, *ljb14;     /* [sum T^2 over table systems] 1:4 A / B                     */
   const double *pb_radii;          /* [sum atoms] intrinsic radii (A)                           */
   const double *gb_screen;         /* [sum atoms] screening factors                             */
   const int32_t *bond_counts;      /* [nsys]                                                    */
@@ -94,12 +94,20 @@ typedef struct {
   const int32_t *dihe_improper;    /* 0 proper, 1 improper                                      */
   const double *dihe_scee;         /* 1:4 electrostatic divisor; <= 0: no 1:4 pair on this term */
   const double *dihe_scnb;         /* 1:4 Lennard-Jones divisor                                 */
-  const int32_t *exclusion_counts; /* [nsys] number of excluded (i < j) pairs                   */
-  const int32_t *exclusion_pairs;  /* [2 * sum exclusions]                                      */
+  const int32_t *exclusion_counts; /* [nsys] number of excluded (i < j) pairs; null: the pairs   */
+  const int32_t *exclusion_pairs;  /* within three bonds [2 * sum exclusions]                   */
   int32_t igb;                     /* one implicit-solvent model for the whole synthesis        */
   double dielectric;
   double salt_concentration;
   double coulomb_constant;
+  /* Optional compact Lennard-Jones form (no counterpart in the reference, whose AtomGraph always
+   * holds the T x T tables).  lj_combining[s] = 1: system s follows Lorentz-Berthelot combining
+   * of per-type sigma / epsilon given in lj_sigma / lj_epsilon ([sum T over such systems]), the
+   * 1:4 pairs use the same coefficients, and the system has no block in lja..ljb14.  Null
+   * lj_combining: every system sends its tables. */
+  const int32_t *lj_combining;     /* [nsys] or null                                            */
+  const double *lj_sigma;          /* A                                                         */
+  const double *lj_epsilon;        /* kcal/mol                                                  */
 } stormm_topology_batch;
 
 /* Work-unit shaping, chosen by the autotuner (or fixed by the caller).  Zero = default. */

@@ -129,10 +129,11 @@ class SystemHandle:
         s.charge = _ptr(f64(topo.charge))
         s.n_lj_types = topo.n_lj_types
         s.lj_idx = _ptr(i32(topo.lj_idx))
-        s.lja = _ptr(f64(topo.lja))
-        s.ljb = _ptr(f64(topo.ljb))
-        s.lja14 = _ptr(f64(topo.lja14))
-        s.ljb14 = _ptr(f64(topo.ljb14))
+        lja, ljb, lja14, ljb14 = topo.lj_tables()
+        s.lja = _ptr(f64(lja))
+        s.ljb = _ptr(f64(ljb))
+        s.lja14 = _ptr(f64(lja14))
+        s.ljb14 = _ptr(f64(ljb14))
         ex = np.ascontiguousarray(topo.exclusion_matrix(), dtype=np.uint8)
         keep.append(ex)
         s.excl = _ptr(ex)

@@ -41,7 +41,8 @@ class TopologyBatch(ctypes.Structure):
         ("dihe_amp", vp), ("dihe_phase", vp), ("dihe_freq", vp), ("dihe_improper", vp),
         ("dihe_scee", vp), ("dihe_scnb", vp), ("exclusion_counts", vp), ("exclusion_pairs", vp),
         ("igb", c_int32), ("dielectric", c_double), ("salt_concentration", c_double),
-        ("coulomb_constant", c_double),
+        ("coulomb_constant", c_double), ("lj_combining", vp), ("lj_sigma", vp),
+        ("lj_epsilon", vp),
     ]
 
 

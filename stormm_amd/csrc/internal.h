@@ -48,32 +48,45 @@ enum NrgSlot : int {
 // Term selection for the fused kernel.
 enum TermMask : int { kTermNonbonded = 1, kTermValence = 2 };
 
-// Valence term kinds packed in VTerm.x bits 0..3; the work-unit system slot sits in bits 8..15.
+// Valence term kinds (VTerm.x).
 enum VKind : int { kVBond = 0, kVAngle = 1, kVDihedral = 2 };
 
-// Work-unit descriptor, 8 ints.
+// Work-unit descriptor, 8 ints.  The unit's tile tasks are the off-diagonal tasks of all of its
+// systems followed by their diagonal tasks (so that wavefronts given only diagonal tiles run the
+// short rotation); its valence terms are those of its systems in order.
 struct WuDesc {
   int sys_first;     // index into the work-unit system list
   int sys_count;
-  int task_first;    // index into the task list
-  int task_count;
+  int offd_count;    // off-diagonal tile tasks of all systems
+  int task_count;    // all tile tasks
   int lds_atoms;     // atoms staged in LDS (each system padded to kTile)
-  int term_first;    // index into the valence term list
-  int term_count;
+  int term_count;    // valence terms of all systems
   int lj_table;      // 1 = systems in this unit read their stored LJ tables
+  int pad;
 };
 
-// One system inside a work unit.
+// One system inside a work unit and the ranges of its topology template: the template's tile
+// tasks (off-diagonal first, then diagonal) and valence terms carry atom indices relative to the
+// system's first atom, so every system with the same topology shares one template.
 struct WuSys {
-  int sys_index;
-  int lds_base;
-  int global_start;
+  int sys_index;     // system number: ScoreCard row and per-system minimizer state
+  int lds_base;      // first LDS atom of the system in the unit
+  int global_start;  // first atom in the global per-atom arrays
   int natom;
+  int task_first;    // template tile tasks
+  int offd_count;
+  int diag_count;
+  int term_first;    // template valence terms
+  int term_count;
+  int lj_off;        // stored LJ table (offset, type count)
+  int lj_nt;
+  int pad;
 };
 
-// One 16 x 16 tile task: x = LDS base of the i tile | (LDS base of the j tile << 16);
-// y = system slot | (diagonal << 8) | (ni << 12) | (nj << 20); z = exclusion-mask offset (-1 when
-// the tile holds no excluded pair); w = unused.
+// One 16 x 16 tile task of a template: x = first atom of the i tile | (first atom of the j tile
+// << 16), relative to the system; y = (diagonal << 8) | (ni << 12) | (nj << 20); z =
+// exclusion-mask offset (-1 when the tile holds no excluded pair); w = unused.  When a work unit
+// fetches a task it adds the system's LDS base to x and its slot in the unit to y bits 0..7.
 using Task = int4;
 
 // Everything the fused kernel reads, passed by value.
@@ -108,7 +121,6 @@ struct MMKit {
   const int *lj_type;        // type index (table systems)
   const double2 *ljab;       // A, B per system table
   const double2 *ljab14;     // 1:4 A, B per system table
-  const int *lj_off, *lj_nt; // per-system table offset and type count
   const double *rho;         // intrinsic radius - offset
   const double *scr;         // screening factor * rho
   const double *pbr;         // intrinsic radius

@@ -190,16 +190,16 @@ template <typename T> struct WuLds {
   crd_t *cx, *cy, *cz;
   int *gidx, *slot;
   WuSys *sys;
-  int2 *lj;        // per system slot: table offset, type count
-  unsigned *nearq; // kNearQueue packed pair entries per wavefront
+  int *opre, *dpre, *vpre;  // per slot: first off-diagonal task, diagonal task, valence term
+  unsigned *nearq;          // kNearQueue packed pair entries per wavefront
 
   static __host__ __device__ size_t bytes(int maxa, int waves) {
     const size_t m = static_cast<size_t>(maxa);
     return (5 * m * sizeof(long long)) + (kMaxWuSystems * 16 * sizeof(long long)) +
            (3 * kMaxWuSystems * sizeof(long long)) + (8 * m * sizeof(T)) +
            (3 * m * sizeof(crd_t)) + (2 * m * sizeof(int)) + (kMaxWuSystems * sizeof(WuSys)) +
-           (kMaxWuSystems * sizeof(int2)) + (static_cast<size_t>(waves) * kNearQueue *
-                                              sizeof(unsigned));
+           (3 * (kMaxWuSystems + 1) * sizeof(int)) + (static_cast<size_t>(waves) * kNearQueue *
+                                                     sizeof(unsigned));
   }
 
   __device__ WuLds(unsigned char *base, int maxa, int waves) {
@@ -231,7 +231,9 @@ template <typename T> struct WuLds {
     gidx = reinterpret_cast<int *>(take(m * sizeof(int)));
     slot = reinterpret_cast<int *>(take(m * sizeof(int)));
     sys = reinterpret_cast<WuSys *>(take(kMaxWuSystems * sizeof(WuSys)));
-    lj = reinterpret_cast<int2 *>(take(kMaxWuSystems * sizeof(int2)));
+    opre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
+    dpre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
+    vpre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
     nearq = reinterpret_cast<unsigned *>(take(static_cast<size_t>(waves) * kNearQueue *
                                               sizeof(unsigned)));
   }
@@ -242,6 +244,11 @@ __device__ __forceinline__ float crd_delta(int a, int b, float scale) {
   return static_cast<float>(a - b) * scale;
 }
 __device__ __forceinline__ double crd_delta(double a, double b, double) { return a - b; }
+
+// Displacement in LDS coordinate units (the exact integer difference, rounded once, in single
+// precision; Angstroms in double precision, whose coordinate unit is 1).
+__device__ __forceinline__ float crd_raw(int a, int b) { return static_cast<float>(a - b); }
+__device__ __forceinline__ double crd_raw(double a, double b) { return a - b; }
 
 // Conversion of a fixed-precision global coordinate to the LDS representation.
 template <typename T> struct CrdConv;

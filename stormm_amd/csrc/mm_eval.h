@@ -32,11 +32,23 @@ __device__ void stage_load_params(StageCtx<T> &c, const MMKit &k) {
   WuLds<T> &L = c.L;
   const WuDesc &wu = c.wu;
   if (c.tid < wu.sys_count) {
-    const WuSys ws = k.wu_sys[wu.sys_first + c.tid];
-    L.sys[c.tid] = ws;
-    L.lj[c.tid] = make_int2(k.lj_off[ws.sys_index], k.lj_nt[ws.sys_index]);
+    L.sys[c.tid] = k.wu_sys[wu.sys_first + c.tid];
   }
   __syncthreads();
+  if (c.tid == 0) {
+    int o = 0, d = 0, v = 0;
+    for (int s = 0; s < wu.sys_count; s++) {
+      L.opre[s] = o;
+      L.dpre[s] = d;
+      L.vpre[s] = v;
+      o += L.sys[s].offd_count;
+      d += L.sys[s].diag_count;
+      v += L.sys[s].term_count;
+    }
+    L.opre[wu.sys_count] = o;
+    L.dpre[wu.sys_count] = d;
+    L.vpre[wu.sys_count] = v;
+  }
   for (int a = c.tid; a < wu.lds_atoms; a += c.nthr) {
     int s = 0;
     for (int t = 1; t < wu.sys_count; t++) {
@@ -138,11 +150,13 @@ __device__ void init_ctx(StageCtx<T> &c, const MMKit &k, int wu_index, int terms
   c.gb = (k.igb != 0) && ((terms & kTermNonbonded) != 0);
 }
 
-// Pair-validity inside a tile task at rotation step kstep for row lane il.
-__device__ __forceinline__ bool pair_valid(bool ivalid, int jl, int nj, bool diag, int kstep,
-                                           int il) {
-  const bool dok = (kstep >= 1) && ((kstep < 8) || ((kstep == 8) && (il < 8)));
-  return ivalid && (jl < nj) && (diag ? dok : true);
+// Rotation-step mask of a 16-bit mask over the j atoms of a tile: bit ks of the result is bit
+// (il - ks) mod 16 of jmask, the j atom lane il holds after ks rotations.  The reversed mask,
+// rotated right by 15 - il, puts those bits in step order.
+__device__ __forceinline__ unsigned step_mask(unsigned jmask, int il) {
+  const unsigned r = __brev(jmask) >> 16;
+  const int sh = (15 - il) & 15;
+  return ((r >> sh) | (r << (16 - sh))) & 0xffffu;
 }
 
 // Number of rotation steps the wave runs for its current tasks: 16 when any of its rows holds
@@ -152,14 +166,83 @@ __device__ __forceinline__ int wave_steps(bool diag) {
   return (offd != 0ULL) ? 16 : 9;
 }
 
+// A tile task as seen by one lane of its 16-lane row.  Bit ks of `valid` is set when the lane's
+// pair at rotation step ks exists: both atoms are present, and a diagonal tile takes each pair
+// once (steps 1..7 on every lane, step 8 on lanes 0..7).
+struct TileTask {
+  int ib, jb, slot, ni, nj, i, j0, nsteps;
+  bool diag, ivalid;
+  unsigned valid;
+  __device__ __forceinline__ TileTask(const Task &tk, int il) {
+    ib = tk.x & 0xffff;
+    jb = tk.x >> 16;
+    slot = tk.y & 0xff;
+    diag = ((tk.y >> 8) & 1) != 0;
+    ni = (tk.y >> 12) & 0xff;
+    nj = (tk.y >> 20) & 0xff;
+    ivalid = il < ni;
+    i = ib + il;
+    j0 = jb + il;
+    const unsigned jm = (nj >= 16) ? 0xffffu : ((1u << nj) - 1u);
+    unsigned v = step_mask(jm, il);
+    if (diag) {
+      v &= (il < 8) ? 0x1feu : 0xfeu;
+    }
+    valid = ivalid ? v : 0u;
+    nsteps = wave_steps(diag);
+  }
+};
+
+__device__ __forceinline__ bool step_bit(unsigned mask, int ks) {
+  return ((mask >> ks) & 1u) != 0u;
+}
+
 // ------------------------------------------------------------------------------------------
+// Slot s of the unit's system that holds item t, given ascending per-slot starts pre[0..n]
+// (pre[0] = 0 <= t < pre[n]): the s with pre[s] <= t < pre[s + 1].
+__device__ __forceinline__ int find_slot(const int *pre, int n, int t) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= t) {
+      lo = mid;
+    }
+    else {
+      hi = mid;
+    }
+  }
+  return lo;
+}
+
+// Tile task t of the work unit, placed at its system's LDS base and tagged with its slot.
+template <typename T>
+__device__ __forceinline__ Task fetch_task(const StageCtx<T> &c, const MMKit &k, int t) {
+  const WuLds<T> &L = c.L;
+  int s, idx;
+  if (t < c.wu.offd_count) {
+    s = find_slot(L.opre, c.wu.sys_count, t);
+    idx = L.sys[s].task_first + (t - L.opre[s]);
+  }
+  else {
+    const int td = t - c.wu.offd_count;
+    s = find_slot(L.dpre, c.wu.sys_count, td);
+    idx = L.sys[s].task_first + L.sys[s].offd_count + (td - L.dpre[s]);
+  }
+  Task tk = k.tasks[idx];
+  const int base = L.sys[s].lds_base;
+  tk.x += base | (base << 16);
+  tk.y |= s;
+  return tk;
+}
+
 // Task dealing for the Born-integral stages: pass tb gives the four 16-lane rows of a
 // wavefront four consecutive tasks; rows past the end of the list get an empty diagonal task,
 // so every lane of a wavefront runs the same passes and rotation steps (the near queue below
 // needs the whole wavefront at each step).
-__device__ __forceinline__ Task wave_task(const MMKit &k, const WuDesc &wu, int t) {
-  if (t < wu.task_count) {
-    return k.tasks[wu.task_first + t];
+template <typename T>
+__device__ __forceinline__ Task wave_task(const StageCtx<T> &c, const MMKit &k, int t) {
+  if (t < c.wu.task_count) {
+    return fetch_task(c, k, t);
   }
   return make_int4(0, 1 << 8, -1, 0);
 }
@@ -240,6 +323,8 @@ __device__ __forceinline__ void psi_drain(StageCtx<T> &c, const unsigned *q, int
 // Born-radius integrals (src/Potential/nonbonded_potential.tpp:560-650).  Each lane evaluates
 // the far-field series of its pair at every rotation step and queues the pair when either
 // direction is within 4 screened radii; a full wavefront of queued pairs is evaluated at once.
+// The series terms of one task are summed in the working precision (at most 16 small terms per
+// lane) and added to the fixed-precision accumulators once per task.
 template <typename T>
 __device__ void stage_psi(StageCtx<T> &c, const MMKit &k) {
   using P = Prec<T>;
@@ -250,40 +335,38 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k) {
   const int nwaves = c.nthr >> 6;
   unsigned *q = L.nearq + (wave * kNearQueue);
   int qn = 0;
+  const T cs2 = c.crd_scale * c.crd_scale;
   const double psi_scale = ldexp(1.0, kPsiBits);
   for (int tb = 4 * wave; tb < c.wu.task_count; tb += 4 * nwaves) {
-    const Task tk = wave_task(k, c.wu, tb + ((c.tid >> 4) & 3));
-    const int ib = tk.x & 0xffff, jb = tk.x >> 16;
-    const bool diag = (tk.y >> 8) & 1;
-    const int ni = (tk.y >> 12) & 0xff, nj = (tk.y >> 20) & 0xff;
-    const bool ivalid = il < ni;
-    const int i = ib + il;
-    const int j0 = jb + il;
-    const auto xi = L.cx[i], yi = L.cy[i], zi = L.cz[i];
-    const T si = L.scr[i];
+    const TileTask tt(wave_task(c, k, tb + ((c.tid >> 4) & 3)), il);
+    const auto xi = L.cx[tt.i], yi = L.cy[tt.i], zi = L.cz[tt.i];
+    const T si = L.scr[tt.i];
     const T si4 = T(4) * si;
-    auto xj = L.cx[j0], yj = L.cy[j0], zj = L.cz[j0];
-    T sj = L.scr[j0];
-    // The far-field series is summed in double precision.
-    double acci = 0.0, accj = 0.0;
-    const int nsteps = wave_steps(diag);
-    for (int ks = 0; ks < nsteps; ks++) {
-      const int jl = (il - ks) & 15;
-      const bool valid = pair_valid(ivalid, jl, nj, diag, ks, il);
-      const T dx = crd_delta(xj, xi, c.crd_scale);
-      const T dy = crd_delta(yj, yi, c.crd_scale);
-      const T dz = crd_delta(zj, zi, c.crd_scale);
-      const T r2 = (dx * dx) + (dy * dy) + (dz * dz);
+    auto xj = L.cx[tt.j0], yj = L.cy[tt.j0], zj = L.cz[tt.j0];
+    T sj = L.scr[tt.j0];
+    // Queue entry bits of the pair: i in bits 0..8, the rotating j atom in bits 9..17.
+    const unsigned ei = static_cast<unsigned>(tt.i);
+    unsigned ej = static_cast<unsigned>(tt.j0) << 9;
+    T acci = T(0), accj = T(0);
+    for (int ks = 0; ks < tt.nsteps; ks++) {
+      const bool valid = step_bit(tt.valid, ks);
+      const T dx = crd_raw(xj, xi);
+      const T dy = crd_raw(yj, yi);
+      const T dz = crd_raw(zj, zi);
+      const T r2 = valid ? ((dx * dx) + (dy * dy) + (dz * dz)) * cs2 : T(1);
       const T invr = P::rsqrt(r2);
       const T invr2 = invr * invr;
       const T r = r2 * invr;
       const bool far_i = r > T(4) * sj;
       const bool far_j = r > si4;
-      acci += (valid && far_i) ? static_cast<double>(psi_far(sj, invr2)) : 0.0;
-      accj += (valid && far_j) ? static_cast<double>(psi_far(si, invr2)) : 0.0;
+      const T ti = psi_far(sj, invr2);
+      const T tj = psi_far(si, invr2);
+      acci += (valid && far_i) ? ti : T(0);
+      accj += (valid && far_j) ? tj : T(0);
       const bool near_i = valid && !far_i;
       const bool near_j = valid && !far_j;
-      nearq_push(q, qn, near_i || near_j, near_entry(i, jb + jl, near_i, near_j));
+      nearq_push(q, qn, near_i || near_j,
+                 ei | ej | (near_i ? kNearI : 0u) | (near_j ? kNearJ : 0u));
       if (qn >= 64) {
         psi_drain(c, q, 64, lane);
         nearq_shift(q, qn, lane);
@@ -293,13 +376,15 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k) {
       zj = row_rotate(zj);
       sj = row_rotate(sj);
       accj = row_rotate(accj);
+      ej = static_cast<unsigned>(row_rotate(static_cast<int>(ej)));
     }
-    if (ivalid) {
-      lds_add(&L.psi[i], Prec<double>::fixed(acci * psi_scale));
+    if (tt.ivalid) {
+      lds_add(&L.psi[tt.i], Prec<double>::fixed(static_cast<double>(acci) * psi_scale));
     }
-    const int jl_home = (il - nsteps) & 15;
-    if (jl_home < nj) {
-      lds_add(&L.psi[jb + jl_home], Prec<double>::fixed(accj * psi_scale));
+    const int jl_home = (il - tt.nsteps) & 15;
+    if (jl_home < tt.nj) {
+      lds_add(&L.psi[tt.jb + jl_home], Prec<double>::fixed(static_cast<double>(accj) *
+                                                           psi_scale));
     }
   }
   psi_drain(c, q, qn, lane);
@@ -353,114 +438,110 @@ __device__ void stage_radii(StageCtx<T> &c, const MMKit &k, double coulomb_inv_s
 
 // ------------------------------------------------------------------------------------------
 // Coulomb + Lennard-Jones (src/Potential/nonbonded_potential.tpp:174-231,312-373) and the GB
-// pair energy, force and radius derivative (:811-972).
-template <typename T, bool FORCES, bool LJTABLE>
-__device__ void stage_pairs(StageCtx<T> &c, const MMKit &k, bool do_nb) {
+// pair energy, force and radius derivative (:811-972).  Every lane evaluates its pair at every
+// step; absent and excluded pairs are removed by zeroing the charge product and the LJ
+// coefficients, and the squared distance of an absent pair is replaced by 1 to keep the
+// arithmetic finite.  Forces are accumulated in LDS coordinate units and scaled once per task.
+template <typename T, bool FORCES, bool LJTABLE, bool GB>
+__device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
   using P = Prec<T>;
   WuLds<T> &L = c.L;
   const int row = c.tid >> 4;
   const int nrows = c.nthr >> 4;
   const int il = c.tid & 15;
-  const T frc_scale = T(ldexp(1.0, k.frc_bits));
+  const T cs2 = c.crd_scale * c.crd_scale;
+  const T frc_scale = T(ldexp(1.0, k.frc_bits)) * c.crd_scale;
   const T dedr_scale = T(ldexp(1.0, kDedrBits));
   const double nrg_scale = ldexp(1.0, k.nrg_bits);
   const T inv_diel = T(1.0 / k.dielectric);
   const T kap = T(k.kappa * kGbKscale);
   const bool use_kappa = (k.kappa != 0.0);
-  const bool gb = c.gb;
   for (int t = row; t < c.wu.task_count; t += nrows) {
-    const Task tk = k.tasks[c.wu.task_first + t];
-    const int ib = tk.x & 0xffff, jb = tk.x >> 16;
-    const int slot = tk.y & 0xff;
-    const bool diag = (tk.y >> 8) & 1;
-    const int ni = (tk.y >> 12) & 0xff, nj = (tk.y >> 20) & 0xff;
-    const bool ivalid = il < ni;
-    const int i = ib + il;
-    const int j0 = jb + il;
-    const unsigned emask = (tk.z >= 0 && ivalid) ? k.masks[tk.z + il] : 0u;
-    const auto xi = L.cx[i], yi = L.cy[i], zi = L.cz[i];
-    const T qi = L.q[i], p1i = L.p1[i], p2i = L.p2[i], Ri = L.rad[i];
-    auto xj = L.cx[j0], yj = L.cy[j0], zj = L.cz[j0];
-    T qj = L.q[j0], p1j = L.p1[j0], p2j = L.p2[j0], Rj = L.rad[j0];
+    const Task tk = fetch_task(c, k, t);
+    const TileTask tt(tk, il);
+    const unsigned emask = (tk.z >= 0 && tt.ivalid) ? k.masks[tk.z + il] : 0u;
+    const unsigned nbmask = tt.valid & ~step_mask(emask, il);
+    const auto xi = L.cx[tt.i], yi = L.cy[tt.i], zi = L.cz[tt.i];
+    const T qi = L.q[tt.i], p1i = L.p1[tt.i], p2i = L.p2[tt.i], Ri = L.rad[tt.i];
+    auto xj = L.cx[tt.j0], yj = L.cy[tt.j0], zj = L.cz[tt.j0];
+    T qj = L.q[tt.j0], p1j = L.p1[tt.j0], p2j = L.p2[tt.j0], Rj = L.rad[tt.j0];
     T fxi = T(0), fyi = T(0), fzi = T(0), dri = T(0);
     T fxj = T(0), fyj = T(0), fzj = T(0), drj = T(0);
     // Energies are summed in double precision (one conversion and add per term and pair).
     double e_qq = 0.0, e_lj = 0.0, e_gb = 0.0;
     int lj_off = 0, lj_nt = 0;
     if (LJTABLE) {
-      lj_off = L.lj[slot].x;
-      lj_nt = L.lj[slot].y;
+      lj_off = L.sys[tt.slot].lj_off;
+      lj_nt = L.sys[tt.slot].lj_nt;
     }
-    const int nsteps = wave_steps(diag);
-    for (int ks = 0; ks < nsteps; ks++) {
-      const int jl = (il - ks) & 15;
-      if (pair_valid(ivalid, jl, nj, diag, ks, il)) {
-        const T dx = crd_delta(xj, xi, c.crd_scale);
-        const T dy = crd_delta(yj, yi, c.crd_scale);
-        const T dz = crd_delta(zj, zi, c.crd_scale);
-        const T r2 = (dx * dx) + (dy * dy) + (dz * dz);
-        const T qiqj = qi * qj;
-        T fmag = T(0);
-        if (gb) {
-          const T rr = Ri * Rj;
-          const T efac = P::exp(-T(0.25) * r2 * P::rcp(rr));
-          const T fgb2 = r2 + (rr * efac);
-          const T fgbi = P::rsqrt(fgb2);
-          T dielfac, temp6;
-          if (use_kappa) {
-            const T fgbk = -kap * fgb2 * fgbi;
-            const T expmkf = P::exp(fgbk) * inv_diel;
-            dielfac = T(1) - expmkf;
-            temp6 = qiqj * fgbi * fgbi * fgbi * (dielfac + (fgbk * expmkf));
-          }
-          else {
-            dielfac = T(1) - inv_diel;
-            temp6 = qiqj * fgbi * fgbi * fgbi * dielfac;
-          }
-          e_gb -= static_cast<double>(qiqj * dielfac * fgbi);
-          if (FORCES) {
-            fmag += temp6 * (T(1) - (T(0.25) * efac));
-            const T temp5 = T(0.5) * efac * temp6 * (rr + (T(0.25) * r2));
-            dri += Ri * temp5;
-            drj += Rj * temp5;
-          }
+    for (int ks = 0; ks < tt.nsteps; ks++) {
+      const bool valid = step_bit(tt.valid, ks);
+      const bool nbok = step_bit(nbmask, ks);
+      const T dx = crd_raw(xj, xi);
+      const T dy = crd_raw(yj, yi);
+      const T dz = crd_raw(zj, zi);
+      const T r2 = valid ? ((dx * dx) + (dy * dy) + (dz * dz)) * cs2 : T(1);
+      const T qiqj = qi * qj;
+      T fmag = T(0);
+      if (GB) {
+        const T qgb = valid ? qiqj : T(0);
+        const T rr = Ri * Rj;
+        const T efac = P::exp(-T(0.25) * r2 * P::rcp(rr));
+        const T fgb2 = r2 + (rr * efac);
+        const T fgbi = P::rsqrt(fgb2);
+        T dielfac, temp6;
+        if (use_kappa) {
+          const T fgbk = -kap * fgb2 * fgbi;
+          const T expmkf = P::exp(fgbk) * inv_diel;
+          dielfac = T(1) - expmkf;
+          temp6 = qgb * fgbi * fgbi * fgbi * (dielfac + (fgbk * expmkf));
         }
-        if (do_nb && !((emask >> jl) & 1u)) {
-          const T invr = P::rsqrt(r2);
-          const T invr2 = invr * invr;
-          e_qq += static_cast<double>(qiqj * invr);
-          if (LJTABLE) {
-            const double2 ab = k.ljab[lj_off + (static_cast<int>(p1i) * lj_nt) +
-                                      static_cast<int>(p1j)];
-            const T lja = static_cast<T>(ab.x), ljb = static_cast<T>(ab.y);
-            const T invr4 = invr2 * invr2;
-            e_lj += static_cast<double>(((lja * invr4 * invr4) - (ljb * invr2)) * invr4);
-            if (FORCES) {
-              fmag += ((T(6) * ljb) - (T(12) * lja * invr4 * invr2)) * invr4 * invr4;
-            }
-          }
-          else {
-            const T sig = T(0.5) * (p1i + p1j);
-            const T s2 = sig * sig * invr2;
-            const T s6 = s2 * s2 * s2;
-            const T e4 = p2i * p2j;
-            e_lj += static_cast<double>(e4 * ((s6 * s6) - s6));
-            if (FORCES) {
-              fmag += e4 * ((T(6) * s6) - (T(12) * s6 * s6)) * invr2;
-            }
-          }
-          if (FORCES) {
-            fmag -= qiqj * invr * invr2;
-          }
+        else {
+          dielfac = T(1) - inv_diel;
+          temp6 = qgb * fgbi * fgbi * fgbi * dielfac;
         }
+        e_gb -= static_cast<double>(qgb * dielfac * fgbi);
         if (FORCES) {
-          fxi += fmag * dx;
-          fyi += fmag * dy;
-          fzi += fmag * dz;
-          fxj -= fmag * dx;
-          fyj -= fmag * dy;
-          fzj -= fmag * dz;
+          fmag += temp6 * (T(1) - (T(0.25) * efac));
+          const T temp5 = T(0.5) * efac * temp6 * (rr + (T(0.25) * r2));
+          dri += Ri * temp5;
+          drj += Rj * temp5;
         }
+      }
+      const T qnb = nbok ? qiqj : T(0);
+      const T invr = P::rsqrt(r2);
+      const T invr2 = invr * invr;
+      const T eqq = qnb * invr;
+      e_qq += static_cast<double>(eqq);
+      if (LJTABLE) {
+        const double2 ab = k.ljab[lj_off + (static_cast<int>(p1i) * lj_nt) +
+                                  static_cast<int>(p1j)];
+        const T lja = nbok ? static_cast<T>(ab.x) : T(0);
+        const T ljb = nbok ? static_cast<T>(ab.y) : T(0);
+        const T invr4 = invr2 * invr2;
+        e_lj += static_cast<double>(((lja * invr4 * invr4) - (ljb * invr2)) * invr4);
+        if (FORCES) {
+          fmag += ((T(6) * ljb) - (T(12) * lja * invr4 * invr2)) * invr4 * invr4;
+        }
+      }
+      else {
+        const T sig = T(0.5) * (p1i + p1j);
+        const T s2 = sig * sig * invr2;
+        const T s6 = s2 * s2 * s2;
+        const T e4 = nbok ? p2i * p2j : T(0);
+        e_lj += static_cast<double>(e4 * ((s6 * s6) - s6));
+        if (FORCES) {
+          fmag += e4 * ((T(6) * s6) - (T(12) * s6 * s6)) * invr2;
+        }
+      }
+      if (FORCES) {
+        fmag -= eqq * invr2;
+        fxi += fmag * dx;
+        fyi += fmag * dy;
+        fzi += fmag * dz;
+        fxj -= fmag * dx;
+        fyj -= fmag * dy;
+        fzj -= fmag * dz;
       }
       xj = row_rotate(xj);
       yj = row_rotate(yj);
@@ -468,44 +549,46 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k, bool do_nb) {
       qj = row_rotate(qj);
       p1j = row_rotate(p1j);
       p2j = row_rotate(p2j);
-      Rj = row_rotate(Rj);
+      if (GB) {
+        Rj = row_rotate(Rj);
+      }
       if (FORCES) {
         fxj = row_rotate(fxj);
         fyj = row_rotate(fyj);
         fzj = row_rotate(fzj);
-        drj = row_rotate(drj);
+        if (GB) {
+          drj = row_rotate(drj);
+        }
       }
     }
     if (FORCES) {
-      if (ivalid) {
-        lds_add(&L.fx[i], P::fixed(fxi * frc_scale));
-        lds_add(&L.fy[i], P::fixed(fyi * frc_scale));
-        lds_add(&L.fz[i], P::fixed(fzi * frc_scale));
-        if (gb) {
-          lds_add(&L.dedr[i], P::fixed(dri * dedr_scale));
+      if (tt.ivalid) {
+        lds_add(&L.fx[tt.i], P::fixed(fxi * frc_scale));
+        lds_add(&L.fy[tt.i], P::fixed(fyi * frc_scale));
+        lds_add(&L.fz[tt.i], P::fixed(fzi * frc_scale));
+        if (GB) {
+          lds_add(&L.dedr[tt.i], P::fixed(dri * dedr_scale));
         }
       }
-      const int jl_home = (il - nsteps) & 15;
-      if (jl_home < nj) {
-        const int j = jb + jl_home;
+      const int jl_home = (il - tt.nsteps) & 15;
+      if (jl_home < tt.nj) {
+        const int j = tt.jb + jl_home;
         lds_add(&L.fx[j], P::fixed(fxj * frc_scale));
         lds_add(&L.fy[j], P::fixed(fyj * frc_scale));
         lds_add(&L.fz[j], P::fixed(fzj * frc_scale));
-        if (gb) {
+        if (GB) {
           lds_add(&L.dedr[j], P::fixed(drj * dedr_scale));
         }
       }
     }
     const long long vqq = row_sum(Prec<double>::fixed(e_qq * nrg_scale));
     const long long vlj = row_sum(Prec<double>::fixed(e_lj * nrg_scale));
-    const long long vgb = row_sum(Prec<double>::fixed(e_gb * nrg_scale));
+    const long long vgb = GB ? row_sum(Prec<double>::fixed(e_gb * nrg_scale)) : 0LL;
     if (il == 0) {
-      if (do_nb) {
-        lds_add(&L.nrg[slot * 16 + kElec], vqq);
-        lds_add(&L.nrg[slot * 16 + kVdw], vlj);
-      }
-      if (gb) {
-        lds_add(&L.nrg[slot * 16 + kGb], vgb);
+      lds_add(&L.nrg[tt.slot * 16 + kElec], vqq);
+      lds_add(&L.nrg[tt.slot * 16 + kVdw], vlj);
+      if (GB) {
+        lds_add(&L.nrg[tt.slot * 16 + kGb], vgb);
       }
     }
   }
@@ -570,7 +653,8 @@ __device__ __forceinline__ void deriv_drain(StageCtx<T> &c, const unsigned *q, i
 }
 
 // Forces from the Born-radius derivatives, src/Potential/nonbonded_potential.tpp:1018-1292,
-// with the same far-field / queued near-field split as stage_psi.
+// with the same far-field / queued near-field split as stage_psi.  Far-field forces are
+// accumulated in LDS coordinate units and scaled once per task.
 template <typename T>
 __device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
   using P = Prec<T>;
@@ -581,30 +665,25 @@ __device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
   const int nwaves = c.nthr >> 6;
   unsigned *q = L.nearq + (wave * kNearQueue);
   int qn = 0;
-  const T frc_scale = T(ldexp(1.0, k.frc_bits));
+  const T cs2 = c.crd_scale * c.crd_scale;
+  const T frc_scale = T(ldexp(1.0, k.frc_bits)) * c.crd_scale;
   for (int tb = 4 * wave; tb < c.wu.task_count; tb += 4 * nwaves) {
-    const Task tk = wave_task(k, c.wu, tb + ((c.tid >> 4) & 3));
-    const int ib = tk.x & 0xffff, jb = tk.x >> 16;
-    const bool diag = (tk.y >> 8) & 1;
-    const int ni = (tk.y >> 12) & 0xff, nj = (tk.y >> 20) & 0xff;
-    const bool ivalid = il < ni;
-    const int i = ib + il;
-    const int j0 = jb + il;
-    const auto xi = L.cx[i], yi = L.cy[i], zi = L.cz[i];
-    const T si = L.scr[i], sdi = L.sd[i];
+    const TileTask tt(wave_task(c, k, tb + ((c.tid >> 4) & 3)), il);
+    const auto xi = L.cx[tt.i], yi = L.cy[tt.i], zi = L.cz[tt.i];
+    const T si = L.scr[tt.i], sdi = L.sd[tt.i];
     const T si4 = T(4) * si;
-    auto xj = L.cx[j0], yj = L.cy[j0], zj = L.cz[j0];
-    T sj = L.scr[j0], sdj = L.sd[j0];
+    auto xj = L.cx[tt.j0], yj = L.cy[tt.j0], zj = L.cz[tt.j0];
+    T sj = L.scr[tt.j0], sdj = L.sd[tt.j0];
+    const unsigned ei = static_cast<unsigned>(tt.i);
+    unsigned ej = static_cast<unsigned>(tt.j0) << 9;
     T fxi = T(0), fyi = T(0), fzi = T(0);
     T fxj = T(0), fyj = T(0), fzj = T(0);
-    const int nsteps = wave_steps(diag);
-    for (int ks = 0; ks < nsteps; ks++) {
-      const int jl = (il - ks) & 15;
-      const bool valid = pair_valid(ivalid, jl, nj, diag, ks, il);
-      const T dx = crd_delta(xj, xi, c.crd_scale);
-      const T dy = crd_delta(yj, yi, c.crd_scale);
-      const T dz = crd_delta(zj, zi, c.crd_scale);
-      const T r2 = (dx * dx) + (dy * dy) + (dz * dz);
+    for (int ks = 0; ks < tt.nsteps; ks++) {
+      const bool valid = step_bit(tt.valid, ks);
+      const T dx = crd_raw(xj, xi);
+      const T dy = crd_raw(yj, yi);
+      const T dz = crd_raw(zj, zi);
+      const T r2 = valid ? ((dx * dx) + (dy * dy) + (dz * dz)) * cs2 : T(1);
       const T invr = P::rsqrt(r2);
       const T invr2 = invr * invr;
       const T r = r2 * invr;
@@ -621,7 +700,8 @@ __device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
       fzj += fmag * dz;
       const bool near_i = valid && !far_i;
       const bool near_j = valid && !far_j;
-      nearq_push(q, qn, near_i || near_j, near_entry(i, jb + jl, near_i, near_j));
+      nearq_push(q, qn, near_i || near_j,
+                 ei | ej | (near_i ? kNearI : 0u) | (near_j ? kNearJ : 0u));
       if (qn >= 64) {
         deriv_drain(c, q, 64, lane, k.frc_bits);
         nearq_shift(q, qn, lane);
@@ -634,15 +714,16 @@ __device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
       fxj = row_rotate(fxj);
       fyj = row_rotate(fyj);
       fzj = row_rotate(fzj);
+      ej = static_cast<unsigned>(row_rotate(static_cast<int>(ej)));
     }
-    if (ivalid) {
-      lds_add(&L.fx[i], P::fixed(fxi * frc_scale));
-      lds_add(&L.fy[i], P::fixed(fyi * frc_scale));
-      lds_add(&L.fz[i], P::fixed(fzi * frc_scale));
+    if (tt.ivalid) {
+      lds_add(&L.fx[tt.i], P::fixed(fxi * frc_scale));
+      lds_add(&L.fy[tt.i], P::fixed(fyi * frc_scale));
+      lds_add(&L.fz[tt.i], P::fixed(fzi * frc_scale));
     }
-    const int jl_home = (il - nsteps) & 15;
-    if (jl_home < nj) {
-      const int j = jb + jl_home;
+    const int jl_home = (il - tt.nsteps) & 15;
+    if (jl_home < tt.nj) {
+      const int j = tt.jb + jl_home;
       lds_add(&L.fx[j], P::fixed(fxj * frc_scale));
       lds_add(&L.fy[j], P::fixed(fyj * frc_scale));
       lds_add(&L.fz[j], P::fixed(fzj * frc_scale));
@@ -864,11 +945,12 @@ __device__ void stage_valence(StageCtx<T> &c, const MMKit &k) {
   v.fy = c.L.fy;
   v.fz = c.L.fz;
   for (int t = c.tid; t < c.wu.term_count; t += c.nthr) {
-    const int4 vt = k.vterms[c.wu.term_first + t];
-    const int kind = vt.x & 0xf;
-    const int slot = (vt.x >> 8) & 0xff;
-    const int i = vt.y & 0xffff, j = vt.y >> 16;
-    const int kk = vt.z & 0xffff, l = vt.z >> 16;
+    const int slot = find_slot(c.L.vpre, c.wu.sys_count, t);
+    const WuSys &ws = c.L.sys[slot];
+    const int4 vt = k.vterms[ws.term_first + (t - c.L.vpre[slot])];
+    const int kind = vt.x;
+    const int i = ws.lds_base + (vt.y & 0xffff), j = ws.lds_base + (vt.y >> 16);
+    const int kk = ws.lds_base + (vt.z & 0xffff), l = ws.lds_base + (vt.z >> 16);
     long long *nrg = &c.L.nrg[slot * 16];
     if (kind == kVBond) {
       const double2 p = k.bond_par[vt.w];
@@ -886,7 +968,8 @@ __device__ void stage_valence(StageCtx<T> &c, const MMKit &k) {
       const double2 a14 = k.dihe14[vt.w];
       if (a14.x > 0.0) {
         double ele, vdw;
-        eval_14<FORCES, LJTABLE>(v, k, c.L.lj[slot], i, l, a14.x, a14.y, fscale, ele, vdw);
+        eval_14<FORCES, LJTABLE>(v, k, make_int2(ws.lj_off, ws.lj_nt), i, l, a14.x, a14.y,
+                                 fscale, ele, vdw);
         lds_add(&nrg[kElec14], fix64(ele * nrg_scale));
         lds_add(&nrg[kVdw14], fix64(vdw * nrg_scale));
       }
@@ -908,10 +991,20 @@ __device__ void evaluate_wu(StageCtx<T> &c, const MMKit &k, int terms) {
   }
   if (terms & kTermNonbonded) {
     if (lj_table) {
-      stage_pairs<T, FORCES, true>(c, k, true);
+      if (c.gb) {
+        stage_pairs<T, FORCES, true, true>(c, k);
+      }
+      else {
+        stage_pairs<T, FORCES, true, false>(c, k);
+      }
     }
     else {
-      stage_pairs<T, FORCES, false>(c, k, true);
+      if (c.gb) {
+        stage_pairs<T, FORCES, false, true>(c, k);
+      }
+      else {
+        stage_pairs<T, FORCES, false, false>(c, k);
+      }
     }
   }
   if (FORCES && c.gb) {

@@ -251,27 +251,25 @@ __global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
     }
     evaluate_wu<T, true>(c, k, terms);
 
-    // Convergence test and conjugate-gradient direction, one wave per system.
+    // Convergence test and conjugate-gradient direction, one wave per system.  The three
+    // Cartesian components are handled by the same code on their own arrays.
     for (int s = wave; s < nsys; s += nwaves) {
       const WuSys ws = c.L.sys[s];
-      long long *lf[3] = {c.L.fx, c.L.fy, c.L.fz};
-      long long *gf[3] = {k.xfrc, k.yfrc, k.zfrc};
-      long long *pv[3] = {mk.xprv, mk.yprv, mk.zprv};
-      long long *cg[3] = {mk.xcg, mk.ycg, mk.zcg};
       double ff = 0.0, gg = 0.0, dgg = 0.0;
-      for (int i = lane; i < ws.natom; i += 64) {
-        const int a = ws.lds_base + i;
-        const int g = ws.global_start + i;
-        for (int d = 0; d < 3; d++) {
-          const long long lfv = lf[d][a];
-          const long long lpv = pv[d][g];
+      auto sums = [&](const long long *lf, const long long *pv) {
+        for (int i = lane; i < ws.natom; i += 64) {
+          const long long lfv = lf[ws.lds_base + i];
+          const long long lpv = pv[ws.global_start + i];
           const double df = static_cast<double>(lfv);
           const double dp = static_cast<double>(lpv);
           ff += df * df;
           gg += dp * dp;
           dgg += static_cast<double>(lfv - lpv) * df;
         }
-      }
+      };
+      sums(c.L.fx, mk.xprv);
+      sums(c.L.fy, mk.yprv);
+      sums(c.L.fz, mk.zprv);
       ff = wave_sum(ff);
       gg = wave_sum(gg);
       dgg = wave_sum(dgg);
@@ -281,37 +279,43 @@ __global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
       if (!done) {
         const int step = cyc + 1;
         const double gam = (step >= mk.sd_cycles) ? (dgg / gg) : 0.0;
+        const bool use_gam = fabs(gam) > 1.0e-12;
         double msum = 0.0;
-        for (int i = lane; i < ws.natom; i += 64) {
-          const int a = ws.lds_base + i;
-          const int g = ws.global_start + i;
-          for (int d = 0; d < 3; d++) {
-            const long long ifv = lf[d][a];
-            pv[d][g] = ifv;
-            if (fabs(gam) > 1.0e-12) {
-              const double dfcg = static_cast<double>(ifv) + (gam * static_cast<double>(cg[d][g]));
+        auto direction = [&](const long long *lf, long long *pv, long long *cg, long long *gf) {
+          for (int i = lane; i < ws.natom; i += 64) {
+            const int g = ws.global_start + i;
+            const long long ifv = lf[ws.lds_base + i];
+            pv[g] = ifv;
+            if (use_gam) {
+              const double dfcg = static_cast<double>(ifv) + (gam * static_cast<double>(cg[g]));
               msum += dfcg * dfcg;
               const long long v = static_cast<long long>(__builtin_rint(dfcg));
-              cg[d][g] = v;
-              gf[d][g] = v;
+              cg[g] = v;
+              gf[g] = v;
             }
             else {
               msum += static_cast<double>(ifv) * static_cast<double>(ifv);
-              cg[d][g] = ifv;
-              gf[d][g] = ifv;
+              cg[g] = ifv;
+              gf[g] = ifv;
             }
           }
-        }
+        };
+        direction(c.L.fx, mk.xprv, mk.xcg, k.xfrc);
+        direction(c.L.fy, mk.yprv, mk.ycg, k.yfrc);
+        direction(c.L.fz, mk.zprv, mk.zcg, k.zfrc);
         msum = wave_sum(msum);
         const double inv_factor = 1.0 / (sqrt(msum) * inv_frc);
-        for (int i = lane; i < ws.natom; i += 64) {
-          const int g = ws.global_start + i;
-          for (int d = 0; d < 3; d++) {
-            const double v = static_cast<double>(gf[d][g]) * inv_factor;
-            gf[d][g] = dp_mode ? static_cast<long long>(__builtin_rint(v)) :
-                                 static_cast<long long>(v);
+        auto normalize = [&](long long *gf) {
+          for (int i = lane; i < ws.natom; i += 64) {
+            const int g = ws.global_start + i;
+            const double v = static_cast<double>(gf[g]) * inv_factor;
+            gf[g] = dp_mode ? static_cast<long long>(__builtin_rint(v)) :
+                              static_cast<long long>(v);
           }
-        }
+        };
+        normalize(k.xfrc);
+        normalize(k.yfrc);
+        normalize(k.zfrc);
       }
       if (lane == 0) {
         M->rms[s] = rms;

@@ -11,6 +11,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/stormm_amd.h"
@@ -73,6 +74,31 @@ struct DevMem {
 // ------------------------------------------------------------------------------------------
 // Host plan of a synthesis (no device memory)
 // ------------------------------------------------------------------------------------------
+// Offsets of one system's terms in the concatenated topology batch.
+struct SystemTerms {
+  int bond_first = 0, bond_count = 0;
+  int angl_first = 0, angl_count = 0;
+  int dihe_first = 0, dihe_count = 0;
+  int excl_first = 0, excl_count = 0;
+  int lj_types = 0;
+  bool combining = false;  // per-type sigma / epsilon given (Lorentz-Berthelot), no tables
+  int lj_first = 0;        // table block (T^2 entries) when !combining
+  int sig_first = 0;       // per-type parameters (T entries) when combining
+};
+
+// Work template of one topology: its tile tasks (off-diagonal, then diagonal), exclusion masks,
+// valence terms and parameters, with atom indices relative to the system, plus its LJ table.
+struct TopoTemplate {
+  int natom = 0;
+  int source_system = 0;  // a system of the batch with this topology
+  int task_first = 0, offd_count = 0, diag_count = 0;
+  int term_first = 0, term_count = 0;
+  int lj_off = 0, lj_nt = 0;
+  bool lb = false;        // the LJ table follows Lorentz-Berthelot combining
+  std::vector<double> sig, eps;
+  double cost = 0.0;      // estimated evaluation cost (rotation steps of the pair passes)
+};
+
 struct SynthesisPlan {
   int nsys = 0;
   int padded_atoms = 0;
@@ -84,8 +110,9 @@ struct SynthesisPlan {
   std::vector<double> q, ljp1, ljp2, rho, scr, pbr, alpha, beta, gamma;
   std::vector<int> lj_type;
   std::vector<double2> ljab, ljab14;
-  std::vector<int> lj_off, lj_nt;
   std::vector<char> lj_table_mode;
+  std::vector<TopoTemplate> templates;
+  std::vector<int> sys_template;
   std::vector<WuDesc> wu;
   std::vector<WuSys> wu_sys;
   std::vector<Task> tasks;
@@ -94,15 +121,9 @@ struct SynthesisPlan {
   std::vector<double2> bond_par, angl_par, dihe14;
   std::vector<DiheParam> dihe_par;
   long long pairs = 0;
+  long long task_count = 0;
   long long vterm_count = 0;
   int lj_table_systems = 0;
-};
-
-struct SystemTerms {
-  int bond_first = 0, bond_count = 0;
-  int angl_first = 0, angl_count = 0;
-  int dihe_first = 0, dihe_count = 0;
-  int excl_first = 0, excl_count = 0;
 };
 
 static int round_up(int v, int m) { return ((v + m - 1) / m) * m; }
@@ -132,7 +153,8 @@ static bool lorentz_berthelot(int t, const double *A, const double *B, const dou
     for (int j = 0; j < t; j++) {
       const double s = 0.5 * (sig[i] + sig[j]);
       const double e = std::sqrt(eps[i] * eps[j]);
-      const double s6 = std::pow(s, 6.0);
+      const double s2 = s * s;
+      const double s6 = s2 * s2 * s2;
       const double a_lb = 4.0 * e * s6 * s6;
       const double b_lb = 4.0 * e * s6;
       const double a = A[i * t + j], b = B[i * t + j];
@@ -153,6 +175,385 @@ static int cu_count_or_default() {
     return prop.multiProcessorCount;
   }
   return 256;
+}
+
+// FNV-1a over the parts of a system's topology that its work template depends on.
+struct Fnv {
+  unsigned long long h = 1469598103934665603ULL;
+  void bytes(const void *p, size_t n) {
+    const unsigned char *c = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < n; i++) {
+      h = (h ^ c[i]) * 1099511628211ULL;
+    }
+  }
+  template <typename T> void span(const T *p, size_t n) {
+    if (p != nullptr && n > 0) {
+      bytes(p, n * sizeof(T));
+    }
+  }
+};
+
+static unsigned long long topology_hash(const stormm_topology_batch *b, const SystemTerms &t,
+                                        int natom) {
+  Fnv f;
+  f.bytes(&natom, sizeof(natom));
+  f.bytes(&t.lj_types, sizeof(t.lj_types));
+  f.bytes(&t.combining, sizeof(t.combining));
+  f.span(b->exclusion_pairs + 2 * t.excl_first, 2 * static_cast<size_t>(t.excl_count));
+  f.span(b->bond_atoms + 2 * t.bond_first, 2 * static_cast<size_t>(t.bond_count));
+  f.span(b->bond_k + t.bond_first, t.bond_count);
+  f.span(b->bond_r0 + t.bond_first, t.bond_count);
+  f.span(b->angle_atoms + 3 * t.angl_first, 3 * static_cast<size_t>(t.angl_count));
+  f.span(b->angle_k + t.angl_first, t.angl_count);
+  f.span(b->angle_theta + t.angl_first, t.angl_count);
+  f.span(b->dihedral_atoms + 4 * t.dihe_first, 4 * static_cast<size_t>(t.dihe_count));
+  f.span(b->dihe_amp + t.dihe_first, t.dihe_count);
+  f.span(b->dihe_phase + t.dihe_first, t.dihe_count);
+  f.span(b->dihe_freq + t.dihe_first, t.dihe_count);
+  f.span(b->dihe_improper + t.dihe_first, t.dihe_count);
+  f.span(b->dihe_scee + t.dihe_first, t.dihe_count);
+  f.span(b->dihe_scnb + t.dihe_first, t.dihe_count);
+  if (t.combining) {
+    f.span(b->lj_sigma + t.sig_first, t.lj_types);
+    f.span(b->lj_epsilon + t.sig_first, t.lj_types);
+  }
+  else {
+    const size_t t2 = static_cast<size_t>(t.lj_types) * t.lj_types;
+    f.span(b->lja + t.lj_first, t2);
+    f.span(b->ljb + t.lj_first, t2);
+    f.span(b->lja14 + t.lj_first, t2);
+    f.span(b->ljb14 + t.lj_first, t2);
+  }
+  return f.h;
+}
+
+template <typename T>
+static bool same_span(const T *base, int first_a, int first_b, size_t n) {
+  if (n == 0 || base == nullptr) {
+    return true;
+  }
+  return std::memcmp(base + first_a, base + first_b, n * sizeof(T)) == 0;
+}
+
+static bool same_topology(const stormm_topology_batch *b, const SystemTerms &x, int nx,
+                          const SystemTerms &y, int ny) {
+  if (nx != ny || x.bond_count != y.bond_count || x.angl_count != y.angl_count ||
+      x.dihe_count != y.dihe_count || x.excl_count != y.excl_count ||
+      x.lj_types != y.lj_types || x.combining != y.combining) {
+    return false;
+  }
+  const size_t t2 = x.combining ? 0 : static_cast<size_t>(x.lj_types) * x.lj_types;
+  const size_t t1 = x.combining ? static_cast<size_t>(x.lj_types) : 0;
+  return same_span(b->lj_sigma, x.sig_first, y.sig_first, t1) &&
+         same_span(b->lj_epsilon, x.sig_first, y.sig_first, t1) &&
+         same_span(b->exclusion_pairs, 2 * x.excl_first, 2 * y.excl_first, 2 * x.excl_count) &&
+         same_span(b->bond_atoms, 2 * x.bond_first, 2 * y.bond_first, 2 * x.bond_count) &&
+         same_span(b->bond_k, x.bond_first, y.bond_first, x.bond_count) &&
+         same_span(b->bond_r0, x.bond_first, y.bond_first, x.bond_count) &&
+         same_span(b->angle_atoms, 3 * x.angl_first, 3 * y.angl_first, 3 * x.angl_count) &&
+         same_span(b->angle_k, x.angl_first, y.angl_first, x.angl_count) &&
+         same_span(b->angle_theta, x.angl_first, y.angl_first, x.angl_count) &&
+         same_span(b->dihedral_atoms, 4 * x.dihe_first, 4 * y.dihe_first, 4 * x.dihe_count) &&
+         same_span(b->dihe_amp, x.dihe_first, y.dihe_first, x.dihe_count) &&
+         same_span(b->dihe_phase, x.dihe_first, y.dihe_first, x.dihe_count) &&
+         same_span(b->dihe_freq, x.dihe_first, y.dihe_first, x.dihe_count) &&
+         same_span(b->dihe_improper, x.dihe_first, y.dihe_first, x.dihe_count) &&
+         same_span(b->dihe_scee, x.dihe_first, y.dihe_first, x.dihe_count) &&
+         same_span(b->dihe_scnb, x.dihe_first, y.dihe_first, x.dihe_count) &&
+         same_span(b->lja, x.lj_first, y.lj_first, t2) &&
+         same_span(b->ljb, x.lj_first, y.lj_first, t2) &&
+         same_span(b->lja14, x.lj_first, y.lj_first, t2) &&
+         same_span(b->ljb14, x.lj_first, y.lj_first, t2);
+}
+
+// Builds the work template of system s (tile tasks with exclusion masks, valence terms and
+// their parameters, the LJ table) and appends it to the plan.
+static int add_template(const stormm_topology_batch *b, const SystemTerms &t, int n, int s,
+                        SynthesisPlan &P) {
+  TopoTemplate tp;
+  tp.natom = n;
+  tp.source_system = s;
+  // Lennard-Jones: per-type Lorentz-Berthelot parameters, or a table that may follow the rule
+  // (then per-atom parameters replace it) or must be read as stored.
+  const int nt_lj = t.lj_types;
+  tp.lj_nt = nt_lj;
+  tp.lj_off = 0;
+  if (t.combining) {
+    tp.lb = true;
+    tp.sig.assign(b->lj_sigma + t.sig_first, b->lj_sigma + t.sig_first + nt_lj);
+    tp.eps.assign(b->lj_epsilon + t.sig_first, b->lj_epsilon + t.sig_first + nt_lj);
+  }
+  else {
+    const double *A = b->lja + t.lj_first, *B = b->ljb + t.lj_first;
+    const double *A14 = b->lja14 + t.lj_first, *B14 = b->ljb14 + t.lj_first;
+    tp.lb = lorentz_berthelot(nt_lj, A, B, A14, B14, tp.sig, tp.eps);
+    if (!tp.lb) {
+      tp.lj_off = static_cast<int>(P.ljab.size());
+      for (int k = 0; k < nt_lj * nt_lj; k++) {
+        P.ljab.push_back(make_double2(A[k], B[k]));
+        P.ljab14.push_back(make_double2(A14[k], B14[k]));
+      }
+    }
+  }
+  // Exclusion matrix (bit per pair): the given list, or, when the batch carries none, every pair
+  // within three bonds (1:2, 1:3 and 1:4 by shortest bond path, the reference's exclusion map
+  // for systems without virtual sites, src/Topology/atomgraph_refinement.cpp).
+  std::vector<unsigned char> ex(static_cast<size_t>(n) * n, 0);
+  if (b->exclusion_counts != nullptr) {
+    for (int e = 0; e < t.excl_count; e++) {
+      const int i = b->exclusion_pairs[2 * (t.excl_first + e)];
+      const int j = b->exclusion_pairs[2 * (t.excl_first + e) + 1];
+      if (i < 0 || j < 0 || i >= n || j >= n) {
+        return fail(STORMM_ERR_INVALID_ARGUMENT, "exclusion index out of range in system " +
+                                                     std::to_string(s));
+      }
+      ex[static_cast<size_t>(i) * n + j] = 1;
+      ex[static_cast<size_t>(j) * n + i] = 1;
+    }
+  }
+  else {
+    std::vector<std::vector<int>> adj(n);
+    for (int e = 0; e < t.bond_count; e++) {
+      const int k = t.bond_first + e;
+      const int i = b->bond_atoms[2 * k], j = b->bond_atoms[2 * k + 1];
+      if (i < 0 || j < 0 || i >= n || j >= n) {
+        return fail(STORMM_ERR_INVALID_ARGUMENT, "bond index out of range");
+      }
+      adj[i].push_back(j);
+      adj[j].push_back(i);
+    }
+    std::vector<int> dist(n, -1), frontier, next, seen;
+    for (int src = 0; src < n; src++) {
+      frontier.assign(1, src);
+      seen.assign(1, src);
+      dist[src] = 0;
+      for (int depth = 1; depth <= 3 && !frontier.empty(); depth++) {
+        next.clear();
+        for (int u : frontier) {
+          for (int v : adj[u]) {
+            if (dist[v] < 0) {
+              dist[v] = depth;
+              next.push_back(v);
+              seen.push_back(v);
+              ex[static_cast<size_t>(src) * n + v] = 1;
+            }
+          }
+        }
+        frontier.swap(next);
+      }
+      for (int v : seen) {
+        dist[v] = -1;
+      }
+    }
+  }
+  // Tile tasks: off-diagonal tiles first, then the diagonal ones.
+  const int nt = round_up(n, kTile) / kTile;
+  tp.task_first = static_cast<int>(P.tasks.size());
+  for (int pass = 0; pass < 2; pass++) {
+    for (int ti = 0; ti < nt; ti++) {
+      for (int tj = ti; tj < nt; tj++) {
+        const bool diag = (ti == tj);
+        if (diag != (pass == 1)) {
+          continue;
+        }
+        const int ni = std::min(kTile, n - ti * kTile);
+        const int nj = std::min(kTile, n - tj * kTile);
+        unsigned m[kTile];
+        bool any = false;
+        for (int il = 0; il < kTile; il++) {
+          m[il] = 0u;
+          if (il >= ni) {
+            continue;
+          }
+          for (int jl = 0; jl < nj; jl++) {
+            if (ex[static_cast<size_t>(ti * kTile + il) * n + (tj * kTile + jl)]) {
+              m[il] |= (1u << jl);
+              any = true;
+            }
+          }
+        }
+        Task tk;
+        tk.x = (ti * kTile) | ((tj * kTile) << 16);
+        tk.y = ((diag ? 1 : 0) << 8) | (ni << 12) | (nj << 20);
+        tk.z = -1;
+        tk.w = 0;
+        if (any) {
+          tk.z = static_cast<int>(P.masks.size());
+          for (int il = 0; il < kTile; il++) {
+            P.masks.push_back(m[il]);
+          }
+        }
+        P.tasks.push_back(tk);
+        if (diag) {
+          tp.diag_count++;
+        }
+        else {
+          tp.offd_count++;
+        }
+      }
+    }
+  }
+  // Valence terms with system-relative atom indices.
+  tp.term_first = static_cast<int>(P.vterms.size());
+  for (int e = 0; e < t.bond_count; e++) {
+    const int k = t.bond_first + e;
+    const int i = b->bond_atoms[2 * k], j = b->bond_atoms[2 * k + 1];
+    if (i < 0 || j < 0 || i >= n || j >= n) {
+      return fail(STORMM_ERR_INVALID_ARGUMENT, "bond index out of range");
+    }
+    P.vterms.push_back(make_int4(kVBond, i | (j << 16), 0,
+                                 static_cast<int>(P.bond_par.size())));
+    P.bond_par.push_back(make_double2(b->bond_k[k], std::fabs(b->bond_r0[k])));
+  }
+  for (int e = 0; e < t.angl_count; e++) {
+    const int k = t.angl_first + e;
+    const int i = b->angle_atoms[3 * k], j = b->angle_atoms[3 * k + 1];
+    const int kk = b->angle_atoms[3 * k + 2];
+    if (i < 0 || j < 0 || kk < 0 || i >= n || j >= n || kk >= n) {
+      return fail(STORMM_ERR_INVALID_ARGUMENT, "angle index out of range");
+    }
+    P.vterms.push_back(make_int4(kVAngle, i | (j << 16), kk,
+                                 static_cast<int>(P.angl_par.size())));
+    P.angl_par.push_back(make_double2(b->angle_k[k], b->angle_theta[k]));
+  }
+  for (int e = 0; e < t.dihe_count; e++) {
+    const int k = t.dihe_first + e;
+    const int *q4 = &b->dihedral_atoms[4 * k];
+    for (int m = 0; m < 4; m++) {
+      if (q4[m] < 0 || q4[m] >= n) {
+        return fail(STORMM_ERR_INVALID_ARGUMENT, "dihedral index out of range");
+      }
+    }
+    const double freq = b->dihe_freq[k];
+    if (!(freq >= 0.0 && freq <= kMaxDihePeriod && freq == std::rint(freq))) {
+      return fail(STORMM_ERR_INVALID_ARGUMENT,
+                  "dihedral periodicity must be an integer between 0 and 32");
+    }
+    P.vterms.push_back(make_int4(kVDihedral, q4[0] | (q4[1] << 16), q4[2] | (q4[3] << 16),
+                                 static_cast<int>(P.dihe_par.size())));
+    DiheParam dp;
+    dp.amp = b->dihe_amp[k];
+    dp.cos_phase = std::cos(b->dihe_phase[k]);
+    dp.sin_phase = std::sin(b->dihe_phase[k]);
+    dp.period = static_cast<int>(freq);
+    dp.improper = b->dihe_improper[k] ? 1 : 0;
+    P.dihe_par.push_back(dp);
+    const double scee = b->dihe_scee[k], scnb = b->dihe_scnb[k];
+    P.dihe14.push_back(make_double2(scee > 0.0 ? 1.0 / scee : 0.0,
+                                    (scee > 0.0 && scnb > 0.0) ? 1.0 / scnb : 0.0));
+  }
+  tp.term_count = static_cast<int>(P.vterms.size()) - tp.term_first;
+  // Cost model: rotation steps of the three pair passes plus valence work.
+  tp.cost = (16.0 * tp.offd_count) + (9.0 * tp.diag_count) +
+            0.25 * (t.bond_count + t.angl_count + 2 * t.dihe_count) + 0.1 * n;
+  P.templates.push_back(std::move(tp));
+  return STORMM_OK;
+}
+
+// Longest-processing-time assignment of systems onto about `target_total` work units, within
+// the LDS atom capacity and the per-unit system limit; stored-table and combining-rule systems
+// go to separate units.  Units come out in order of decreasing cost.
+static void pack_systems(const std::vector<double> &cost, const std::vector<int> &natom,
+                         const std::vector<char> &table_mode, int cap, int target_total,
+                         std::vector<std::vector<int>> &bins_out, std::vector<int> &mode_out) {
+  const int ns = static_cast<int>(cost.size());
+  double total_cost = 0.0;
+  for (int s = 0; s < ns; s++) {
+    total_cost += cost[s];
+  }
+  struct Bin {
+    double cost = 0.0;
+    int atoms = 0;
+    int mode = 0;
+    std::vector<int> systems;
+  };
+  std::vector<Bin> bins;
+  for (int mode = 0; mode < 2; mode++) {
+    std::vector<int> order;
+    double mode_cost = 0.0;
+    for (int s = 0; s < ns; s++) {
+      if (table_mode[s] == mode) {
+        order.push_back(s);
+        mode_cost += cost[s];
+      }
+    }
+    if (order.empty()) {
+      continue;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+    const int target = std::max(1, static_cast<int>(std::lround(target_total * mode_cost /
+                                                                std::max(total_cost, 1.0))));
+    const size_t first_bin = bins.size();
+    for (int s : order) {
+      const int need = round_up(natom[s], kTile);
+      int best = -1;
+      for (size_t k = first_bin; k < bins.size(); k++) {
+        if (bins[k].atoms + need <= cap &&
+            static_cast<int>(bins[k].systems.size()) < kMaxWuSystems &&
+            (best < 0 || bins[k].cost < bins[best].cost)) {
+          best = static_cast<int>(k);
+        }
+      }
+      const bool open_new = (best < 0) ||
+                            (static_cast<int>(bins.size() - first_bin) < target &&
+                             bins[best].cost > 0.0);
+      if (open_new) {
+        Bin nb;
+        nb.mode = mode;
+        bins.push_back(nb);
+        best = static_cast<int>(bins.size()) - 1;
+      }
+      bins[best].systems.push_back(s);
+      bins[best].atoms += need;
+      bins[best].cost += cost[s];
+    }
+  }
+  std::stable_sort(bins.begin(), bins.end(), [](const Bin &a, const Bin &c) {
+    return a.cost > c.cost;
+  });
+  bins_out.clear();
+  mode_out.clear();
+  for (Bin &bn : bins) {
+    bins_out.push_back(std::move(bn.systems));
+    mode_out.push_back(bn.mode);
+  }
+}
+
+// Appends one work unit holding `systems` to the plan; returns its LDS atom count.
+static int append_work_unit(const std::vector<int> &systems, int lj_table, SynthesisPlan &P) {
+  WuDesc d;
+  d.sys_first = static_cast<int>(P.wu_sys.size());
+  d.sys_count = static_cast<int>(systems.size());
+  d.offd_count = 0;
+  d.task_count = 0;
+  d.term_count = 0;
+  d.lj_table = lj_table;
+  d.pad = 0;
+  int lds = 0;
+  for (int s : systems) {
+    const TopoTemplate &tp = P.templates[P.sys_template[s]];
+    WuSys ws;
+    ws.sys_index = s;
+    ws.lds_base = lds;
+    ws.global_start = P.atom_starts[s];
+    ws.natom = P.atom_counts[s];
+    ws.task_first = tp.task_first;
+    ws.offd_count = tp.offd_count;
+    ws.diag_count = tp.diag_count;
+    ws.term_first = tp.term_first;
+    ws.term_count = tp.term_count;
+    ws.lj_off = tp.lj_off;
+    ws.lj_nt = tp.lj_nt;
+    ws.pad = 0;
+    P.wu_sys.push_back(ws);
+    d.offd_count += tp.offd_count;
+    d.task_count += tp.offd_count + tp.diag_count;
+    d.term_count += tp.term_count;
+    lds += round_up(P.atom_counts[s], kTile);
+  }
+  d.lds_atoms = lds;
+  P.wu.push_back(d);
+  return lds;
 }
 
 static int build_plan(const stormm_topology_batch *b, const stormm_launch_shape *shape_in,
@@ -223,39 +624,81 @@ static int build_plan(const stormm_topology_batch *b, const stormm_launch_shape 
 
   // Term offsets per system.
   std::vector<SystemTerms> st(ns);
-  int ab = 0, an = 0, ad = 0, ae = 0, at = 0, atom_cursor = 0;
-  P.lj_off.resize(ns);
-  P.lj_nt.resize(ns);
-  P.lj_table_mode.assign(ns, 0);
+  {
+    int ab = 0, an = 0, ad = 0, ae = 0, at = 0, asg = 0;
+    for (int s = 0; s < ns; s++) {
+      st[s].bond_first = ab;
+      st[s].bond_count = b->bond_counts ? b->bond_counts[s] : 0;
+      ab += st[s].bond_count;
+      st[s].angl_first = an;
+      st[s].angl_count = b->angle_counts ? b->angle_counts[s] : 0;
+      an += st[s].angl_count;
+      st[s].dihe_first = ad;
+      st[s].dihe_count = b->dihedral_counts ? b->dihedral_counts[s] : 0;
+      ad += st[s].dihe_count;
+      st[s].excl_first = ae;
+      st[s].excl_count = b->exclusion_counts ? b->exclusion_counts[s] : 0;
+      ae += st[s].excl_count;
+      st[s].lj_types = b->lj_type_counts[s];
+      if (st[s].lj_types <= 0) {
+        return fail(STORMM_ERR_INVALID_ARGUMENT, "system " + std::to_string(s) +
+                                                     " has no Lennard-Jones types");
+      }
+      st[s].combining = (b->lj_combining != nullptr) && (b->lj_combining[s] != 0);
+      if (st[s].combining) {
+        if (b->lj_sigma == nullptr || b->lj_epsilon == nullptr) {
+          return fail(STORMM_ERR_INVALID_ARGUMENT, "combining-rule systems need lj_sigma and "
+                                                   "lj_epsilon");
+        }
+        st[s].sig_first = asg;
+        asg += st[s].lj_types;
+      }
+      else {
+        if (b->lja == nullptr || b->ljb == nullptr || b->lja14 == nullptr ||
+            b->ljb14 == nullptr) {
+          return fail(STORMM_ERR_INVALID_ARGUMENT, "system " + std::to_string(s) +
+                                                       " has no Lennard-Jones tables");
+        }
+        st[s].lj_first = at;
+        at += st[s].lj_types * st[s].lj_types;
+      }
+    }
+  }
+
+  // Topology templates: systems with identical atom counts, exclusions, valence terms and LJ
+  // tables share one set of tile tasks, exclusion masks, valence terms and parameters.
+  P.sys_template.assign(ns, -1);
+  std::unordered_map<unsigned long long, std::vector<int>> by_hash;
   for (int s = 0; s < ns; s++) {
-    st[s].bond_first = ab;
-    st[s].bond_count = b->bond_counts ? b->bond_counts[s] : 0;
-    ab += st[s].bond_count;
-    st[s].angl_first = an;
-    st[s].angl_count = b->angle_counts ? b->angle_counts[s] : 0;
-    an += st[s].angl_count;
-    st[s].dihe_first = ad;
-    st[s].dihe_count = b->dihedral_counts ? b->dihedral_counts[s] : 0;
-    ad += st[s].dihe_count;
-    st[s].excl_first = ae;
-    st[s].excl_count = b->exclusion_counts ? b->exclusion_counts[s] : 0;
-    ae += st[s].excl_count;
-    const int t = b->lj_type_counts[s];
-    if (t <= 0) {
-      return fail(STORMM_ERR_INVALID_ARGUMENT, "system " + std::to_string(s) +
-                                                   " has no Lennard-Jones types");
+    const unsigned long long h = topology_hash(b, st[s], b->atom_counts[s]);
+    std::vector<int> &bucket = by_hash[h];
+    int tid = -1;
+    for (int cand : bucket) {
+      const int src = P.templates[cand].source_system;
+      if (same_topology(b, st[src], b->atom_counts[src], st[s], b->atom_counts[s])) {
+        tid = cand;
+        break;
+      }
     }
-    const double *A = b->lja + at, *B = b->ljb + at, *A14 = b->lja14 + at, *B14 = b->ljb14 + at;
-    std::vector<double> sig, eps;
-    const bool lb = lorentz_berthelot(t, A, B, A14, B14, sig, eps);
-    P.lj_table_mode[s] = lb ? 0 : 1;
-    P.lj_table_systems += lb ? 0 : 1;
-    P.lj_off[s] = static_cast<int>(P.ljab.size());
-    P.lj_nt[s] = t;
-    for (int k = 0; k < t * t; k++) {
-      P.ljab.push_back(make_double2(A[k], B[k]));
-      P.ljab14.push_back(make_double2(A14[k], B14[k]));
+    if (tid < 0) {
+      const int rc = add_template(b, st[s], b->atom_counts[s], s, P);
+      if (rc != STORMM_OK) {
+        return rc;
+      }
+      tid = static_cast<int>(P.templates.size()) - 1;
+      bucket.push_back(tid);
     }
+    P.sys_template[s] = tid;
+  }
+
+  // Per-atom parameters.
+  P.lj_table_mode.assign(ns, 0);
+  int atom_cursor = 0;
+  for (int s = 0; s < ns; s++) {
+    const TopoTemplate &tp = P.templates[P.sys_template[s]];
+    P.lj_table_mode[s] = tp.lb ? 0 : 1;
+    P.lj_table_systems += tp.lb ? 0 : 1;
+    const int t = st[s].lj_types;
     const int n = b->atom_counts[s];
     for (int i = 0; i < n; i++) {
       const int g = P.atom_starts[s] + i;
@@ -267,9 +710,9 @@ static int build_plan(const stormm_topology_batch *b, const stormm_launch_shape 
       }
       P.q[g] = b->charge[src] * sqrt_coul;
       P.lj_type[g] = ty;
-      if (lb) {
-        P.ljp1[g] = sig[ty];
-        P.ljp2[g] = 2.0 * std::sqrt(eps[ty]);
+      if (tp.lb) {
+        P.ljp1[g] = tp.sig[ty];
+        P.ljp2[g] = 2.0 * std::sqrt(tp.eps[ty]);
       }
       const double pr = (b->pb_radii != nullptr) ? b->pb_radii[src] : 0.0;
       const double sc = (b->gb_screen != nullptr) ? b->gb_screen[src] : 0.0;
@@ -285,214 +728,26 @@ static int build_plan(const stormm_topology_batch *b, const stormm_launch_shape 
       }
     }
     atom_cursor += n;
-    at += t * t;
-  }
-
-  // Cost model per system (rotation steps of the three pair passes plus valence).
-  std::vector<double> cost(ns);
-  for (int s = 0; s < ns; s++) {
-    const int nt = round_up(P.atom_counts[s], kTile) / kTile;
-    const double offd = 0.5 * nt * (nt - 1);
-    cost[s] = (16.0 * offd) + (9.0 * nt) + 0.25 * (st[s].bond_count + st[s].angl_count +
-                                                   2 * st[s].dihe_count) +
-              0.1 * P.atom_counts[s];
+    P.task_count += tp.offd_count + tp.diag_count;
+    P.vterm_count += tp.term_count;
   }
 
   // Work-unit packing: longest-processing-time assignment onto a target number of units,
   // respecting the LDS atom capacity and the per-unit system limit; systems that read stored
   // LJ tables are packed separately from combining-rule systems.
   const int cus = cu_count_or_default();
-  const int target_total = std::max(1, cus * wu_per_cu);
-  double total_cost = 0.0;
+  std::vector<double> cost(ns);
   for (int s = 0; s < ns; s++) {
-    total_cost += cost[s];
+    cost[s] = P.templates[P.sys_template[s]].cost;
   }
-  struct Bin {
-    double cost = 0.0;
-    int atoms = 0;
-    int lj_table = 0;
-    std::vector<int> systems;
-  };
-  std::vector<Bin> bins;
-  for (int mode = 0; mode < 2; mode++) {
-    std::vector<int> order;
-    double mode_cost = 0.0;
-    for (int s = 0; s < ns; s++) {
-      if (P.lj_table_mode[s] == mode) {
-        order.push_back(s);
-        mode_cost += cost[s];
-      }
-    }
-    if (order.empty()) {
-      continue;
-    }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
-    const int target = std::max(1, static_cast<int>(std::lround(target_total * mode_cost /
-                                                                std::max(total_cost, 1.0))));
-    const size_t first_bin = bins.size();
-    for (int s : order) {
-      const int need = round_up(P.atom_counts[s], kTile);
-      int best = -1;
-      for (size_t k = first_bin; k < bins.size(); k++) {
-        if (bins[k].atoms + need <= cap &&
-            static_cast<int>(bins[k].systems.size()) < kMaxWuSystems &&
-            (best < 0 || bins[k].cost < bins[best].cost)) {
-          best = static_cast<int>(k);
-        }
-      }
-      const bool open_new = (best < 0) ||
-                            (static_cast<int>(bins.size() - first_bin) < target &&
-                             bins[best].cost > 0.0);
-      if (open_new) {
-        Bin nb;
-        nb.lj_table = mode;
-        bins.push_back(nb);
-        best = static_cast<int>(bins.size()) - 1;
-      }
-      bins[best].systems.push_back(s);
-      bins[best].atoms += need;
-      bins[best].cost += cost[s];
-    }
-  }
-  std::stable_sort(bins.begin(), bins.end(), [](const Bin &a, const Bin &c) {
-    return a.cost > c.cost;
-  });
-
-  // Exclusion lookup per system, built when its unit is laid out.
+  std::vector<std::vector<int>> bins;
+  std::vector<int> bin_mode;
+  pack_systems(cost, P.atom_counts, P.lj_table_mode, cap, std::max(1, cus * wu_per_cu), bins,
+               bin_mode);
   P.maxa = 0;
-  for (const Bin &bn : bins) {
-    WuDesc d;
-    d.sys_first = static_cast<int>(P.wu_sys.size());
-    d.sys_count = static_cast<int>(bn.systems.size());
-    d.task_first = static_cast<int>(P.tasks.size());
-    d.term_first = static_cast<int>(P.vterms.size());
-    d.lj_table = bn.lj_table;
-    int lds = 0;
-    std::vector<Task> offd_tasks, diag_tasks;
-    for (int slot = 0; slot < d.sys_count; slot++) {
-      const int s = bn.systems[slot];
-      const int n = P.atom_counts[s];
-      WuSys ws;
-      ws.sys_index = s;
-      ws.lds_base = lds;
-      ws.global_start = P.atom_starts[s];
-      ws.natom = n;
-      P.wu_sys.push_back(ws);
-      // Exclusion matrix (bit per pair).
-      std::vector<unsigned char> ex(static_cast<size_t>(n) * n, 0);
-      for (int e = 0; e < st[s].excl_count; e++) {
-        const int i = b->exclusion_pairs[2 * (st[s].excl_first + e)];
-        const int j = b->exclusion_pairs[2 * (st[s].excl_first + e) + 1];
-        if (i < 0 || j < 0 || i >= n || j >= n) {
-          return fail(STORMM_ERR_INVALID_ARGUMENT, "exclusion index out of range in system " +
-                                                       std::to_string(s));
-        }
-        ex[static_cast<size_t>(i) * n + j] = 1;
-        ex[static_cast<size_t>(j) * n + i] = 1;
-      }
-      const int nt = round_up(n, kTile) / kTile;
-      for (int ti = 0; ti < nt; ti++) {
-        for (int tj = ti; tj < nt; tj++) {
-          const int ni = std::min(kTile, n - ti * kTile);
-          const int nj = std::min(kTile, n - tj * kTile);
-          unsigned m[kTile];
-          bool any = false;
-          for (int il = 0; il < kTile; il++) {
-            m[il] = 0u;
-            if (il >= ni) {
-              continue;
-            }
-            for (int jl = 0; jl < nj; jl++) {
-              if (ex[static_cast<size_t>(ti * kTile + il) * n + (tj * kTile + jl)]) {
-                m[il] |= (1u << jl);
-                any = true;
-              }
-            }
-          }
-          Task tk;
-          tk.x = (lds + ti * kTile) | ((lds + tj * kTile) << 16);
-          tk.y = slot | ((ti == tj ? 1 : 0) << 8) | (ni << 12) | (nj << 20);
-          tk.z = -1;
-          tk.w = s;
-          if (any) {
-            tk.z = static_cast<int>(P.masks.size());
-            for (int il = 0; il < kTile; il++) {
-              P.masks.push_back(m[il]);
-            }
-          }
-          if (ti == tj) {
-            diag_tasks.push_back(tk);
-          }
-          else {
-            offd_tasks.push_back(tk);
-          }
-        }
-      }
-      // Valence terms with LDS atom indices.
-      auto lidx = [&](int a) { return lds + a; };
-      for (int e = 0; e < st[s].bond_count; e++) {
-        const int k = st[s].bond_first + e;
-        const int i = b->bond_atoms[2 * k], j = b->bond_atoms[2 * k + 1];
-        if (i < 0 || j < 0 || i >= n || j >= n) {
-          return fail(STORMM_ERR_INVALID_ARGUMENT, "bond index out of range");
-        }
-        P.vterms.push_back(make_int4(kVBond | (slot << 8), lidx(i) | (lidx(j) << 16), 0,
-                                     static_cast<int>(P.bond_par.size())));
-        P.bond_par.push_back(make_double2(b->bond_k[k], std::fabs(b->bond_r0[k])));
-      }
-      for (int e = 0; e < st[s].angl_count; e++) {
-        const int k = st[s].angl_first + e;
-        const int i = b->angle_atoms[3 * k], j = b->angle_atoms[3 * k + 1];
-        const int kk = b->angle_atoms[3 * k + 2];
-        if (i < 0 || j < 0 || kk < 0 || i >= n || j >= n || kk >= n) {
-          return fail(STORMM_ERR_INVALID_ARGUMENT, "angle index out of range");
-        }
-        P.vterms.push_back(make_int4(kVAngle | (slot << 8), lidx(i) | (lidx(j) << 16), lidx(kk),
-                                     static_cast<int>(P.angl_par.size())));
-        P.angl_par.push_back(make_double2(b->angle_k[k], b->angle_theta[k]));
-      }
-      for (int e = 0; e < st[s].dihe_count; e++) {
-        const int k = st[s].dihe_first + e;
-        const int *q4 = &b->dihedral_atoms[4 * k];
-        for (int m = 0; m < 4; m++) {
-          if (q4[m] < 0 || q4[m] >= n) {
-            return fail(STORMM_ERR_INVALID_ARGUMENT, "dihedral index out of range");
-          }
-        }
-        P.vterms.push_back(make_int4(kVDihedral | (slot << 8), lidx(q4[0]) | (lidx(q4[1]) << 16),
-                                     lidx(q4[2]) | (lidx(q4[3]) << 16),
-                                     static_cast<int>(P.dihe_par.size())));
-        const double freq = b->dihe_freq[k];
-        if (!(freq >= 0.0 && freq <= kMaxDihePeriod && freq == std::rint(freq))) {
-          return fail(STORMM_ERR_INVALID_ARGUMENT,
-                      "dihedral periodicity must be an integer between 0 and 32");
-        }
-        DiheParam dp;
-        dp.amp = b->dihe_amp[k];
-        dp.cos_phase = std::cos(b->dihe_phase[k]);
-        dp.sin_phase = std::sin(b->dihe_phase[k]);
-        dp.period = static_cast<int>(freq);
-        dp.improper = b->dihe_improper[k] ? 1 : 0;
-        P.dihe_par.push_back(dp);
-        const double scee = b->dihe_scee[k], scnb = b->dihe_scnb[k];
-        P.dihe14.push_back(make_double2(scee > 0.0 ? 1.0 / scee : 0.0,
-                                        (scee > 0.0 && scnb > 0.0) ? 1.0 / scnb : 0.0));
-      }
-      lds += round_up(n, kTile);
-    }
-    for (const Task &tk : offd_tasks) {
-      P.tasks.push_back(tk);
-    }
-    for (const Task &tk : diag_tasks) {
-      P.tasks.push_back(tk);
-    }
-    d.task_count = static_cast<int>(P.tasks.size()) - d.task_first;
-    d.term_count = static_cast<int>(P.vterms.size()) - d.term_first;
-    d.lds_atoms = lds;
-    P.maxa = std::max(P.maxa, lds);
-    P.wu.push_back(d);
+  for (size_t u = 0; u < bins.size(); u++) {
+    P.maxa = std::max(P.maxa, append_work_unit(bins[u], bin_mode[u], P));
   }
-  P.vterm_count = static_cast<long long>(P.vterms.size());
   P.block = (shape.block_threads > 0) ? shape.block_threads : kEvalBlock;
   if (P.block != 256 && P.block != 512 && P.block != 1024) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "block_threads must be 256, 512 or 1024");
@@ -505,7 +760,7 @@ static int build_plan(const stormm_topology_batch *b, const stormm_launch_shape 
 // ------------------------------------------------------------------------------------------
 struct stormm_synthesis {
   SynthesisPlan plan;
-  DevMem atom_starts, atom_counts, q, ljp1, ljp2, lj_type, ljab, ljab14, lj_off, lj_nt, rho, scr,
+  DevMem atom_starts, atom_counts, q, ljp1, ljp2, lj_type, ljab, ljab14, rho, scr,
       pbr, alpha, beta, gamma, wu, wu_sys, tasks, masks, vterms, bond_par, angl_par, dihe_par,
       dihe14, status, cycles_run, rms_force;
   MMKit kit;
@@ -567,7 +822,7 @@ extern "C" int stormm_synthesis_plan(const stormm_topology_batch *batch,
     info->system_count = P.nsys;
     info->padded_atom_count = P.padded_atoms;
     info->work_unit_count = static_cast<int>(P.wu.size());
-    info->task_count = static_cast<int>(P.tasks.size());
+    info->task_count = static_cast<int>(P.task_count);
     info->max_wu_atoms = P.maxa;
     info->lj_table_systems = P.lj_table_systems;
     info->pair_count = P.pairs;
@@ -596,8 +851,6 @@ extern "C" int stormm_synthesis_create(const stormm_topology_batch *batch,
   HIP_TRY(ag->lj_type.upload(P.lj_type));
   HIP_TRY(ag->ljab.upload(P.ljab));
   HIP_TRY(ag->ljab14.upload(P.ljab14));
-  HIP_TRY(ag->lj_off.upload(P.lj_off));
-  HIP_TRY(ag->lj_nt.upload(P.lj_nt));
   HIP_TRY(ag->rho.upload(P.rho));
   HIP_TRY(ag->scr.upload(P.scr));
   HIP_TRY(ag->pbr.upload(P.pbr));
@@ -645,8 +898,6 @@ extern "C" int stormm_synthesis_create(const stormm_topology_batch *batch,
   k.lj_type = ag->lj_type.as<int>();
   k.ljab = ag->ljab.as<double2>();
   k.ljab14 = ag->ljab14.as<double2>();
-  k.lj_off = ag->lj_off.as<int>();
-  k.lj_nt = ag->lj_nt.as<int>();
   k.rho = ag->rho.as<double>();
   k.scr = ag->scr.as<double>();
   k.pbr = ag->pbr.as<double>();
@@ -671,7 +922,7 @@ extern "C" int stormm_synthesis_get_info(const stormm_synthesis *ag, stormm_synt
   info->system_count = P.nsys;
   info->padded_atom_count = P.padded_atoms;
   info->work_unit_count = static_cast<int>(P.wu.size());
-  info->task_count = static_cast<int>(P.tasks.size());
+  info->task_count = static_cast<int>(P.task_count);
   info->max_wu_atoms = P.maxa;
   info->lj_table_systems = P.lj_table_systems;
   info->pair_count = P.pairs;

@@ -88,10 +88,17 @@ class _BatchArrays:
         b.charge = _ptr(cat([t.charge for t in topos], np.float64))
         b.lj_type = _ptr(cat([t.lj_idx for t in topos], np.int32))
         b.lj_type_counts = _ptr(counts([t.n_lj_types for t in topos]))
-        b.lja = _ptr(cat([t.lja for t in topos], np.float64))
-        b.ljb = _ptr(cat([t.ljb for t in topos], np.float64))
-        b.lja14 = _ptr(cat([t.lja14 for t in topos], np.float64))
-        b.ljb14 = _ptr(cat([t.ljb14 for t in topos], np.float64))
+        # Systems with per-type Lorentz-Berthelot parameters send those; the others send their
+        # T x T tables.
+        comb = [t.lj_sigma is not None for t in topos]
+        tabled = [t.lj_tables() for t, c in zip(topos, comb) if not c]
+        b.lj_combining = _ptr(counts([1 if c else 0 for c in comb]))
+        b.lj_sigma = _ptr(cat([t.lj_sigma for t, c in zip(topos, comb) if c], np.float64))
+        b.lj_epsilon = _ptr(cat([t.lj_epsilon for t, c in zip(topos, comb) if c], np.float64))
+        b.lja = _ptr(cat([x[0] for x in tabled], np.float64))
+        b.ljb = _ptr(cat([x[1] for x in tabled], np.float64))
+        b.lja14 = _ptr(cat([x[2] for x in tabled], np.float64))
+        b.ljb14 = _ptr(cat([x[3] for x in tabled], np.float64))
         b.pb_radii = _ptr(cat([t.pb_radii for t in topos], np.float64))
         b.gb_screen = _ptr(cat([t.gb_screen for t in topos], np.float64))
         b.bond_counts = _ptr(counts([len(t.bonds) for t in topos]))
@@ -110,9 +117,10 @@ class _BatchArrays:
         b.dihe_improper = _ptr(cat([t.dihe_improper for t in topos], np.int32))
         b.dihe_scee = _ptr(cat([t.dihe_scee for t in topos], np.float64))
         b.dihe_scnb = _ptr(cat([t.dihe_scnb for t in topos], np.float64))
-        excl = [t.exclusion_pairs() for t in topos]
-        b.exclusion_counts = _ptr(counts([len(e) for e in excl]))
-        b.exclusion_pairs = _ptr(cat(excl, np.int32))
+        # Exclusions are the pairs within three bonds, which the planner derives from the bonds
+        # once per distinct topology (null exclusion arrays).
+        b.exclusion_counts = None
+        b.exclusion_pairs = None
         b.igb = topos[0].igb
         b.dielectric = topos[0].dielectric
         b.salt_concentration = topos[0].salt

@@ -47,23 +47,33 @@ def _grid_charges(rng: np.random.Generator, n: int) -> np.ndarray:
     return iq.astype(np.float64) * CHARGE_GRID
 
 
+def _cross(a, b):
+    return (a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0])
+
+
 def _place(parent: np.ndarray, grand: np.ndarray, great: np.ndarray, r0: float, theta: float,
            phi: float) -> np.ndarray:
     """Position at distance r0 from parent, angle theta to grand, torsion phi to great."""
-    bc = parent - grand
-    bc /= np.linalg.norm(bc)
-    ab = grand - great
-    nrm = np.cross(ab, bc)
-    ln = np.linalg.norm(nrm)
+    px, py, pz = float(parent[0]), float(parent[1]), float(parent[2])
+    bc = (px - float(grand[0]), py - float(grand[1]), pz - float(grand[2]))
+    lb = math.sqrt(bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2])
+    bc = (bc[0] / lb, bc[1] / lb, bc[2] / lb)
+    ab = (float(grand[0]) - float(great[0]), float(grand[1]) - float(great[1]),
+          float(grand[2]) - float(great[2]))
+    nrm = _cross(ab, bc)
+    ln = math.sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2])
     if ln < 1.0e-8:
-        trial = np.array([1.0, 0.0, 0.0]) if abs(bc[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
-        nrm = np.cross(trial, bc)
-        ln = np.linalg.norm(nrm)
-    nrm /= ln
-    m = np.cross(nrm, bc)
-    d2 = np.array([-r0 * math.cos(theta), r0 * math.sin(theta) * math.cos(phi),
-                   r0 * math.sin(theta) * math.sin(phi)])
-    return parent + d2[0] * bc + d2[1] * m + d2[2] * nrm
+        trial = (1.0, 0.0, 0.0) if abs(bc[0]) < 0.9 else (0.0, 1.0, 0.0)
+        nrm = _cross(trial, bc)
+        ln = math.sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2])
+    nrm = (nrm[0] / ln, nrm[1] / ln, nrm[2] / ln)
+    m = _cross(nrm, bc)
+    d0 = -r0 * math.cos(theta)
+    d1 = r0 * math.sin(theta) * math.cos(phi)
+    d2 = r0 * math.sin(theta) * math.sin(phi)
+    return np.array([px + d0 * bc[0] + d1 * m[0] + d2 * nrm[0],
+                     py + d0 * bc[1] + d1 * m[1] + d2 * nrm[1],
+                     pz + d0 * bc[2] + d1 * m[2] + d2 * nrm[2]])
 
 
 def branched_molecule(natom: int, seed: int, gb: bool = True,
@@ -167,16 +177,11 @@ def branched_molecule(natom: int, seed: int, gb: bool = True,
     nd = len(dihedrals)
     sigma = rng.uniform(2.0, 3.8, natom)
     eps = rng.uniform(0.02, 0.2, natom)
-    sij = 0.5 * (sigma[:, None] + sigma[None, :])
-    eij = np.sqrt(eps[:, None] * eps[None, :])
-    A = (4.0 * eij * sij ** 12).reshape(-1)
-    B = (4.0 * eij * sij ** 6).reshape(-1)
     q = _grid_charges(rng, natom)
     radii = rng.uniform(1.2, 2.0, natom)
     screen = rng.uniform(0.7, 0.9, natom)
     topo = Topology(
         natom=natom, charge=q, lj_idx=np.arange(natom, dtype=np.int32), n_lj_types=natom,
-        lja=A, ljb=B, lja14=A.copy(), ljb14=B.copy(),
         bonds=np.array(bonds, dtype=np.int32).reshape(-1, 2),
         bond_k=np.array(bond_k), bond_r0=np.array(bond_r0),
         angles=np.array(angles, dtype=np.int32).reshape(-1, 3),

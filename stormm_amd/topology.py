@@ -54,10 +54,6 @@ class Topology:
     charge: np.ndarray                  # (N,) e
     lj_idx: np.ndarray                  # (N,) int32
     n_lj_types: int
-    lja: np.ndarray                     # (T*T,) A coefficients, index ti * T + tj
-    ljb: np.ndarray                     # (T*T,) B coefficients
-    lja14: np.ndarray                   # (T*T,) 1:4 A coefficients
-    ljb14: np.ndarray                   # (T*T,) 1:4 B coefficients
     bonds: np.ndarray                   # (nb, 2) int32
     bond_k: np.ndarray
     bond_r0: np.ndarray
@@ -81,9 +77,15 @@ class Topology:
     infr14_scee: Optional[np.ndarray] = None
     infr14_scnb: Optional[np.ndarray] = None
     name: str = ""
-    # Per-type sigma / epsilon when the tables follow Lorentz-Berthelot combining (None if not).
+    # Lennard-Jones parameters: either per-type sigma / epsilon under Lorentz-Berthelot combining
+    # (the tables below are then built on demand), or the stored T x T tables (index ti * T + tj)
+    # for the ordinary and the 1:4 interactions, as an AMBER topology holds them.
     lj_sigma: Optional[np.ndarray] = None
     lj_epsilon: Optional[np.ndarray] = None
+    lja: Optional[np.ndarray] = None
+    ljb: Optional[np.ndarray] = None
+    lja14: Optional[np.ndarray] = None
+    ljb14: Optional[np.ndarray] = None
     _excl_cache: Optional[np.ndarray] = dataclasses.field(default=None, repr=False)
 
     # ------------------------------------------------------------------------------------------
@@ -138,6 +140,21 @@ class Topology:
             m[p[:, 1], p[:, 0]] = 1
         return m
 
+    def lj_tables(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+        """(A, B, A14, B14), each (T*T,): the stored tables, or the Lorentz-Berthelot tables
+        A_ij = 4 eps_ij sigma_ij^12, B_ij = 4 eps_ij sigma_ij^6 of the per-type parameters."""
+        if self.lja is None:
+            if self.lj_sigma is None:
+                raise ValueError("topology has neither LJ tables nor per-type parameters")
+            sig = np.asarray(self.lj_sigma, dtype=np.float64)
+            eps = np.asarray(self.lj_epsilon, dtype=np.float64)
+            sij = 0.5 * (sig[:, None] + sig[None, :])
+            eij = np.sqrt(eps[:, None] * eps[None, :])
+            a = (4.0 * eij * sij ** 12).reshape(-1)
+            b = (4.0 * eij * sij ** 6).reshape(-1)
+            return a, b, a, b
+        return self.lja, self.ljb, self.lja14, self.ljb14
+
     def lorentz_berthelot(self, rtol: float = 1.0e-12) -> Optional[Tuple[np.ndarray, np.ndarray]]:
         """Per-type (sigma, epsilon) if every table entry follows Lorentz-Berthelot combining.
 
@@ -145,6 +162,9 @@ class Topology:
         and eps_ij = sqrt(e_i e_j).  Returns None when any entry departs by more than rtol or a
         type has a zero coefficient paired with a nonzero one.
         """
+        if self.lj_sigma is not None:
+            return (np.asarray(self.lj_sigma, dtype=np.float64),
+                    np.asarray(self.lj_epsilon, dtype=np.float64))
         t = self.n_lj_types
         A = self.lja.reshape(t, t)
         B = self.ljb.reshape(t, t)
@@ -385,8 +405,9 @@ def write_prmtop(topo: Topology, path: str, title: str = "synthetic") -> None:
     """
     n = topo.natom
     t = topo.n_lj_types
-    A = topo.lja.reshape(t, t)
-    B = topo.ljb.reshape(t, t)
+    lja, ljb, _, _ = topo.lj_tables()
+    A = lja.reshape(t, t)
+    B = ljb.reshape(t, t)
     nb_index = np.zeros((t, t), dtype=np.int64)
     acoef, bcoef = [], []
     for ti in range(t):

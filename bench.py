@@ -20,6 +20,10 @@ Two further workloads ride on the same line (skip them with --no-extra):
     sequence, with the oracle's minimizer on host threads as the CPU rate.
   "large_batch": 16384 systems, N ~ U[32, 96]: FP32 and FP64 evaluation throughput and the
     errors of both modes against the FP64 oracle on a 64-system subsample.
+  "serving": the served-minimization stream (Poisson arrivals at 200 requests/s over
+    --serve-seconds, 1-64 conformers of one molecule with N ~ logU[16, 512] per request, 300
+    cycles or RMS force <= 0.1): continuous batching and plain batching in real time (p50 / p99
+    latency), and the capacity of continuous serving against one batched launch per request.
 
   python bench.py [--gpus 1] [--steps 500] [--warmup 50] [--cpu-seconds 12] [--no-cpu]
                   [--no-extra]
@@ -167,6 +171,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the minimization and large-batch workloads")
     ap.add_argument("--min-cycles", type=int, default=500)
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="arrival window of the served-minimization stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this path runs every system of a batch on a single "
@@ -262,6 +268,11 @@ def main():
         extra["large_batch"] = large_batch_workload(S, synthetic, oracle, 16384,
                                                     max(20, args.steps // 5))
         log(f"large batch: {extra['large_batch']}")
+        from stormm_amd import serving
+        extra["serving"] = serving.serving_benchmark(seconds=args.serve_seconds, log=log)
+        log(f"serving: { {k: v for k, v in extra['serving'].items() if k != 'capacity'} }")
+        log(f"serving capacity: {extra['serving']['capacity']['speedup_vs_plain_batched']:.2f}x "
+            "the plain batched launches")
 
     out = {
         "metric": METRIC,

@@ -266,6 +266,54 @@ int stormm_time_mm_evaluation(int32_t precision, const stormm_synthesis *ag,
                               stormm_phasespace *ps, stormm_scorecard *sc, int32_t eval_force,
                               int32_t repeats, void *stream, double *mean_ms);
 
+/* ---- Serving pool ------------------------------------------------------------------------- */
+/* No counterpart in the reference, which minimizes one fixed batch per launchMinimization call
+ * (src/MolecularMechanics/hpc_minimization.cu:129-596).  A pool keeps systems resident on the
+ * device with all of their minimization state and advances whatever is resident a few cycles at
+ * a time, so that requests can join and leave between calls (continuous batching).  Each
+ * admission takes device memory from a fixed budget; a system stops at the cycle limit or when
+ * its RMS force reaches the threshold.  A system's trajectory does not depend on what else is
+ * resident or on how its cycles are split across calls. */
+typedef struct stormm_pool stormm_pool;
+typedef struct {
+  int32_t precision;               /* STORMM_PREC_SINGLE or STORMM_PREC_DOUBLE                  */
+  int32_t gpos_bits, frc_bits, nrg_bits;
+  int64_t hbm_budget_bytes;        /* device memory for resident systems and their templates     */
+  int32_t max_systems;             /* resident system slots                                     */
+  int32_t igb;                     /* implicit-solvent model shared by every admitted system     */
+  double dielectric;
+  double salt_concentration;
+  double coulomb_constant;
+  stormm_minimize_controls minimize; /* cycle limit, steepest-descent cycles, dx0, RMS stop    */
+  stormm_launch_shape shape;       /* work-unit shaping (zero = defaults)                       */
+} stormm_pool_config;
+
+int stormm_pool_create(const stormm_pool_config *config, stormm_pool **out);
+int stormm_pool_destroy(stormm_pool *pool);
+/* Free atoms and slots, bytes held by work templates, and the atom capacity of the budget. */
+int stormm_pool_capacity(const stormm_pool *pool, int64_t *free_atoms, int32_t *free_slots,
+                         int64_t *template_bytes, int64_t *capacity_atoms);
+/* Admit every system of `batch` with coordinates xyz ([sum atoms][3] A); writes their slots.
+ * STORMM_ERR_CAPACITY (nothing admitted) when the budget or the slots cannot hold them. */
+int stormm_pool_admit(stormm_pool *pool, const stormm_topology_batch *batch, const double *xyz,
+                      int32_t *slots);
+/* Admit `count` conformers of the one system in `molecule`; xyz is [count][N][3] A. */
+int stormm_pool_admit_conformers(stormm_pool *pool, const stormm_topology_batch *molecule,
+                                 int32_t count, const double *xyz, int32_t *slots);
+/* Return systems' slots and (once all systems of an admission have left) their atoms. */
+int stormm_pool_release(stormm_pool *pool, const int32_t *slots, int32_t count);
+/* Advance every resident, still-running system by up to `cycles` minimization cycles (the
+ * reference's per-cycle launch sequence) and evaluate its energy at the coordinates reached;
+ * writes how many systems were running.  Work is queued on `stream`. */
+int stormm_pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *running, void *stream);
+/* Per slot: cycles completed, finished flag (threshold or cycle limit), last RMS force
+ * (kcal/mol/A) and potential energy (kcal/mol) at the current coordinates. */
+int stormm_pool_status(const stormm_pool *pool, const int32_t *slots, int32_t count,
+                       int32_t *steps, int32_t *done, double *rms, double *energy);
+/* Coordinates of one system, [N][3] in A and/or fixed precision (either pointer may be null). */
+int stormm_pool_coordinates(const stormm_pool *pool, int32_t slot, double *xyz,
+                            int64_t *fixed_xyz);
+
 /* ---- Device microbenchmarks (roofline calibration) ------------------------------------------ */
 /* Stream-copy `bytes` bytes `repeats` times; returns read+write GB/s.  No reference counterpart:
  * used to confirm the HBM ceiling on the device and to scale profiler memory counters. */

@@ -19,6 +19,10 @@ vp = ctypes.c_void_p
 P = ctypes.POINTER
 
 STORMM_OK = 0
+STORMM_ERR_INVALID_ARGUMENT = -1
+STORMM_ERR_HIP = -2
+STORMM_ERR_CAPACITY = -3
+STORMM_ERR_STATE = -4
 PREC_DOUBLE = 0
 PREC_SINGLE = 1
 
@@ -73,9 +77,26 @@ class MinimizeControls(ctypes.Structure):
                 ("initial_step", c_double), ("convergence_rms_force", c_double)]
 
 
+class PoolConfig(ctypes.Structure):
+    _fields_ = [("precision", c_int32), ("gpos_bits", c_int32), ("frc_bits", c_int32),
+                ("nrg_bits", c_int32), ("hbm_budget_bytes", c_int64), ("max_systems", c_int32),
+                ("igb", c_int32), ("dielectric", c_double), ("salt_concentration", c_double),
+                ("coulomb_constant", c_double), ("minimize", MinimizeControls),
+                ("shape", LaunchShape)]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "stormm_last_error": (ctypes.c_char_p, []),
+    "stormm_pool_create": (c_int32, [P(PoolConfig), P(vp)]),
+    "stormm_pool_destroy": (c_int32, [vp]),
+    "stormm_pool_capacity": (c_int32, [vp, P(c_int64), P(c_int32), P(c_int64), P(c_int64)]),
+    "stormm_pool_admit": (c_int32, [vp, P(TopologyBatch), vp, vp]),
+    "stormm_pool_admit_conformers": (c_int32, [vp, P(TopologyBatch), c_int32, vp, vp]),
+    "stormm_pool_release": (c_int32, [vp, vp, c_int32]),
+    "stormm_pool_minimize": (c_int32, [vp, c_int32, P(c_int32), vp]),
+    "stormm_pool_status": (c_int32, [vp, vp, c_int32, vp, vp, vp, vp]),
+    "stormm_pool_coordinates": (c_int32, [vp, c_int32, vp, vp]),
     "stormm_device_info": (c_int32, [P(c_int32), P(c_int32), ctypes.c_char_p, c_int32]),
     "stormm_device_synchronize": (c_int32, []),
     "stormm_synthesis_create": (c_int32, [P(TopologyBatch), P(LaunchShape), P(vp)]),

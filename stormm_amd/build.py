@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libstormm_amd.so")
-SOURCES = ["mm_kernels.hip", "mm_minimize.hip", "probes.hip", "synthesis.cpp"]
+SOURCES = ["mm_kernels.hip", "mm_minimize.hip", "probes.hip", "synthesis.cpp", "plan.cpp", "pool.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("STORMM_AMD_ARCH", "gfx950")
 

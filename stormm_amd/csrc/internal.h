@@ -129,6 +129,7 @@ struct MMKit {
   long long *xfrc, *yfrc, *zfrc;
   long long *scorecard;
   int *status;
+  int store_overwrite;       // 1: forces and energy rows are written, not added to
 };
 
 // Status bits reported by kernels.

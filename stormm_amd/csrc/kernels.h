@@ -14,7 +14,13 @@ hipError_t launch_scorecard_commit(const long long *inst, double *running, doubl
 // Minimizer pieces (mm_minimize.hip).
 struct CgKit {
   int nsys;
+  const int *slots;       // system of each block (null: block index)
   const int *sys_start, *sys_natom;
+  int *steps;             // per-system completed cycles, advanced here (null: use `step`)
+  int *done;              // per-system stop flag (null: none)
+  double *rms;            // per-system RMS force of the last direction update (optional)
+  double conv_rms;        // stop a system when its RMS force falls to this (0 = never)
+  int max_steps;          // stop a system after this many cycles (0 = no limit)
   long long *xfrc, *yfrc, *zfrc;
   long long *xprv, *yprv, *zprv;
   long long *xcg, *ycg, *zcg;
@@ -23,6 +29,8 @@ struct CgKit {
 };
 struct LineKit {
   int nsys;
+  const int *slots;       // system of each block (null: block index)
+  const int *done;        // per-system stop flag (null: none)
   const int *sys_start, *sys_natom;
   long long *xcrd, *ycrd, *zcrd;
   const long long *xfrc, *yfrc, *zfrc;
@@ -49,5 +57,17 @@ struct MinKit {
   double *rms_force;      // per-system final RMS force
 };
 hipError_t launch_minimize_persistent(int precision, const MinKit &k, hipStream_t stream);
+
+// Fresh per-system state of newly admitted pool slots: layout, zero cycle count and stop flag,
+// line-search state primed with dx0, zeroed ScoreCard row.
+struct SlotInitKit {
+  int count, cap_slots;
+  const int *slots, *starts, *natoms;   // per new entry
+  int *slot_start, *slot_natom, *steps, *done;
+  double *rms, *line;                   // line: 9 arrays of cap_slots
+  long long *scorecard;
+  double dx0;
+};
+hipError_t launch_pool_init_slots(const SlotInitKit &k, hipStream_t stream);
 
 }  // namespace stormm_amd

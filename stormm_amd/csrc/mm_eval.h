@@ -1031,9 +1031,16 @@ __device__ void stage_store(StageCtx<T> &c, const MMKit &k, bool eval_energy) {
     for (int a = c.tid; a < c.wu.lds_atoms; a += c.nthr) {
       const int g = L.gidx[a];
       if (g >= 0) {
-        k.xfrc[g] += L.fx[a];
-        k.yfrc[g] += L.fy[a];
-        k.zfrc[g] += L.fz[a];
+        if (k.store_overwrite) {
+          k.xfrc[g] = L.fx[a];
+          k.yfrc[g] = L.fy[a];
+          k.zfrc[g] = L.fz[a];
+        }
+        else {
+          k.xfrc[g] += L.fx[a];
+          k.yfrc[g] += L.fy[a];
+          k.zfrc[g] += L.fz[a];
+        }
       }
     }
   }
@@ -1042,8 +1049,13 @@ __device__ void stage_store(StageCtx<T> &c, const MMKit &k, bool eval_energy) {
       const int s = e / kNrgSlots;
       const int comp = e - (s * kNrgSlots);
       const long long v = L.nrg[s * 16 + comp];
-      if (v != 0) {
-        k.scorecard[(static_cast<size_t>(L.sys[s].sys_index) * kScoreStride) + comp] += v;
+      long long *dst = &k.scorecard[(static_cast<size_t>(L.sys[s].sys_index) * kScoreStride) +
+                                    comp];
+      if (k.store_overwrite) {
+        *dst = v;
+      }
+      else if (v != 0) {
+        *dst += v;
       }
     }
   }

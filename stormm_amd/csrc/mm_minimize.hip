@@ -21,27 +21,54 @@ __device__ __forceinline__ double wave_sum64(double v) {
 // src/Math/conjugate_gradient.cui:259-370 (all-reduce kernel).  The per-lane accumulation order
 // and the butterfly reductions match the persistent minimizer, so both paths produce the same
 // directions bit for bit.
+//
+// With per-system step counters (serving), each system advances its own counter and stops (its
+// `done` flag is set, the direction left untouched and no line move follows) once its RMS force
+// reaches `conv_rms` or its counter reaches `max_steps`: the persistent minimizer's stopping rule.
 __global__ void __launch_bounds__(64) k_conjugate_gradient(CgKit k, int step, int sd_cycles) {
-  const int s = blockIdx.x;
+  const int s = (k.slots != nullptr) ? k.slots[blockIdx.x] : static_cast<int>(blockIdx.x);
   const int lane = threadIdx.x;
+  if (k.done != nullptr && k.done[s] != 0) {
+    return;
+  }
   const int start = k.sys_start[s];
   const int n = k.sys_natom[s];
   long long *frc[3] = {k.xfrc, k.yfrc, k.zfrc};
   long long *prv[3] = {k.xprv, k.yprv, k.zprv};
   long long *cg[3] = {k.xcg, k.ycg, k.zcg};
-  double gg = 0.0, dgg = 0.0;
+  double ff = 0.0, gg = 0.0, dgg = 0.0;
   for (int i = lane; i < n; i += 64) {
     for (int d = 0; d < 3; d++) {
       const long long lf = frc[d][start + i];
       const long long lp = prv[d][start + i];
       const double df = static_cast<double>(lf);
       const double dp = static_cast<double>(lp);
+      ff += df * df;
       gg += dp * dp;
       dgg += static_cast<double>(lf - lp) * df;
     }
   }
+  ff = wave_sum64(ff);
   gg = wave_sum64(gg);
   dgg = wave_sum64(dgg);
+  if (k.steps != nullptr) {
+    const double rms = sqrt(ff / (3.0 * n)) * k.inv_frc_scale;
+    if (lane == 0 && k.rms != nullptr) {
+      k.rms[s] = rms;
+    }
+    const bool stop = (k.conv_rms > 0.0 && rms <= k.conv_rms) ||
+                      (k.max_steps > 0 && k.steps[s] >= k.max_steps);
+    if (stop) {
+      if (lane == 0) {
+        k.done[s] = 1;
+      }
+      return;
+    }
+    step = k.steps[s] + 1;
+    if (lane == 0) {
+      k.steps[s] = step;
+    }
+  }
   const double gam = (step >= sd_cycles) ? (dgg / gg) : 0.0;
   double msum = 0.0;
   for (int i = lane; i < n; i += 64) {
@@ -83,10 +110,14 @@ hipError_t launch_conjugate_gradient(const CgKit &k, int step, int sd_cycles,
   return hipGetLastError();
 }
 
-// One line move per system, src/MolecularMechanics/line_movement.cui:4-415.
+// One line move per system, src/MolecularMechanics/line_movement.cui:4-415.  Stopped systems
+// (serving) keep their coordinates and line state.
 __global__ void __launch_bounds__(256) k_line_advance(LineKit k, int move_number) {
   __shared__ double sh_move;
-  const int s = blockIdx.x;
+  const int s = (k.slots != nullptr) ? k.slots[blockIdx.x] : static_cast<int>(blockIdx.x);
+  if (k.done != nullptr && k.done[s] != 0) {
+    return;
+  }
   if (threadIdx.x == 0) {
     long long acc = 0;
     for (int c = 0; c <= 12; c++) {
@@ -388,6 +419,33 @@ hipError_t launch_minimize_persistent(int precision, const MinKit &k, hipStream_
     return hipSuccess;
   }
   return (precision == 0) ? launch_min_t<double>(k, stream) : launch_min_t<float>(k, stream);
+}
+
+__global__ void __launch_bounds__(256) k_pool_init_slots(SlotInitKit k) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= k.count) {
+    return;
+  }
+  const int s = k.slots[e];
+  k.slot_start[s] = k.starts[e];
+  k.slot_natom[s] = k.natoms[e];
+  k.steps[s] = 0;
+  k.done[s] = 0;
+  k.rms[s] = 0.0;
+  for (int v = 0; v < 9; v++) {
+    k.line[static_cast<size_t>(v) * k.cap_slots + s] = (v == 0) ? k.dx0 : 0.0;
+  }
+  for (int c = 0; c < kScoreStride; c++) {
+    k.scorecard[static_cast<size_t>(s) * kScoreStride + c] = 0;
+  }
+}
+
+hipError_t launch_pool_init_slots(const SlotInitKit &k, hipStream_t stream) {
+  if (k.count == 0) {
+    return hipSuccess;
+  }
+  hipLaunchKernelGGL(k_pool_init_slots, dim3((k.count + 255) / 256), dim3(256), 0, stream, k);
+  return hipGetLastError();
 }
 
 }  // namespace stormm_amd

@@ -15,8 +15,10 @@
 #include <vector>
 
 #include "../../include/stormm_amd.h"
+#include "host_common.h"
 #include "internal.h"
 #include "kernels.h"
+#include "plan.h"
 
 using namespace stormm_amd;
 
@@ -25,735 +27,14 @@ using namespace stormm_amd;
 // ------------------------------------------------------------------------------------------
 static thread_local std::string g_last_error;
 
-static int fail(int code, const std::string &msg) {
+namespace stormm_amd {
+int fail(int code, const std::string &msg) {
   g_last_error = msg;
   return code;
 }
-
-#define HIP_TRY(expr)                                                                      \
-  do {                                                                                     \
-    hipError_t _e = (expr);                                                                \
-    if (_e != hipSuccess) {                                                                \
-      return fail(STORMM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));      \
-    }                                                                                      \
-  } while (0)
+}  // namespace stormm_amd
 
 extern "C" const char *stormm_last_error(void) { return g_last_error.c_str(); }
-
-// ------------------------------------------------------------------------------------------
-// Device buffers
-// ------------------------------------------------------------------------------------------
-struct DevMem {
-  void *ptr = nullptr;
-  size_t bytes = 0;
-  DevMem() = default;
-  DevMem(const DevMem &) = delete;
-  DevMem &operator=(const DevMem &) = delete;
-  ~DevMem() {
-    if (ptr != nullptr) {
-      (void)hipFree(ptr);
-    }
-  }
-  hipError_t alloc(size_t n) {
-    bytes = n;
-    if (n == 0) {
-      return hipSuccess;
-    }
-    return hipMalloc(&ptr, n);
-  }
-  template <typename T> hipError_t upload(const std::vector<T> &v) {
-    hipError_t e = alloc(v.size() * sizeof(T));
-    if (e != hipSuccess || v.empty()) {
-      return e;
-    }
-    return hipMemcpy(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-  }
-  template <typename T> T *as() const { return static_cast<T *>(ptr); }
-};
-
-// ------------------------------------------------------------------------------------------
-// Host plan of a synthesis (no device memory)
-// ------------------------------------------------------------------------------------------
-// Offsets of one system's terms in the concatenated topology batch.
-struct SystemTerms {
-  int bond_first = 0, bond_count = 0;
-  int angl_first = 0, angl_count = 0;
-  int dihe_first = 0, dihe_count = 0;
-  int excl_first = 0, excl_count = 0;
-  int lj_types = 0;
-  bool combining = false;  // per-type sigma / epsilon given (Lorentz-Berthelot), no tables
-  int lj_first = 0;        // table block (T^2 entries) when !combining
-  int sig_first = 0;       // per-type parameters (T entries) when combining
-};
-
-// Work template of one topology: its tile tasks (off-diagonal, then diagonal), exclusion masks,
-// valence terms and parameters, with atom indices relative to the system, plus its LJ table.
-struct TopoTemplate {
-  int natom = 0;
-  int source_system = 0;  // a system of the batch with this topology
-  int task_first = 0, offd_count = 0, diag_count = 0;
-  int term_first = 0, term_count = 0;
-  int lj_off = 0, lj_nt = 0;
-  bool lb = false;        // the LJ table follows Lorentz-Berthelot combining
-  std::vector<double> sig, eps;
-  double cost = 0.0;      // estimated evaluation cost (rotation steps of the pair passes)
-};
-
-struct SynthesisPlan {
-  int nsys = 0;
-  int padded_atoms = 0;
-  int igb = 0;
-  double dielectric = 78.5, kappa = 0.0, coulomb = 332.05221729;
-  int maxa = 0;
-  int block = kEvalBlock;
-  std::vector<int> atom_starts, atom_counts;
-  std::vector<double> q, ljp1, ljp2, rho, scr, pbr, alpha, beta, gamma;
-  std::vector<int> lj_type;
-  std::vector<double2> ljab, ljab14;
-  std::vector<char> lj_table_mode;
-  std::vector<TopoTemplate> templates;
-  std::vector<int> sys_template;
-  std::vector<WuDesc> wu;
-  std::vector<WuSys> wu_sys;
-  std::vector<Task> tasks;
-  std::vector<unsigned> masks;
-  std::vector<int4> vterms;
-  std::vector<double2> bond_par, angl_par, dihe14;
-  std::vector<DiheParam> dihe_par;
-  long long pairs = 0;
-  long long task_count = 0;
-  long long vterm_count = 0;
-  int lj_table_systems = 0;
-};
-
-static int round_up(int v, int m) { return ((v + m - 1) / m) * m; }
-
-// Per-type sigma and epsilon when the table follows Lorentz-Berthelot combining.
-static bool lorentz_berthelot(int t, const double *A, const double *B, const double *A14,
-                              const double *B14, std::vector<double> &sig,
-                              std::vector<double> &eps) {
-  sig.assign(t, 0.0);
-  eps.assign(t, 0.0);
-  for (int i = 0; i < t * t; i++) {
-    if (A[i] != A14[i] || B[i] != B14[i]) {
-      return false;
-    }
-  }
-  for (int i = 0; i < t; i++) {
-    const double a = A[i * t + i], b = B[i * t + i];
-    if (a > 0.0 && b > 0.0) {
-      sig[i] = std::pow(a / b, 1.0 / 6.0);
-      eps[i] = (b * b) / (4.0 * a);
-    }
-    else if (a != 0.0 || b != 0.0) {
-      return false;
-    }
-  }
-  for (int i = 0; i < t; i++) {
-    for (int j = 0; j < t; j++) {
-      const double s = 0.5 * (sig[i] + sig[j]);
-      const double e = std::sqrt(eps[i] * eps[j]);
-      const double s2 = s * s;
-      const double s6 = s2 * s2 * s2;
-      const double a_lb = 4.0 * e * s6 * s6;
-      const double b_lb = 4.0 * e * s6;
-      const double a = A[i * t + j], b = B[i * t + j];
-      if (std::fabs(a_lb - a) > 1.0e-10 * std::max(std::fabs(a), 1.0e-300) ||
-          std::fabs(b_lb - b) > 1.0e-10 * std::max(std::fabs(b), 1.0e-300)) {
-        return false;
-      }
-    }
-  }
-  return true;
-}
-
-static int cu_count_or_default() {
-  int dev = 0;
-  hipDeviceProp_t prop;
-  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
-      prop.multiProcessorCount > 0) {
-    return prop.multiProcessorCount;
-  }
-  return 256;
-}
-
-// FNV-1a over the parts of a system's topology that its work template depends on.
-struct Fnv {
-  unsigned long long h = 1469598103934665603ULL;
-  void bytes(const void *p, size_t n) {
-    const unsigned char *c = static_cast<const unsigned char *>(p);
-    for (size_t i = 0; i < n; i++) {
-      h = (h ^ c[i]) * 1099511628211ULL;
-    }
-  }
-  template <typename T> void span(const T *p, size_t n) {
-    if (p != nullptr && n > 0) {
-      bytes(p, n * sizeof(T));
-    }
-  }
-};
-
-static unsigned long long topology_hash(const stormm_topology_batch *b, const SystemTerms &t,
-                                        int natom) {
-  Fnv f;
-  f.bytes(&natom, sizeof(natom));
-  f.bytes(&t.lj_types, sizeof(t.lj_types));
-  f.bytes(&t.combining, sizeof(t.combining));
-  f.span(b->exclusion_pairs + 2 * t.excl_first, 2 * static_cast<size_t>(t.excl_count));
-  f.span(b->bond_atoms + 2 * t.bond_first, 2 * static_cast<size_t>(t.bond_count));
-  f.span(b->bond_k + t.bond_first, t.bond_count);
-  f.span(b->bond_r0 + t.bond_first, t.bond_count);
-  f.span(b->angle_atoms + 3 * t.angl_first, 3 * static_cast<size_t>(t.angl_count));
-  f.span(b->angle_k + t.angl_first, t.angl_count);
-  f.span(b->angle_theta + t.angl_first, t.angl_count);
-  f.span(b->dihedral_atoms + 4 * t.dihe_first, 4 * static_cast<size_t>(t.dihe_count));
-  f.span(b->dihe_amp + t.dihe_first, t.dihe_count);
-  f.span(b->dihe_phase + t.dihe_first, t.dihe_count);
-  f.span(b->dihe_freq + t.dihe_first, t.dihe_count);
-  f.span(b->dihe_improper + t.dihe_first, t.dihe_count);
-  f.span(b->dihe_scee + t.dihe_first, t.dihe_count);
-  f.span(b->dihe_scnb + t.dihe_first, t.dihe_count);
-  if (t.combining) {
-    f.span(b->lj_sigma + t.sig_first, t.lj_types);
-    f.span(b->lj_epsilon + t.sig_first, t.lj_types);
-  }
-  else {
-    const size_t t2 = static_cast<size_t>(t.lj_types) * t.lj_types;
-    f.span(b->lja + t.lj_first, t2);
-    f.span(b->ljb + t.lj_first, t2);
-    f.span(b->lja14 + t.lj_first, t2);
-    f.span(b->ljb14 + t.lj_first, t2);
-  }
-  return f.h;
-}
-
-template <typename T>
-static bool same_span(const T *base, int first_a, int first_b, size_t n) {
-  if (n == 0 || base == nullptr) {
-    return true;
-  }
-  return std::memcmp(base + first_a, base + first_b, n * sizeof(T)) == 0;
-}
-
-static bool same_topology(const stormm_topology_batch *b, const SystemTerms &x, int nx,
-                          const SystemTerms &y, int ny) {
-  if (nx != ny || x.bond_count != y.bond_count || x.angl_count != y.angl_count ||
-      x.dihe_count != y.dihe_count || x.excl_count != y.excl_count ||
-      x.lj_types != y.lj_types || x.combining != y.combining) {
-    return false;
-  }
-  const size_t t2 = x.combining ? 0 : static_cast<size_t>(x.lj_types) * x.lj_types;
-  const size_t t1 = x.combining ? static_cast<size_t>(x.lj_types) : 0;
-  return same_span(b->lj_sigma, x.sig_first, y.sig_first, t1) &&
-         same_span(b->lj_epsilon, x.sig_first, y.sig_first, t1) &&
-         same_span(b->exclusion_pairs, 2 * x.excl_first, 2 * y.excl_first, 2 * x.excl_count) &&
-         same_span(b->bond_atoms, 2 * x.bond_first, 2 * y.bond_first, 2 * x.bond_count) &&
-         same_span(b->bond_k, x.bond_first, y.bond_first, x.bond_count) &&
-         same_span(b->bond_r0, x.bond_first, y.bond_first, x.bond_count) &&
-         same_span(b->angle_atoms, 3 * x.angl_first, 3 * y.angl_first, 3 * x.angl_count) &&
-         same_span(b->angle_k, x.angl_first, y.angl_first, x.angl_count) &&
-         same_span(b->angle_theta, x.angl_first, y.angl_first, x.angl_count) &&
-         same_span(b->dihedral_atoms, 4 * x.dihe_first, 4 * y.dihe_first, 4 * x.dihe_count) &&
-         same_span(b->dihe_amp, x.dihe_first, y.dihe_first, x.dihe_count) &&
-         same_span(b->dihe_phase, x.dihe_first, y.dihe_first, x.dihe_count) &&
-         same_span(b->dihe_freq, x.dihe_first, y.dihe_first, x.dihe_count) &&
-         same_span(b->dihe_improper, x.dihe_first, y.dihe_first, x.dihe_count) &&
-         same_span(b->dihe_scee, x.dihe_first, y.dihe_first, x.dihe_count) &&
-         same_span(b->dihe_scnb, x.dihe_first, y.dihe_first, x.dihe_count) &&
-         same_span(b->lja, x.lj_first, y.lj_first, t2) &&
-         same_span(b->ljb, x.lj_first, y.lj_first, t2) &&
-         same_span(b->lja14, x.lj_first, y.lj_first, t2) &&
-         same_span(b->ljb14, x.lj_first, y.lj_first, t2);
-}
-
-// Builds the work template of system s (tile tasks with exclusion masks, valence terms and
-// their parameters, the LJ table) and appends it to the plan.
-static int add_template(const stormm_topology_batch *b, const SystemTerms &t, int n, int s,
-                        SynthesisPlan &P) {
-  TopoTemplate tp;
-  tp.natom = n;
-  tp.source_system = s;
-  // Lennard-Jones: per-type Lorentz-Berthelot parameters, or a table that may follow the rule
-  // (then per-atom parameters replace it) or must be read as stored.
-  const int nt_lj = t.lj_types;
-  tp.lj_nt = nt_lj;
-  tp.lj_off = 0;
-  if (t.combining) {
-    tp.lb = true;
-    tp.sig.assign(b->lj_sigma + t.sig_first, b->lj_sigma + t.sig_first + nt_lj);
-    tp.eps.assign(b->lj_epsilon + t.sig_first, b->lj_epsilon + t.sig_first + nt_lj);
-  }
-  else {
-    const double *A = b->lja + t.lj_first, *B = b->ljb + t.lj_first;
-    const double *A14 = b->lja14 + t.lj_first, *B14 = b->ljb14 + t.lj_first;
-    tp.lb = lorentz_berthelot(nt_lj, A, B, A14, B14, tp.sig, tp.eps);
-    if (!tp.lb) {
-      tp.lj_off = static_cast<int>(P.ljab.size());
-      for (int k = 0; k < nt_lj * nt_lj; k++) {
-        P.ljab.push_back(make_double2(A[k], B[k]));
-        P.ljab14.push_back(make_double2(A14[k], B14[k]));
-      }
-    }
-  }
-  // Exclusion matrix (bit per pair): the given list, or, when the batch carries none, every pair
-  // within three bonds (1:2, 1:3 and 1:4 by shortest bond path, the reference's exclusion map
-  // for systems without virtual sites, src/Topology/atomgraph_refinement.cpp).
-  std::vector<unsigned char> ex(static_cast<size_t>(n) * n, 0);
-  if (b->exclusion_counts != nullptr) {
-    for (int e = 0; e < t.excl_count; e++) {
-      const int i = b->exclusion_pairs[2 * (t.excl_first + e)];
-      const int j = b->exclusion_pairs[2 * (t.excl_first + e) + 1];
-      if (i < 0 || j < 0 || i >= n || j >= n) {
-        return fail(STORMM_ERR_INVALID_ARGUMENT, "exclusion index out of range in system " +
-                                                     std::to_string(s));
-      }
-      ex[static_cast<size_t>(i) * n + j] = 1;
-      ex[static_cast<size_t>(j) * n + i] = 1;
-    }
-  }
-  else {
-    std::vector<std::vector<int>> adj(n);
-    for (int e = 0; e < t.bond_count; e++) {
-      const int k = t.bond_first + e;
-      const int i = b->bond_atoms[2 * k], j = b->bond_atoms[2 * k + 1];
-      if (i < 0 || j < 0 || i >= n || j >= n) {
-        return fail(STORMM_ERR_INVALID_ARGUMENT, "bond index out of range");
-      }
-      adj[i].push_back(j);
-      adj[j].push_back(i);
-    }
-    std::vector<int> dist(n, -1), frontier, next, seen;
-    for (int src = 0; src < n; src++) {
-      frontier.assign(1, src);
-      seen.assign(1, src);
-      dist[src] = 0;
-      for (int depth = 1; depth <= 3 && !frontier.empty(); depth++) {
-        next.clear();
-        for (int u : frontier) {
-          for (int v : adj[u]) {
-            if (dist[v] < 0) {
-              dist[v] = depth;
-              next.push_back(v);
-              seen.push_back(v);
-              ex[static_cast<size_t>(src) * n + v] = 1;
-            }
-          }
-        }
-        frontier.swap(next);
-      }
-      for (int v : seen) {
-        dist[v] = -1;
-      }
-    }
-  }
-  // Tile tasks: off-diagonal tiles first, then the diagonal ones.
-  const int nt = round_up(n, kTile) / kTile;
-  tp.task_first = static_cast<int>(P.tasks.size());
-  for (int pass = 0; pass < 2; pass++) {
-    for (int ti = 0; ti < nt; ti++) {
-      for (int tj = ti; tj < nt; tj++) {
-        const bool diag = (ti == tj);
-        if (diag != (pass == 1)) {
-          continue;
-        }
-        const int ni = std::min(kTile, n - ti * kTile);
-        const int nj = std::min(kTile, n - tj * kTile);
-        unsigned m[kTile];
-        bool any = false;
-        for (int il = 0; il < kTile; il++) {
-          m[il] = 0u;
-          if (il >= ni) {
-            continue;
-          }
-          for (int jl = 0; jl < nj; jl++) {
-            if (ex[static_cast<size_t>(ti * kTile + il) * n + (tj * kTile + jl)]) {
-              m[il] |= (1u << jl);
-              any = true;
-            }
-          }
-        }
-        Task tk;
-        tk.x = (ti * kTile) | ((tj * kTile) << 16);
-        tk.y = ((diag ? 1 : 0) << 8) | (ni << 12) | (nj << 20);
-        tk.z = -1;
-        tk.w = 0;
-        if (any) {
-          tk.z = static_cast<int>(P.masks.size());
-          for (int il = 0; il < kTile; il++) {
-            P.masks.push_back(m[il]);
-          }
-        }
-        P.tasks.push_back(tk);
-        if (diag) {
-          tp.diag_count++;
-        }
-        else {
-          tp.offd_count++;
-        }
-      }
-    }
-  }
-  // Valence terms with system-relative atom indices.
-  tp.term_first = static_cast<int>(P.vterms.size());
-  for (int e = 0; e < t.bond_count; e++) {
-    const int k = t.bond_first + e;
-    const int i = b->bond_atoms[2 * k], j = b->bond_atoms[2 * k + 1];
-    if (i < 0 || j < 0 || i >= n || j >= n) {
-      return fail(STORMM_ERR_INVALID_ARGUMENT, "bond index out of range");
-    }
-    P.vterms.push_back(make_int4(kVBond, i | (j << 16), 0,
-                                 static_cast<int>(P.bond_par.size())));
-    P.bond_par.push_back(make_double2(b->bond_k[k], std::fabs(b->bond_r0[k])));
-  }
-  for (int e = 0; e < t.angl_count; e++) {
-    const int k = t.angl_first + e;
-    const int i = b->angle_atoms[3 * k], j = b->angle_atoms[3 * k + 1];
-    const int kk = b->angle_atoms[3 * k + 2];
-    if (i < 0 || j < 0 || kk < 0 || i >= n || j >= n || kk >= n) {
-      return fail(STORMM_ERR_INVALID_ARGUMENT, "angle index out of range");
-    }
-    P.vterms.push_back(make_int4(kVAngle, i | (j << 16), kk,
-                                 static_cast<int>(P.angl_par.size())));
-    P.angl_par.push_back(make_double2(b->angle_k[k], b->angle_theta[k]));
-  }
-  for (int e = 0; e < t.dihe_count; e++) {
-    const int k = t.dihe_first + e;
-    const int *q4 = &b->dihedral_atoms[4 * k];
-    for (int m = 0; m < 4; m++) {
-      if (q4[m] < 0 || q4[m] >= n) {
-        return fail(STORMM_ERR_INVALID_ARGUMENT, "dihedral index out of range");
-      }
-    }
-    const double freq = b->dihe_freq[k];
-    if (!(freq >= 0.0 && freq <= kMaxDihePeriod && freq == std::rint(freq))) {
-      return fail(STORMM_ERR_INVALID_ARGUMENT,
-                  "dihedral periodicity must be an integer between 0 and 32");
-    }
-    P.vterms.push_back(make_int4(kVDihedral, q4[0] | (q4[1] << 16), q4[2] | (q4[3] << 16),
-                                 static_cast<int>(P.dihe_par.size())));
-    DiheParam dp;
-    dp.amp = b->dihe_amp[k];
-    dp.cos_phase = std::cos(b->dihe_phase[k]);
-    dp.sin_phase = std::sin(b->dihe_phase[k]);
-    dp.period = static_cast<int>(freq);
-    dp.improper = b->dihe_improper[k] ? 1 : 0;
-    P.dihe_par.push_back(dp);
-    const double scee = b->dihe_scee[k], scnb = b->dihe_scnb[k];
-    P.dihe14.push_back(make_double2(scee > 0.0 ? 1.0 / scee : 0.0,
-                                    (scee > 0.0 && scnb > 0.0) ? 1.0 / scnb : 0.0));
-  }
-  tp.term_count = static_cast<int>(P.vterms.size()) - tp.term_first;
-  // Cost model: rotation steps of the three pair passes plus valence work.
-  tp.cost = (16.0 * tp.offd_count) + (9.0 * tp.diag_count) +
-            0.25 * (t.bond_count + t.angl_count + 2 * t.dihe_count) + 0.1 * n;
-  P.templates.push_back(std::move(tp));
-  return STORMM_OK;
-}
-
-// Longest-processing-time assignment of systems onto about `target_total` work units, within
-// the LDS atom capacity and the per-unit system limit; stored-table and combining-rule systems
-// go to separate units.  Units come out in order of decreasing cost.
-static void pack_systems(const std::vector<double> &cost, const std::vector<int> &natom,
-                         const std::vector<char> &table_mode, int cap, int target_total,
-                         std::vector<std::vector<int>> &bins_out, std::vector<int> &mode_out) {
-  const int ns = static_cast<int>(cost.size());
-  double total_cost = 0.0;
-  for (int s = 0; s < ns; s++) {
-    total_cost += cost[s];
-  }
-  struct Bin {
-    double cost = 0.0;
-    int atoms = 0;
-    int mode = 0;
-    std::vector<int> systems;
-  };
-  std::vector<Bin> bins;
-  for (int mode = 0; mode < 2; mode++) {
-    std::vector<int> order;
-    double mode_cost = 0.0;
-    for (int s = 0; s < ns; s++) {
-      if (table_mode[s] == mode) {
-        order.push_back(s);
-        mode_cost += cost[s];
-      }
-    }
-    if (order.empty()) {
-      continue;
-    }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
-    const int target = std::max(1, static_cast<int>(std::lround(target_total * mode_cost /
-                                                                std::max(total_cost, 1.0))));
-    const size_t first_bin = bins.size();
-    for (int s : order) {
-      const int need = round_up(natom[s], kTile);
-      int best = -1;
-      for (size_t k = first_bin; k < bins.size(); k++) {
-        if (bins[k].atoms + need <= cap &&
-            static_cast<int>(bins[k].systems.size()) < kMaxWuSystems &&
-            (best < 0 || bins[k].cost < bins[best].cost)) {
-          best = static_cast<int>(k);
-        }
-      }
-      const bool open_new = (best < 0) ||
-                            (static_cast<int>(bins.size() - first_bin) < target &&
-                             bins[best].cost > 0.0);
-      if (open_new) {
-        Bin nb;
-        nb.mode = mode;
-        bins.push_back(nb);
-        best = static_cast<int>(bins.size()) - 1;
-      }
-      bins[best].systems.push_back(s);
-      bins[best].atoms += need;
-      bins[best].cost += cost[s];
-    }
-  }
-  std::stable_sort(bins.begin(), bins.end(), [](const Bin &a, const Bin &c) {
-    return a.cost > c.cost;
-  });
-  bins_out.clear();
-  mode_out.clear();
-  for (Bin &bn : bins) {
-    bins_out.push_back(std::move(bn.systems));
-    mode_out.push_back(bn.mode);
-  }
-}
-
-// Appends one work unit holding `systems` to the plan; returns its LDS atom count.
-static int append_work_unit(const std::vector<int> &systems, int lj_table, SynthesisPlan &P) {
-  WuDesc d;
-  d.sys_first = static_cast<int>(P.wu_sys.size());
-  d.sys_count = static_cast<int>(systems.size());
-  d.offd_count = 0;
-  d.task_count = 0;
-  d.term_count = 0;
-  d.lj_table = lj_table;
-  d.pad = 0;
-  int lds = 0;
-  for (int s : systems) {
-    const TopoTemplate &tp = P.templates[P.sys_template[s]];
-    WuSys ws;
-    ws.sys_index = s;
-    ws.lds_base = lds;
-    ws.global_start = P.atom_starts[s];
-    ws.natom = P.atom_counts[s];
-    ws.task_first = tp.task_first;
-    ws.offd_count = tp.offd_count;
-    ws.diag_count = tp.diag_count;
-    ws.term_first = tp.term_first;
-    ws.term_count = tp.term_count;
-    ws.lj_off = tp.lj_off;
-    ws.lj_nt = tp.lj_nt;
-    ws.pad = 0;
-    P.wu_sys.push_back(ws);
-    d.offd_count += tp.offd_count;
-    d.task_count += tp.offd_count + tp.diag_count;
-    d.term_count += tp.term_count;
-    lds += round_up(P.atom_counts[s], kTile);
-  }
-  d.lds_atoms = lds;
-  P.wu.push_back(d);
-  return lds;
-}
-
-static int build_plan(const stormm_topology_batch *b, const stormm_launch_shape *shape_in,
-                      SynthesisPlan &P) {
-  if (b == nullptr || b->system_count <= 0) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "topology batch is empty");
-  }
-  if (b->igb != 0 && b->igb != 1 && b->igb != 2 && b->igb != 5) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "unsupported implicit solvent model (igb must be "
-                                             "0, 1, 2 or 5)");
-  }
-  stormm_launch_shape shape = {0, 0, 0};
-  if (shape_in != nullptr) {
-    shape = *shape_in;
-  }
-  const int cap = (shape.wu_atom_capacity > 0) ? shape.wu_atom_capacity : kMaxWuAtoms;
-  if (cap < kTile || cap > kMaxWuAtoms || (cap % kTile) != 0) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "work-unit capacity must be a multiple of 16 in "
-                                             "[16, 512]");
-  }
-  const int wu_per_cu = (shape.target_wu_per_cu > 0) ? shape.target_wu_per_cu : 2;
-  const int ns = b->system_count;
-  P.nsys = ns;
-  P.igb = b->igb;
-  P.dielectric = (b->dielectric > 0.0) ? b->dielectric : 78.5;
-  P.kappa = std::sqrt(0.10806 * std::max(b->salt_concentration, 0.0));
-  P.coulomb = (b->coulomb_constant > 0.0) ? b->coulomb_constant : 332.0522172900000;
-  const double sqrt_coul = std::sqrt(P.coulomb);
-
-  // Global layout: each system starts on a 32-atom boundary.
-  P.atom_starts.resize(ns);
-  P.atom_counts.resize(ns);
-  int run = 0;
-  for (int s = 0; s < ns; s++) {
-    const int n = b->atom_counts[s];
-    if (n <= 0 || n > kMaxWuAtoms) {
-      return fail(STORMM_ERR_CAPACITY, "system " + std::to_string(s) + " has " +
-                                           std::to_string(n) + " atoms (supported: 1-512)");
-    }
-    if (round_up(n, kTile) > cap) {
-      return fail(STORMM_ERR_CAPACITY, "system " + std::to_string(s) + " does not fit the "
-                                           "work-unit capacity of " + std::to_string(cap));
-    }
-    P.atom_starts[s] = run;
-    P.atom_counts[s] = n;
-    run += round_up(n, kGlobalPad);
-    P.pairs += static_cast<long long>(n) * (n - 1) / 2;
-  }
-  P.padded_atoms = run;
-  const int pa = run;
-  P.q.assign(pa, 0.0);
-  P.ljp1.assign(pa, 0.0);
-  P.ljp2.assign(pa, 0.0);
-  P.rho.assign(pa, 0.0);
-  P.scr.assign(pa, 0.0);
-  P.pbr.assign(pa, 0.0);
-  P.alpha.assign(pa, 0.0);
-  P.beta.assign(pa, 0.0);
-  P.gamma.assign(pa, 0.0);
-  P.lj_type.assign(pa, 0);
-  double ga = 0.0, gb = 0.0, gg = 0.0;
-  if (b->igb == 2) {
-    ga = 0.8; gb = 0.0; gg = 2.909125;
-  }
-  else if (b->igb == 5) {
-    ga = 1.0; gb = 0.8; gg = 4.85;
-  }
-
-  // Term offsets per system.
-  std::vector<SystemTerms> st(ns);
-  {
-    int ab = 0, an = 0, ad = 0, ae = 0, at = 0, asg = 0;
-    for (int s = 0; s < ns; s++) {
-      st[s].bond_first = ab;
-      st[s].bond_count = b->bond_counts ? b->bond_counts[s] : 0;
-      ab += st[s].bond_count;
-      st[s].angl_first = an;
-      st[s].angl_count = b->angle_counts ? b->angle_counts[s] : 0;
-      an += st[s].angl_count;
-      st[s].dihe_first = ad;
-      st[s].dihe_count = b->dihedral_counts ? b->dihedral_counts[s] : 0;
-      ad += st[s].dihe_count;
-      st[s].excl_first = ae;
-      st[s].excl_count = b->exclusion_counts ? b->exclusion_counts[s] : 0;
-      ae += st[s].excl_count;
-      st[s].lj_types = b->lj_type_counts[s];
-      if (st[s].lj_types <= 0) {
-        return fail(STORMM_ERR_INVALID_ARGUMENT, "system " + std::to_string(s) +
-                                                     " has no Lennard-Jones types");
-      }
-      st[s].combining = (b->lj_combining != nullptr) && (b->lj_combining[s] != 0);
-      if (st[s].combining) {
-        if (b->lj_sigma == nullptr || b->lj_epsilon == nullptr) {
-          return fail(STORMM_ERR_INVALID_ARGUMENT, "combining-rule systems need lj_sigma and "
-                                                   "lj_epsilon");
-        }
-        st[s].sig_first = asg;
-        asg += st[s].lj_types;
-      }
-      else {
-        if (b->lja == nullptr || b->ljb == nullptr || b->lja14 == nullptr ||
-            b->ljb14 == nullptr) {
-          return fail(STORMM_ERR_INVALID_ARGUMENT, "system " + std::to_string(s) +
-                                                       " has no Lennard-Jones tables");
-        }
-        st[s].lj_first = at;
-        at += st[s].lj_types * st[s].lj_types;
-      }
-    }
-  }
-
-  // Topology templates: systems with identical atom counts, exclusions, valence terms and LJ
-  // tables share one set of tile tasks, exclusion masks, valence terms and parameters.
-  P.sys_template.assign(ns, -1);
-  std::unordered_map<unsigned long long, std::vector<int>> by_hash;
-  for (int s = 0; s < ns; s++) {
-    const unsigned long long h = topology_hash(b, st[s], b->atom_counts[s]);
-    std::vector<int> &bucket = by_hash[h];
-    int tid = -1;
-    for (int cand : bucket) {
-      const int src = P.templates[cand].source_system;
-      if (same_topology(b, st[src], b->atom_counts[src], st[s], b->atom_counts[s])) {
-        tid = cand;
-        break;
-      }
-    }
-    if (tid < 0) {
-      const int rc = add_template(b, st[s], b->atom_counts[s], s, P);
-      if (rc != STORMM_OK) {
-        return rc;
-      }
-      tid = static_cast<int>(P.templates.size()) - 1;
-      bucket.push_back(tid);
-    }
-    P.sys_template[s] = tid;
-  }
-
-  // Per-atom parameters.
-  P.lj_table_mode.assign(ns, 0);
-  int atom_cursor = 0;
-  for (int s = 0; s < ns; s++) {
-    const TopoTemplate &tp = P.templates[P.sys_template[s]];
-    P.lj_table_mode[s] = tp.lb ? 0 : 1;
-    P.lj_table_systems += tp.lb ? 0 : 1;
-    const int t = st[s].lj_types;
-    const int n = b->atom_counts[s];
-    for (int i = 0; i < n; i++) {
-      const int g = P.atom_starts[s] + i;
-      const int src = atom_cursor + i;
-      const int ty = b->lj_type[src];
-      if (ty < 0 || ty >= t) {
-        return fail(STORMM_ERR_INVALID_ARGUMENT, "LJ type index out of range in system " +
-                                                     std::to_string(s));
-      }
-      P.q[g] = b->charge[src] * sqrt_coul;
-      P.lj_type[g] = ty;
-      if (tp.lb) {
-        P.ljp1[g] = tp.sig[ty];
-        P.ljp2[g] = 2.0 * std::sqrt(tp.eps[ty]);
-      }
-      const double pr = (b->pb_radii != nullptr) ? b->pb_radii[src] : 0.0;
-      const double sc = (b->gb_screen != nullptr) ? b->gb_screen[src] : 0.0;
-      P.pbr[g] = pr;
-      P.rho[g] = pr - 0.09;
-      P.scr[g] = sc * (pr - 0.09);
-      P.alpha[g] = ga;
-      P.beta[g] = gb;
-      P.gamma[g] = gg;
-      if (b->igb != 0 && P.rho[g] <= 0.0) {
-        return fail(STORMM_ERR_INVALID_ARGUMENT, "Born radius must exceed 0.09 A (system " +
-                                                     std::to_string(s) + ")");
-      }
-    }
-    atom_cursor += n;
-    P.task_count += tp.offd_count + tp.diag_count;
-    P.vterm_count += tp.term_count;
-  }
-
-  // Work-unit packing: longest-processing-time assignment onto a target number of units,
-  // respecting the LDS atom capacity and the per-unit system limit; systems that read stored
-  // LJ tables are packed separately from combining-rule systems.
-  const int cus = cu_count_or_default();
-  std::vector<double> cost(ns);
-  for (int s = 0; s < ns; s++) {
-    cost[s] = P.templates[P.sys_template[s]].cost;
-  }
-  std::vector<std::vector<int>> bins;
-  std::vector<int> bin_mode;
-  pack_systems(cost, P.atom_counts, P.lj_table_mode, cap, std::max(1, cus * wu_per_cu), bins,
-               bin_mode);
-  P.maxa = 0;
-  for (size_t u = 0; u < bins.size(); u++) {
-    P.maxa = std::max(P.maxa, append_work_unit(bins[u], bin_mode[u], P));
-  }
-  P.block = (shape.block_threads > 0) ? shape.block_threads : kEvalBlock;
-  if (P.block != 256 && P.block != 512 && P.block != 1024) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "block_threads must be 256, 512 or 1024");
-  }
-  return STORMM_OK;
-}
 
 // ------------------------------------------------------------------------------------------
 // Objects
@@ -845,27 +126,27 @@ extern "C" int stormm_synthesis_create(const stormm_topology_batch *batch,
   SynthesisPlan &P = ag->plan;
   HIP_TRY(ag->atom_starts.upload(P.atom_starts));
   HIP_TRY(ag->atom_counts.upload(P.atom_counts));
-  HIP_TRY(ag->q.upload(P.q));
-  HIP_TRY(ag->ljp1.upload(P.ljp1));
-  HIP_TRY(ag->ljp2.upload(P.ljp2));
-  HIP_TRY(ag->lj_type.upload(P.lj_type));
-  HIP_TRY(ag->ljab.upload(P.ljab));
-  HIP_TRY(ag->ljab14.upload(P.ljab14));
-  HIP_TRY(ag->rho.upload(P.rho));
-  HIP_TRY(ag->scr.upload(P.scr));
-  HIP_TRY(ag->pbr.upload(P.pbr));
-  HIP_TRY(ag->alpha.upload(P.alpha));
-  HIP_TRY(ag->beta.upload(P.beta));
-  HIP_TRY(ag->gamma.upload(P.gamma));
+  HIP_TRY(ag->q.upload(P.ap.q));
+  HIP_TRY(ag->ljp1.upload(P.ap.ljp1));
+  HIP_TRY(ag->ljp2.upload(P.ap.ljp2));
+  HIP_TRY(ag->lj_type.upload(P.ap.lj_type));
+  HIP_TRY(ag->ljab.upload(P.ts.ljab));
+  HIP_TRY(ag->ljab14.upload(P.ts.ljab14));
+  HIP_TRY(ag->rho.upload(P.ap.rho));
+  HIP_TRY(ag->scr.upload(P.ap.scr));
+  HIP_TRY(ag->pbr.upload(P.ap.pbr));
+  HIP_TRY(ag->alpha.upload(P.ap.alpha));
+  HIP_TRY(ag->beta.upload(P.ap.beta));
+  HIP_TRY(ag->gamma.upload(P.ap.gamma));
   HIP_TRY(ag->wu.upload(P.wu));
   HIP_TRY(ag->wu_sys.upload(P.wu_sys));
-  HIP_TRY(ag->tasks.upload(P.tasks));
-  HIP_TRY(ag->masks.upload(P.masks));
-  HIP_TRY(ag->vterms.upload(P.vterms));
-  HIP_TRY(ag->bond_par.upload(P.bond_par));
-  HIP_TRY(ag->angl_par.upload(P.angl_par));
-  HIP_TRY(ag->dihe_par.upload(P.dihe_par));
-  HIP_TRY(ag->dihe14.upload(P.dihe14));
+  HIP_TRY(ag->tasks.upload(P.ts.tasks));
+  HIP_TRY(ag->masks.upload(P.ts.masks));
+  HIP_TRY(ag->vterms.upload(P.ts.vterms));
+  HIP_TRY(ag->bond_par.upload(P.ts.bond_par));
+  HIP_TRY(ag->angl_par.upload(P.ts.angl_par));
+  HIP_TRY(ag->dihe_par.upload(P.ts.dihe_par));
+  HIP_TRY(ag->dihe14.upload(P.ts.dihe14));
   std::vector<int> zero_status(1, 0);
   HIP_TRY(ag->status.upload(zero_status));
   std::vector<int> zi(P.nsys, 0);
@@ -878,9 +159,9 @@ extern "C" int stormm_synthesis_create(const stormm_topology_batch *batch,
   k.nwu = static_cast<int>(P.wu.size());
   k.maxa = round_up(std::max(P.maxa, kTile), kTile);
   k.block = P.block;
-  k.igb = P.igb;
-  k.dielectric = P.dielectric;
-  k.kappa = P.kappa;
+  k.igb = P.sm.igb;
+  k.dielectric = P.sm.dielectric;
+  k.kappa = P.sm.kappa;
   k.sys_start = ag->atom_starts.as<int>();
   k.sys_natom = ag->atom_counts.as<int>();
   k.wu = ag->wu.as<WuDesc>();
@@ -1373,6 +654,7 @@ extern "C" int stormm_launch_mm_evaluation(int32_t precision, const stormm_synth
 
 static CgKit make_cg_kit(const stormm_synthesis *ag, stormm_phasespace *ps, int precision) {
   CgKit k;
+  std::memset(&k, 0, sizeof(k));
   k.nsys = ag->plan.nsys;
   k.sys_start = ag->atom_starts.as<int>();
   k.sys_natom = ag->atom_counts.as<int>();
@@ -1409,6 +691,7 @@ extern "C" int stormm_launch_conjugate_gradient(int32_t precision, const stormm_
 static LineKit make_line_kit(const stormm_synthesis *ag, stormm_phasespace *ps,
                              stormm_scorecard *sc, stormm_line_min *lm, int precision) {
   LineKit k;
+  std::memset(&k, 0, sizeof(k));
   k.nsys = ag->plan.nsys;
   k.sys_start = ag->atom_starts.as<int>();
   k.sys_natom = ag->atom_counts.as<int>();
@@ -1462,6 +745,9 @@ extern "C" int stormm_launch_line_advance(int32_t precision, const stormm_synthe
 static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag, stormm_phasespace *ps,
                              const stormm_minimize_controls *mc, stormm_scorecard *sc,
                              stormm_line_min *lm, hipStream_t stream) {
+  // The reference's per-cycle launch sequence (src/MolecularMechanics/hpc_minimization.cu:
+  // 262-405).  The evaluations overwrite the force arrays and ScoreCard rows, which stands in
+  // for the force and ScoreCard zeroing launches that precede them there.
   stormm_mm_controls ctrl;
   ctrl.mc = *mc;
   ctrl.step = 0;
@@ -1469,34 +755,27 @@ static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag, stor
   if (rc != STORMM_OK) {
     return rc;
   }
-  const int prec = precision;
+  const int prec = (precision == STORMM_PREC_DOUBLE) ? 0 : 1;
+  MMKit k = make_kit(ag, ps, sc);
+  k.store_overwrite = 1;
+  const int terms = kTermNonbonded | kTermValence;
   for (int cyc = 0; cyc < mc->total_cycles; cyc++) {
-    if ((rc = stormm_phasespace_initialize_forces(ps, stream)) != STORMM_OK ||
-        (rc = stormm_scorecard_initialize(sc, stream)) != STORMM_OK ||
-        (rc = stormm_launch_mm_evaluation(prec, ag, &ctrl, ps, sc, 1, 1, stream)) != STORMM_OK) {
-      return rc;
-    }
+    HIP_TRY(launch_mm_eval(prec, k, terms, 1, 1, stream));
     ctrl.step += 1;
-    if ((rc = stormm_launch_conjugate_gradient(prec, ag, ps, &ctrl, stream)) != STORMM_OK) {
+    if ((rc = stormm_launch_conjugate_gradient(precision, ag, ps, &ctrl, stream)) != STORMM_OK) {
       return rc;
     }
     for (int move = 0; move < 4; move++) {
       if (move > 0) {
-        if ((rc = stormm_scorecard_initialize(sc, stream)) != STORMM_OK ||
-            (rc = stormm_launch_mm_evaluation(prec, ag, &ctrl, ps, sc, 0, 1, stream)) !=
-                STORMM_OK) {
-          return rc;
-        }
+        HIP_TRY(launch_mm_eval(prec, k, terms, 0, 1, stream));
       }
-      if ((rc = stormm_launch_line_advance(prec, ag, ps, sc, lm, move, stream)) != STORMM_OK) {
+      if ((rc = stormm_launch_line_advance(precision, ag, ps, sc, lm, move, stream)) !=
+          STORMM_OK) {
         return rc;
       }
     }
   }
-  if ((rc = stormm_scorecard_initialize(sc, stream)) != STORMM_OK ||
-      (rc = stormm_launch_mm_evaluation(prec, ag, &ctrl, ps, sc, 0, 1, stream)) != STORMM_OK) {
-    return rc;
-  }
+  HIP_TRY(launch_mm_eval(prec, k, terms, 0, 1, stream));
   return STORMM_OK;
 }
 

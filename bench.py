@@ -16,8 +16,9 @@ one thread per host core (up to 16).
 Two further workloads ride on the same line (skip them with --no-extra):
   "minimization": 4096 systems, N ~ U[16, 128], starts perturbed by N(0, 0.3 A), 500 cycles
     (50 steepest descent, then conjugate gradient) in FP32: the persistent single-launch
-    minimizer, one persistent launch per size class, and the reference's per-step launch
-    sequence, with the oracle's minimizer on host threads as the CPU rate.
+    minimizer, one persistent launch per size class, the reference's per-step launch sequence
+    and that sequence captured in a HIP graph and replayed per cycle, with the oracle's
+    minimizer on host threads as the CPU rate.
   "large_batch": 16384 systems, N ~ U[32, 96]: FP32 and FP64 evaluation throughput and the
     errors of both modes against the FP64 oracle on a 64-system subsample.
   "serving": the served-minimization stream (Poisson arrivals at 200 requests/s over
@@ -74,9 +75,11 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu):
     topos, coords = synthetic.config2(nsys=nsys, seed=2)
     sd, dx0 = 50, 0.01
     mc = S.minimize_controls(cycles, sd, dx0, 0.0)
+    # Work-unit shape from the autotuner (timed force evaluations of a 1024-system sample).
+    shape, tuned_ms = S.autotune(topos[:1024], "single", coords[:1024])
 
     def setup(sub_t, sub_c):
-        ag = S.AtomGraphSynthesis(sub_t)
+        ag = S.AtomGraphSynthesis(sub_t, shape)
         ps = S.PhaseSpaceSynthesis(ag, 32, 23)
         sc = S.ScoreCard(ag.system_count, 26)
         lm = S.LineMinimization(ag.system_count, dx0)
@@ -93,7 +96,7 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu):
 
     # Warm-up on a few systems (kernel loading).
     wag, wps, wsc, wlm = setup(topos[:8], coords[:8])
-    for persistent in (True, False):
+    for persistent in (1, 0, 2):
         S.launch_minimization("single", wag, wps, S.minimize_controls(3, 1, dx0, 0.0), wsc, wlm,
                               persistent=persistent)
     whole = [setup(topos, coords)]
@@ -102,6 +105,9 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu):
     whole = [setup(topos, coords)]
     t_step = run(whole, False)
     e_step = whole[0][2].potential_energies()
+    whole = [setup(topos, coords)]
+    t_graph = run(whole, 2)
+    e_graph = whole[0][2].potential_energies()
     # One persistent launch per size class (atom-count quartiles of the range).
     edges = [16, 44, 72, 100, 129]
     parts = []
@@ -111,10 +117,15 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu):
             parts.append(setup([topos[i] for i in idx], [coords[i] for i in idx]))
     t_class = run(parts, True)
     out = {"systems": nsys, "cycles": cycles, "sd_cycles": sd, "precision": "fp32",
+           "autotuned_shape": shape,
            "persistent_s": t_pers, "per_size_class_s": t_class, "per_step_launches_s": t_step,
-           "minimizations_per_s": nsys / t_pers,
+           "graph_replay_s": t_graph,
+           "minimizations_per_s": nsys / min(t_pers, t_graph),
+           "best_mode": "persistent" if t_pers <= t_graph else "graph_replay",
            "persistent_vs_per_step": t_step / t_pers,
+           "graph_vs_per_step": t_step / t_graph,
            "persistent_equals_per_step": bool(np.array_equal(e_pers, e_step)),
+           "graph_equals_per_step": bool(np.array_equal(e_graph, e_step)),
            "size_classes": len(parts)}
     if with_cpu:
         from concurrent.futures import ThreadPoolExecutor

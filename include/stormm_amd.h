@@ -175,9 +175,13 @@ int stormm_synthesis_plan(const stormm_topology_batch *batch, const stormm_launc
                           stormm_synthesis_info *info);
 /* Kernel status bits (1 = coordinates beyond the LDS coordinate range); clear != 0 resets. */
 int stormm_synthesis_status(const stormm_synthesis *ag, int32_t *status, int32_t clear);
-/* Autotuner: time candidate launch shapes on the device and return the fastest. */
+/* Autotuner: time force + energy evaluations of the batch under candidate launch shapes (work-unit
+ * capacity, units per CU, block size) on the device and return the fastest.  coordinates:
+ * [sum atoms][3] A to time on (null: a loose lattice per system).  Replaces the reference's
+ * launch-shape heuristics (src/Accelerator/core_kernel_manager.cpp:54-420). */
 int stormm_autotune_launch_shape(const stormm_topology_batch *batch, int32_t precision,
-                                 stormm_launch_shape *best, double *best_ms);
+                                 const double *coordinates, stormm_launch_shape *best,
+                                 double *best_ms);
 
 /* ---- PhaseSpaceSynthesis ------------------------------------------------------------------ */
 /* src/Synthesis/phasespace_synthesis.h:325-383 */
@@ -248,9 +252,11 @@ int stormm_launch_conjugate_gradient(int32_t precision, const stormm_synthesis *
 int stormm_launch_line_advance(int32_t precision, const stormm_synthesis *ag,
                                stormm_phasespace *ps, stormm_scorecard *sc, stormm_line_min *lm,
                                int32_t move_number, void *stream);
-/* Whole minimization.  persistent != 0 runs every cycle of every system inside one launch with
+/* Whole minimization.  persistent == 1 runs every cycle of every system inside one launch with
  * the systems' state held in LDS; persistent == 0 issues the reference's per-step launch
- * sequence.  Final energies land in the ScoreCard. */
+ * sequence; persistent == 2 captures that sequence for one cycle in a HIP graph and replays it
+ * for every cycle (returns when done).  Modes 1 and 2 stop a system once its RMS force reaches
+ * convergence_rms_force (if > 0).  Final energies land in the ScoreCard. */
 int stormm_launch_minimization(int32_t precision, const stormm_synthesis *ag,
                                stormm_phasespace *ps, const stormm_minimize_controls *mincon,
                                stormm_scorecard *sc, stormm_line_min *lm, int32_t persistent,

@@ -105,7 +105,7 @@ _SIGS = {
     "stormm_synthesis_atom_layout": (c_int32, [vp, vp, vp]),
     "stormm_synthesis_plan": (c_int32, [P(TopologyBatch), P(LaunchShape), P(SynthesisInfo)]),
     "stormm_synthesis_status": (c_int32, [vp, P(c_int32), c_int32]),
-    "stormm_autotune_launch_shape": (c_int32, [P(TopologyBatch), c_int32, P(LaunchShape),
+    "stormm_autotune_launch_shape": (c_int32, [P(TopologyBatch), c_int32, vp, P(LaunchShape),
                                                P(c_double)]),
     "stormm_phasespace_create": (c_int32, [vp, c_int32, c_int32, P(vp)]),
     "stormm_phasespace_destroy": (c_int32, [vp]),

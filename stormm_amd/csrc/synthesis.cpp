@@ -43,7 +43,7 @@ struct stormm_synthesis {
   SynthesisPlan plan;
   DevMem atom_starts, atom_counts, q, ljp1, ljp2, lj_type, ljab, ljab14, rho, scr,
       pbr, alpha, beta, gamma, wu, wu_sys, tasks, masks, vterms, bond_par, angl_par, dihe_par,
-      dihe14, status, cycles_run, rms_force;
+      dihe14, status, cycles_run, rms_force, stop_flags;
   MMKit kit;
 };
 
@@ -153,6 +153,7 @@ extern "C" int stormm_synthesis_create(const stormm_topology_batch *batch,
   std::vector<double> zd(P.nsys, 0.0);
   HIP_TRY(ag->cycles_run.upload(zi));
   HIP_TRY(ag->rms_force.upload(zd));
+  HIP_TRY(ag->stop_flags.upload(zi));
   MMKit &k = ag->kit;
   std::memset(&k, 0, sizeof(k));
   k.nsys = P.nsys;
@@ -779,6 +780,81 @@ static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag, stor
   return STORMM_OK;
 }
 
+// The per-cycle launch sequence captured once in a HIP graph and replayed for every cycle.  The
+// conjugate-gradient kernel reads each system's cycle count from device memory (so one captured
+// cycle serves them all) and applies the RMS-force stop of the persistent minimizer; evaluations
+// overwrite, so the graph holds kernels only.
+static int minimize_graph(int32_t precision, const stormm_synthesis *ag_c, stormm_phasespace *ps,
+                          const stormm_minimize_controls *mc, stormm_scorecard *sc,
+                          stormm_line_min *lm, hipStream_t stream) {
+  stormm_synthesis *ag = const_cast<stormm_synthesis *>(ag_c);
+  int rc = stormm_phasespace_prime_conjugate_gradient(ps, stream);
+  if (rc != STORMM_OK) {
+    return rc;
+  }
+  const int ns = ag->plan.nsys;
+  HIP_TRY(hipMemsetAsync(ag->cycles_run.ptr, 0, ns * sizeof(int), stream));
+  HIP_TRY(hipMemsetAsync(ag->stop_flags.ptr, 0, ns * sizeof(int), stream));
+  HIP_TRY(hipMemsetAsync(ag->rms_force.ptr, 0, ns * sizeof(double), stream));
+  const int prec = (precision == STORMM_PREC_DOUBLE) ? 0 : 1;
+  MMKit k = make_kit(ag, ps, sc);
+  k.store_overwrite = 1;
+  CgKit cg = make_cg_kit(ag, ps, precision);
+  cg.steps = ag->cycles_run.as<int>();
+  cg.done = ag->stop_flags.as<int>();
+  cg.rms = ag->rms_force.as<double>();
+  cg.conv_rms = mc->convergence_rms_force;
+  cg.max_steps = mc->total_cycles;
+  LineKit lk = make_line_kit(ag, ps, sc, lm, precision);
+  lk.done = ag->stop_flags.as<int>();
+  const int terms = kTermNonbonded | kTermValence;
+  const int sd = mc->steepest_descent_cycles;
+  hipStream_t cap = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipError_t err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (err == hipSuccess) {
+    hipError_t e2 = launch_mm_eval(prec, k, terms, 1, 1, cap);
+    if (e2 == hipSuccess) {
+      e2 = launch_conjugate_gradient(cg, 0, sd, cap);
+    }
+    for (int move = 0; move < 4 && e2 == hipSuccess; move++) {
+      if (move > 0) {
+        e2 = launch_mm_eval(prec, k, terms, 0, 1, cap);
+      }
+      if (e2 == hipSuccess) {
+        e2 = launch_line_advance(lk, move, cap);
+      }
+    }
+    err = hipStreamEndCapture(cap, &graph);
+    if (err == hipSuccess) {
+      err = e2;
+    }
+  }
+  if (err == hipSuccess) {
+    err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  }
+  for (int c = 0; c < mc->total_cycles && err == hipSuccess; c++) {
+    err = hipGraphLaunch(exec, stream);
+  }
+  if (err == hipSuccess) {
+    err = launch_mm_eval(prec, k, terms, 0, 1, stream);
+  }
+  if (err == hipSuccess) {
+    err = hipStreamSynchronize(stream);
+  }
+  if (exec != nullptr) {
+    (void)hipGraphExecDestroy(exec);
+  }
+  if (graph != nullptr) {
+    (void)hipGraphDestroy(graph);
+  }
+  (void)hipStreamDestroy(cap);
+  HIP_TRY(err);
+  return STORMM_OK;
+}
+
 extern "C" int stormm_launch_minimization(int32_t precision, const stormm_synthesis *ag,
                                           stormm_phasespace *ps,
                                           const stormm_minimize_controls *mincon,
@@ -796,8 +872,11 @@ extern "C" int stormm_launch_minimization(int32_t precision, const stormm_synthe
     return fail(STORMM_ERR_INVALID_ARGUMENT, "negative cycle count");
   }
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (!persistent) {
+  if (persistent == 0) {
     return minimize_stepwise(precision, ag, ps, mincon, sc, lm, st);
+  }
+  if (persistent == 2) {
+    return minimize_graph(precision, ag, ps, mincon, sc, lm, st);
   }
   MinKit k;
   k.mm = make_kit(ag, ps, sc);
@@ -881,8 +960,8 @@ extern "C" int stormm_time_mm_evaluation(int32_t precision, const stormm_synthes
 }
 
 extern "C" int stormm_autotune_launch_shape(const stormm_topology_batch *batch,
-                                            int32_t precision, stormm_launch_shape *best,
-                                            double *best_ms) {
+                                            int32_t precision, const double *coordinates,
+                                            stormm_launch_shape *best, double *best_ms) {
   if (batch == nullptr || best == nullptr) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
   }
@@ -895,14 +974,16 @@ extern "C" int stormm_autotune_launch_shape(const stormm_topology_batch *batch,
   const int blocks[3] = {256, 512, 1024};
   double best_t = 1.0e30;
   stormm_launch_shape best_shape = {512, 2, kEvalBlock};
-  // Coordinates for timing: a loose cubic lattice per system (1.6 A spacing) keeps every pair
-  // well separated; timing does not depend on the geometry beyond the far/near branch mix.
+  // Timing coordinates: the caller's (the near / far mix of the Born-radius pairs depends on
+  // the geometry), or else a loose cubic lattice per system (1.6 A spacing).
   std::vector<double> xyz;
-  for (int s = 0; s < batch->system_count; s++) {
-    for (int i = 0; i < batch->atom_counts[s]; i++) {
-      xyz.push_back(1.6 * (i % 8));
-      xyz.push_back(1.6 * ((i / 8) % 8));
-      xyz.push_back(1.6 * (i / 64));
+  if (coordinates == nullptr) {
+    for (int s = 0; s < batch->system_count; s++) {
+      for (int i = 0; i < batch->atom_counts[s]; i++) {
+        xyz.push_back(1.6 * (i % 8));
+        xyz.push_back(1.6 * ((i / 8) % 8));
+        xyz.push_back(1.6 * (i / 64));
+      }
     }
   }
   for (int cap : caps) {
@@ -924,7 +1005,7 @@ extern "C" int stormm_autotune_launch_shape(const stormm_topology_batch *batch,
         rc = stormm_scorecard_create(ag->plan.nsys, 26, &sc);
       }
       if (rc == STORMM_OK) {
-        rc = stormm_phasespace_upload_coordinates(ps, xyz.data());
+        rc = stormm_phasespace_upload_coordinates(ps, coordinates ? coordinates : xyz.data());
       }
       double ms = 0.0;
       if (rc == STORMM_OK) {

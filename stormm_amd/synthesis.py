@@ -153,13 +153,19 @@ def plan_synthesis(topos: Sequence[Topology], shape=None) -> SynthesisInfo:
     return info
 
 
-def autotune(topos: Sequence[Topology], precision="single"):
-    """Time candidate work-unit shapes on the device; returns (shape dict, mean ms)."""
+def autotune(topos: Sequence[Topology], precision="single", coords=None):
+    """Time candidate work-unit shapes on the device (on `coords` when given); returns
+    (shape dict, mean ms)."""
     L = _lib.load()
     arrs = _BatchArrays(topos)
     best = LaunchShape()
     ms = ctypes.c_double(0.0)
+    xyz = None
+    if coords is not None:
+        xyz = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.float64).reshape(-1, 3)
+                                                   for c in coords]))
     check(L.stormm_autotune_launch_shape(ctypes.byref(arrs.struct), precision_code(precision),
+                                         None if xyz is None else xyz.ctypes.data,
                                          ctypes.byref(best), ctypes.byref(ms)))
     return ({"wu_atom_capacity": best.wu_atom_capacity,
              "target_wu_per_cu": best.target_wu_per_cu,

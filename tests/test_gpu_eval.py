@@ -206,3 +206,15 @@ def test_mismatched_scorecard_is_rejected():
     sc = S.ScoreCard(3, 26)
     with pytest.raises(S._lib.StormmError, match="not prepared for the number of systems"):
         S.launch_mm_evaluation("single", ag, None, ps, sc, True, True)
+
+
+def test_autotuner_returns_a_valid_shape_that_gives_the_same_results():
+    topos, coords = synthetic.config1(nsys=64, seed=23)
+    shape, ms = S.autotune(topos, "single", coords)
+    assert ms > 0.0
+    assert shape["block_threads"] in (256, 512, 1024)
+    assert shape["wu_atom_capacity"] in (128, 256, 384, 512)
+    ref = evaluate(topos, coords, "single")
+    tuned = evaluate(topos, coords, "single", shape=shape)
+    assert np.array_equal(tuned["E"], ref["E"])
+    assert all(np.array_equal(x, y) for x, y in zip(tuned["Ffix"], ref["Ffix"]))

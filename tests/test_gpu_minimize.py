@@ -48,6 +48,22 @@ def test_persistent_minimizer_does_not_depend_on_block_size(prec):
     assert np.array_equal(a["E"], b["E"])
 
 
+@pytest.mark.parametrize("prec", ["double", "single"])
+def test_graph_replay_equals_per_step_and_persistent(prec):
+    topos, coords = synthetic.config2(nsys=10, seed=16)
+    a = minimize(topos, coords, prec, 30, persistent=0)
+    b = minimize(topos, coords, prec, 30, persistent=2)
+    assert all(np.array_equal(x, y) for x, y in zip(a["X"], b["X"]))
+    assert np.array_equal(a["E"], b["E"])
+    # With an RMS-force stop the graph replay follows the persistent minimizer's rule.
+    c = minimize(topos, coords, prec, 200, persistent=1, rms=2.0)
+    d = minimize(topos, coords, prec, 200, persistent=2, rms=2.0)
+    assert all(np.array_equal(x, y) for x, y in zip(c["X"], d["X"]))
+    assert np.array_equal(c["E"], d["E"])
+    assert np.array_equal(c["cycles"], d["cycles"])
+    assert (c["cycles"] < 200).any()
+
+
 def test_double_precision_minimizer_tracks_the_oracle():
     topos, coords = synthetic.config2(nsys=6, seed=13)
     gpos, frc, nrg = MODES["double"]

@@ -12,8 +12,11 @@ SOURCES = ["mm_kernels.hip", "mm_minimize.hip", "probes.hip", "synthesis.cpp", "
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("STORMM_AMD_ARCH", "gfx950")
 
-# Packed FP32 instructions are disabled for all device code (-packed-fp32-ops).
-DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# Packed FP32 instructions are disabled for all device code (-packed-fp32-ops).  Floating-point
+# contraction is limited to single source expressions (-ffp-contract=on), so a multiply and an
+# add in different statements are never fused: results then follow from the source alone and
+# are the same in every kernel that inlines a stage.
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=on"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
           "-Wno-unused-result"]
 

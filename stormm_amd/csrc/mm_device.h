@@ -20,6 +20,20 @@ constexpr double kGbOffset = 0.09;      // src/Constants/generalized_born.h:112
 // Precision traits.  Single precision uses the hardware transcendental units (v_rsq, v_rcp,
 // v_log, v_exp); double precision uses the correctly rounded library routines.
 // ------------------------------------------------------------------------------------------
+// Nearest integer (ties to even, as rint) of a double as int64.  For |x| < 2^51, adding
+// 1.5 * 2^52 rounds x to an integer held in the low bits of the sum's bit pattern; subtracting
+// the constant's pattern recovers it in two instructions.  Larger or non-finite values take the
+// library conversion.
+__device__ __forceinline__ long long round_to_ll(double x) {
+  // x must be rounded to double before the addition: never fuse it with the product that made x.
+#pragma clang fp contract(off)
+  constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
+  if (__builtin_expect(fabs(x) < 2251799813685248.0, 1)) {
+    return __double_as_longlong(x + kMagic) - __double_as_longlong(kMagic);
+  }
+  return static_cast<long long>(__builtin_rint(x));
+}
+
 template <typename T> struct Prec;
 
 template <> struct Prec<float> {
@@ -34,7 +48,7 @@ template <> struct Prec<float> {
     return __builtin_amdgcn_exp2f(x * 1.44269504088896340736f);
   }
   static __device__ __forceinline__ long long fixed(float x) {
-    return static_cast<long long>(__builtin_rintf(x));
+    return round_to_ll(static_cast<double>(x));
   }
 };
 
@@ -45,9 +59,7 @@ template <> struct Prec<double> {
   static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
   static __device__ __forceinline__ double log(double x) { return ::log(x); }
   static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
-  static __device__ __forceinline__ long long fixed(double x) {
-    return static_cast<long long>(__builtin_rint(x));
-  }
+  static __device__ __forceinline__ long long fixed(double x) { return round_to_ll(x); }
 };
 
 // ------------------------------------------------------------------------------------------
@@ -100,16 +112,16 @@ __device__ __forceinline__ void lds_add(long long *addr, long long v) {
 // since |r - sj| >= ri keeps that product away from zero in those branches.
 // ------------------------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ T psi_far(T sj, T invr2) {
-  const T tmpsd = sj * sj * invr2;
+__device__ __forceinline__ T psi_far(T sj, T sj2, T invr2) {
+  const T tmpsd = sj2 * invr2;
   const T dumbo = T(kGTA) + tmpsd * (T(kGTB) + tmpsd * (T(kGTC) + tmpsd * (T(kGTD) +
                                                                           tmpsd * T(kGTDD))));
   return -(sj * tmpsd * invr2 * dumbo);
 }
 
 template <typename T>
-__device__ __forceinline__ T dpsi_far(T sj, T invr2) {
-  const T tmpsd = sj * sj * invr2;
+__device__ __forceinline__ T dpsi_far(T sj, T sj2, T invr2) {
+  const T tmpsd = sj2 * invr2;
   const T dumbo = T(kGTE) + tmpsd * (T(kGTF) + tmpsd * (T(kGTG) + tmpsd * (T(kGTH) +
                                                                           tmpsd * T(kGTHH))));
   return tmpsd * sj * invr2 * invr2 * dumbo;
@@ -165,10 +177,10 @@ __device__ __forceinline__ long long fixed_contrib(float x, int bits) {
   if (fabsf(v) < 2147483520.0f) {
     return static_cast<long long>(__float2int_rn(v)) << (bits - lo);
   }
-  return static_cast<long long>(__builtin_rint(ldexp(static_cast<double>(x), bits)));
+  return round_to_ll(ldexp(static_cast<double>(x), bits));
 }
 __device__ __forceinline__ long long fixed_contrib(double x, int bits) {
-  return static_cast<long long>(__builtin_rint(ldexp(x, bits)));
+  return round_to_ll(ldexp(x, bits));
 }
 
 // Largest block a kernel of each precision is compiled for.
@@ -245,10 +257,26 @@ __device__ __forceinline__ float crd_delta(int a, int b, float scale) {
 }
 __device__ __forceinline__ double crd_delta(double a, double b, double) { return a - b; }
 
-// Displacement in LDS coordinate units (the exact integer difference, rounded once, in single
-// precision; Angstroms in double precision, whose coordinate unit is 1).
-__device__ __forceinline__ float crd_raw(int a, int b) { return static_cast<float>(a - b); }
-__device__ __forceinline__ double crd_raw(double a, double b) { return a - b; }
+// exp(-r^2 / (4 Ri Rj)) of the GB pair function, given r^2 and Ri Rj.
+__device__ __forceinline__ float gb_efac(float r2, float rr) {
+  return __builtin_amdgcn_exp2f(r2 * __builtin_amdgcn_rcpf(rr) *
+                                (-0.25f * 1.44269504088896340736f));
+}
+__device__ __forceinline__ double gb_efac(double r2, double rr) {
+  return ::exp(-0.25 * r2 * (1.0 / rr));
+}
+
+// Coordinates of a pair tile's atoms in Angstroms in the working precision.  Single precision
+// converts the LDS fixed-precision coordinates relative to a point between the tile's two atom
+// blocks once per task, as the reference imports tile coordinates relative to the tile's centre
+// (src/Potential/nonbonded_potential_tilegroups.cui:395-426), so the rotation steps take plain
+// differences.  Double precision holds Angstroms already.
+__device__ __forceinline__ int tile_centre(int a, int b) { return (a >> 1) + (b >> 1); }
+__device__ __forceinline__ double tile_centre(double, double) { return 0.0; }
+__device__ __forceinline__ float tile_crd(int v, int centre, float scale) {
+  return static_cast<float>(v - centre) * scale;
+}
+__device__ __forceinline__ double tile_crd(double v, double, double) { return v; }
 
 // Conversion of a fixed-precision global coordinate to the LDS representation.
 template <typename T> struct CrdConv;

@@ -67,7 +67,7 @@ __device__ void stage_load_params(StageCtx<T> &c, const MMKit &k) {
         L.p2[a] = T(0);
       }
       else {
-        L.p1[a] = static_cast<T>(k.ljp1[g]);
+        L.p1[a] = static_cast<T>(0.5 * k.ljp1[g]);  // half sigma: sigma_ij = p1_i + p1_j
         L.p2[a] = static_cast<T>(k.ljp2[g]);
       }
       const double rho = k.rho[g];
@@ -192,6 +192,22 @@ struct TileTask {
     nsteps = wave_steps(diag);
   }
 };
+
+// Coordinates of lane il's i atom and first j atom of a tile, in Angstroms.
+template <typename T>
+__device__ __forceinline__ void tile_coords(const StageCtx<T> &c, const TileTask &tt, T &xi,
+                                            T &yi, T &zi, T &xj, T &yj, T &zj) {
+  const WuLds<T> &L = c.L;
+  const auto ox = tile_centre(L.cx[tt.ib], L.cx[tt.jb]);
+  const auto oy = tile_centre(L.cy[tt.ib], L.cy[tt.jb]);
+  const auto oz = tile_centre(L.cz[tt.ib], L.cz[tt.jb]);
+  xi = tile_crd(L.cx[tt.i], ox, c.crd_scale);
+  yi = tile_crd(L.cy[tt.i], oy, c.crd_scale);
+  zi = tile_crd(L.cz[tt.i], oz, c.crd_scale);
+  xj = tile_crd(L.cx[tt.j0], ox, c.crd_scale);
+  yj = tile_crd(L.cy[tt.j0], oy, c.crd_scale);
+  zj = tile_crd(L.cz[tt.j0], oz, c.crd_scale);
+}
 
 __device__ __forceinline__ bool step_bit(unsigned mask, int ks) {
   return ((mask >> ks) & 1u) != 0u;
@@ -335,14 +351,13 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k) {
   const int nwaves = c.nthr >> 6;
   unsigned *q = L.nearq + (wave * kNearQueue);
   int qn = 0;
-  const T cs2 = c.crd_scale * c.crd_scale;
   const double psi_scale = ldexp(1.0, kPsiBits);
   for (int tb = 4 * wave; tb < c.wu.task_count; tb += 4 * nwaves) {
     const TileTask tt(wave_task(c, k, tb + ((c.tid >> 4) & 3)), il);
-    const auto xi = L.cx[tt.i], yi = L.cy[tt.i], zi = L.cz[tt.i];
+    T xi, yi, zi, xj, yj, zj;
+    tile_coords(c, tt, xi, yi, zi, xj, yj, zj);
     const T si = L.scr[tt.i];
-    const T si4 = T(4) * si;
-    auto xj = L.cx[tt.j0], yj = L.cy[tt.j0], zj = L.cz[tt.j0];
+    const T si2 = si * si;
     T sj = L.scr[tt.j0];
     // Queue entry bits of the pair: i in bits 0..8, the rotating j atom in bits 9..17.
     const unsigned ei = static_cast<unsigned>(tt.i);
@@ -350,17 +365,18 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k) {
     T acci = T(0), accj = T(0);
     for (int ks = 0; ks < tt.nsteps; ks++) {
       const bool valid = step_bit(tt.valid, ks);
-      const T dx = crd_raw(xj, xi);
-      const T dy = crd_raw(yj, yi);
-      const T dz = crd_raw(zj, zi);
-      const T r2 = valid ? ((dx * dx) + (dy * dy) + (dz * dz)) * cs2 : T(1);
-      const T invr = P::rsqrt(r2);
-      const T invr2 = invr * invr;
-      const T r = r2 * invr;
-      const bool far_i = r > T(4) * sj;
-      const bool far_j = r > si4;
-      const T ti = psi_far(sj, invr2);
-      const T tj = psi_far(si, invr2);
+      const T dx = xj - xi;
+      const T dy = yj - yi;
+      const T dz = zj - zi;
+      const T r2 = valid ? ((dx * dx) + (dy * dy) + (dz * dz)) : T(1);
+      const T invr2 = P::rcp(r2);
+      // r > 4 s  <=>  r^2 / 16 > s^2
+      const T r2q = T(0.0625) * r2;
+      const T sj2 = sj * sj;
+      const bool far_i = r2q > sj2;
+      const bool far_j = r2q > si2;
+      const T ti = psi_far(sj, sj2, invr2);
+      const T tj = psi_far(si, si2, invr2);
       acci += (valid && far_i) ? ti : T(0);
       accj += (valid && far_j) ? tj : T(0);
       const bool near_i = valid && !far_i;
@@ -441,7 +457,8 @@ __device__ void stage_radii(StageCtx<T> &c, const MMKit &k, double coulomb_inv_s
 // pair energy, force and radius derivative (:811-972).  Every lane evaluates its pair at every
 // step; absent and excluded pairs are removed by zeroing the charge product and the LJ
 // coefficients, and the squared distance of an absent pair is replaced by 1 to keep the
-// arithmetic finite.  Forces are accumulated in LDS coordinate units and scaled once per task.
+// arithmetic finite.  Forces are summed per lane in the working precision and converted to fixed
+// precision once per task.
 template <typename T, bool FORCES, bool LJTABLE, bool GB>
 __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
   using P = Prec<T>;
@@ -449,26 +466,28 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
   const int row = c.tid >> 4;
   const int nrows = c.nthr >> 4;
   const int il = c.tid & 15;
-  const T cs2 = c.crd_scale * c.crd_scale;
-  const T frc_scale = T(ldexp(1.0, k.frc_bits)) * c.crd_scale;
+  const T frc_scale = T(ldexp(1.0, k.frc_bits));
   const T dedr_scale = T(ldexp(1.0, kDedrBits));
-  const double nrg_scale = ldexp(1.0, k.nrg_bits);
+  const T nrg_scale = T(ldexp(1.0, k.nrg_bits));
   const T inv_diel = T(1.0 / k.dielectric);
   const T kap = T(k.kappa * kGbKscale);
   const bool use_kappa = (k.kappa != 0.0);
+  const T dielfac0 = T(1) - inv_diel;
   for (int t = row; t < c.wu.task_count; t += nrows) {
     const Task tk = fetch_task(c, k, t);
     const TileTask tt(tk, il);
     const unsigned emask = (tk.z >= 0 && tt.ivalid) ? k.masks[tk.z + il] : 0u;
     const unsigned nbmask = tt.valid & ~step_mask(emask, il);
-    const auto xi = L.cx[tt.i], yi = L.cy[tt.i], zi = L.cz[tt.i];
+    T xi, yi, zi, xj, yj, zj;
+    tile_coords(c, tt, xi, yi, zi, xj, yj, zj);
     const T qi = L.q[tt.i], p1i = L.p1[tt.i], p2i = L.p2[tt.i], Ri = L.rad[tt.i];
-    auto xj = L.cx[tt.j0], yj = L.cy[tt.j0], zj = L.cz[tt.j0];
     T qj = L.q[tt.j0], p1j = L.p1[tt.j0], p2j = L.p2[tt.j0], Rj = L.rad[tt.j0];
     T fxi = T(0), fyi = T(0), fzi = T(0), dri = T(0);
     T fxj = T(0), fyj = T(0), fzj = T(0), drj = T(0);
-    // Energies are summed in double precision (one conversion and add per term and pair).
-    double e_qq = 0.0, e_lj = 0.0, e_gb = 0.0;
+    // Each lane sums its energies over the task in the calculation precision and converts the
+    // sums to fixed precision once, as the reference does per tile
+    // (src/Potential/nonbonded_potential_tilegroups.cui:461-464,640-645).
+    T e_qq = T(0), e_lj = T(0), e_gb = T(0);
     int lj_off = 0, lj_nt = 0;
     if (LJTABLE) {
       lj_off = L.sys[tt.slot].lj_off;
@@ -477,30 +496,33 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
     for (int ks = 0; ks < tt.nsteps; ks++) {
       const bool valid = step_bit(tt.valid, ks);
       const bool nbok = step_bit(nbmask, ks);
-      const T dx = crd_raw(xj, xi);
-      const T dy = crd_raw(yj, yi);
-      const T dz = crd_raw(zj, zi);
-      const T r2 = valid ? ((dx * dx) + (dy * dy) + (dz * dz)) * cs2 : T(1);
+      const T dx = xj - xi;
+      const T dy = yj - yi;
+      const T dz = zj - zi;
+      const T r2 = valid ? ((dx * dx) + (dy * dy) + (dz * dz)) : T(1);
       const T qiqj = qi * qj;
       T fmag = T(0);
       if (GB) {
         const T qgb = valid ? qiqj : T(0);
         const T rr = Ri * Rj;
-        const T efac = P::exp(-T(0.25) * r2 * P::rcp(rr));
+        const T efac = gb_efac(r2, rr);
         const T fgb2 = r2 + (rr * efac);
         const T fgbi = P::rsqrt(fgb2);
-        T dielfac, temp6;
+        // egb = qq (1 - exp(-kappa fgb) / diel) / fgb, temp6 = -(dE/dfgb) / fgb.
+        T egb, temp6;
         if (use_kappa) {
           const T fgbk = -kap * fgb2 * fgbi;
           const T expmkf = P::exp(fgbk) * inv_diel;
-          dielfac = T(1) - expmkf;
-          temp6 = qgb * fgbi * fgbi * fgbi * (dielfac + (fgbk * expmkf));
+          const T dielfac = T(1) - expmkf;
+          const T qf = qgb * fgbi;
+          egb = qf * dielfac;
+          temp6 = qf * fgbi * fgbi * (dielfac + (fgbk * expmkf));
         }
         else {
-          dielfac = T(1) - inv_diel;
-          temp6 = qgb * fgbi * fgbi * fgbi * dielfac;
+          egb = qgb * fgbi * dielfac0;
+          temp6 = egb * fgbi * fgbi;
         }
-        e_gb -= static_cast<double>(qgb * dielfac * fgbi);
+        e_gb -= egb;
         if (FORCES) {
           fmag += temp6 * (T(1) - (T(0.25) * efac));
           const T temp5 = T(0.5) * efac * temp6 * (rr + (T(0.25) * r2));
@@ -512,26 +534,27 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
       const T invr = P::rsqrt(r2);
       const T invr2 = invr * invr;
       const T eqq = qnb * invr;
-      e_qq += static_cast<double>(eqq);
+      e_qq += eqq;
       if (LJTABLE) {
         const double2 ab = k.ljab[lj_off + (static_cast<int>(p1i) * lj_nt) +
                                   static_cast<int>(p1j)];
         const T lja = nbok ? static_cast<T>(ab.x) : T(0);
         const T ljb = nbok ? static_cast<T>(ab.y) : T(0);
         const T invr4 = invr2 * invr2;
-        e_lj += static_cast<double>(((lja * invr4 * invr4) - (ljb * invr2)) * invr4);
+        e_lj += ((lja * invr4 * invr4) - (ljb * invr2)) * invr4;
         if (FORCES) {
           fmag += ((T(6) * ljb) - (T(12) * lja * invr4 * invr2)) * invr4 * invr4;
         }
       }
       else {
-        const T sig = T(0.5) * (p1i + p1j);
+        const T sig = p1i + p1j;
         const T s2 = sig * sig * invr2;
         const T s6 = s2 * s2 * s2;
         const T e4 = nbok ? p2i * p2j : T(0);
-        e_lj += static_cast<double>(e4 * ((s6 * s6) - s6));
+        const T e4s6 = e4 * s6;
+        e_lj += e4s6 * (s6 - T(1));
         if (FORCES) {
-          fmag += e4 * ((T(6) * s6) - (T(12) * s6 * s6)) * invr2;
+          fmag += e4s6 * (T(6) - (T(12) * s6)) * invr2;
         }
       }
       if (FORCES) {
@@ -581,9 +604,9 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
         }
       }
     }
-    const long long vqq = row_sum(Prec<double>::fixed(e_qq * nrg_scale));
-    const long long vlj = row_sum(Prec<double>::fixed(e_lj * nrg_scale));
-    const long long vgb = GB ? row_sum(Prec<double>::fixed(e_gb * nrg_scale)) : 0LL;
+    const long long vqq = row_sum(P::fixed(e_qq * nrg_scale));
+    const long long vlj = row_sum(P::fixed(e_lj * nrg_scale));
+    const long long vgb = GB ? row_sum(P::fixed(e_gb * nrg_scale)) : 0LL;
     if (il == 0) {
       lds_add(&L.nrg[tt.slot * 16 + kElec], vqq);
       lds_add(&L.nrg[tt.slot * 16 + kVdw], vlj);
@@ -653,8 +676,8 @@ __device__ __forceinline__ void deriv_drain(StageCtx<T> &c, const unsigned *q, i
 }
 
 // Forces from the Born-radius derivatives, src/Potential/nonbonded_potential.tpp:1018-1292,
-// with the same far-field / queued near-field split as stage_psi.  Far-field forces are
-// accumulated in LDS coordinate units and scaled once per task.
+// with the same far-field / queued near-field split as stage_psi.  Far-field forces are summed
+// per lane in the working precision and converted to fixed precision once per task.
 template <typename T>
 __device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
   using P = Prec<T>;
@@ -665,14 +688,13 @@ __device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
   const int nwaves = c.nthr >> 6;
   unsigned *q = L.nearq + (wave * kNearQueue);
   int qn = 0;
-  const T cs2 = c.crd_scale * c.crd_scale;
-  const T frc_scale = T(ldexp(1.0, k.frc_bits)) * c.crd_scale;
+  const T frc_scale = T(ldexp(1.0, k.frc_bits));
   for (int tb = 4 * wave; tb < c.wu.task_count; tb += 4 * nwaves) {
     const TileTask tt(wave_task(c, k, tb + ((c.tid >> 4) & 3)), il);
-    const auto xi = L.cx[tt.i], yi = L.cy[tt.i], zi = L.cz[tt.i];
+    T xi, yi, zi, xj, yj, zj;
+    tile_coords(c, tt, xi, yi, zi, xj, yj, zj);
     const T si = L.scr[tt.i], sdi = L.sd[tt.i];
-    const T si4 = T(4) * si;
-    auto xj = L.cx[tt.j0], yj = L.cy[tt.j0], zj = L.cz[tt.j0];
+    const T si2 = si * si;
     T sj = L.scr[tt.j0], sdj = L.sd[tt.j0];
     const unsigned ei = static_cast<unsigned>(tt.i);
     unsigned ej = static_cast<unsigned>(tt.j0) << 9;
@@ -680,18 +702,20 @@ __device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
     T fxj = T(0), fyj = T(0), fzj = T(0);
     for (int ks = 0; ks < tt.nsteps; ks++) {
       const bool valid = step_bit(tt.valid, ks);
-      const T dx = crd_raw(xj, xi);
-      const T dy = crd_raw(yj, yi);
-      const T dz = crd_raw(zj, zi);
-      const T r2 = valid ? ((dx * dx) + (dy * dy) + (dz * dz)) * cs2 : T(1);
-      const T invr = P::rsqrt(r2);
-      const T invr2 = invr * invr;
-      const T r = r2 * invr;
-      const bool far_i = r > T(4) * sj;
-      const bool far_j = r > si4;
-      const T ti = (valid && far_i) ? dpsi_far(sj, invr2) * sdi : T(0);
-      const T tj = (valid && far_j) ? dpsi_far(si, invr2) * sdj : T(0);
-      const T fmag = ti + tj;
+      const T dx = xj - xi;
+      const T dy = yj - yi;
+      const T dz = zj - zi;
+      const T r2 = valid ? ((dx * dx) + (dy * dy) + (dz * dz)) : T(1);
+      const T invr2 = P::rcp(r2);
+      const T r2q = T(0.0625) * r2;
+      const T sj2 = sj * sj;
+      const bool far_i = r2q > sj2;
+      const bool far_j = r2q > si2;
+      // Both series are evaluated on every lane and selected, which keeps the step free of
+      // branches.
+      const T ti = dpsi_far(sj, sj2, invr2) * sdi;
+      const T tj = dpsi_far(si, si2, invr2) * sdj;
+      const T fmag = ((valid && far_i) ? ti : T(0)) + ((valid && far_j) ? tj : T(0));
       fxi -= fmag * dx;
       fyi -= fmag * dy;
       fzi -= fmag * dz;
@@ -735,8 +759,10 @@ __device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
 
 // ------------------------------------------------------------------------------------------
 // Valence terms.  They are O(N) work, so both precision models evaluate them in double
-// precision from the full fixed-precision global coordinates: a single-precision r - r0 or
-// theta - theta0 near equilibrium loses most of its significant bits.
+// precision: a single-precision r - r0 or theta - theta0 near equilibrium loses most of its
+// significant bits.  Displacements come from the LDS coordinates (exact in the double-precision
+// model; 2^-24 Angstrom steps in the single-precision model, far finer than the bits such a
+// difference keeps in single precision).
 // ------------------------------------------------------------------------------------------
 struct Vec3 {
   double x, y, z;
@@ -748,16 +774,15 @@ __device__ __forceinline__ double dot(const Vec3 &a, const Vec3 &b) {
   return (a.x * b.x) + (a.y * b.y) + (a.z * b.z);
 }
 
-struct ValenceCtx {
-  const long long *xc, *yc, *zc;
+template <typename C> struct ValenceCtxT {
+  const C *cx, *cy, *cz;
   const int *gidx;
-  double inv_gpos;
+  double crd_scale;
   long long *fx, *fy, *fz;
   __device__ __forceinline__ Vec3 disp(int from, int to) const {
-    const int gf = gidx[from], gt = gidx[to];
-    return {static_cast<double>(xc[gt] - xc[gf]) * inv_gpos,
-            static_cast<double>(yc[gt] - yc[gf]) * inv_gpos,
-            static_cast<double>(zc[gt] - zc[gf]) * inv_gpos};
+    return {(static_cast<double>(cx[to]) - static_cast<double>(cx[from])) * crd_scale,
+            (static_cast<double>(cy[to]) - static_cast<double>(cy[from])) * crd_scale,
+            (static_cast<double>(cz[to]) - static_cast<double>(cz[from])) * crd_scale};
   }
   __device__ __forceinline__ void add(int a, long long vx, long long vy, long long vz) const {
     lds_add(&fx[a], vx);
@@ -766,13 +791,11 @@ struct ValenceCtx {
   }
 };
 
-__device__ __forceinline__ long long fix64(double v) {
-  return static_cast<long long>(__builtin_rint(v));
-}
+__device__ __forceinline__ long long fix64(double v) { return round_to_ll(v); }
 
 // Harmonic bond, src/Potential/valence_potential.tpp:9-96.
-template <bool FORCES>
-__device__ __forceinline__ double eval_bond(const ValenceCtx &v, int i, int j, double kb,
+template <bool FORCES, typename V>
+__device__ __forceinline__ double eval_bond(const V &v, int i, int j, double kb,
                                             double r0, double fscale) {
   const Vec3 d = v.disp(i, j);
   const double dr = sqrt(dot(d, d));
@@ -789,8 +812,8 @@ __device__ __forceinline__ double eval_bond(const ValenceCtx &v, int i, int j, d
 }
 
 // Harmonic angle, src/Potential/valence_potential.tpp:170-287.
-template <bool FORCES>
-__device__ __forceinline__ double eval_angle(const ValenceCtx &v, int i, int j, int kk,
+template <bool FORCES, typename V>
+__device__ __forceinline__ double eval_angle(const V &v, int i, int j, int kk,
                                              double ka, double th0, double fscale) {
   const Vec3 ba = v.disp(j, i);
   const Vec3 bc = v.disp(j, kk);
@@ -828,8 +851,8 @@ __device__ __forceinline__ double eval_angle(const ValenceCtx &v, int i, int j, 
 // product of the two plane normals and the signed sin(theta) their cross product projected on
 // the central bond (the reference's sign convention for theta), so no inverse or forward
 // trigonometric function is evaluated.
-template <bool FORCES>
-__device__ __forceinline__ double eval_dihedral(const ValenceCtx &v, int i, int j, int kk, int l,
+template <bool FORCES, typename V>
+__device__ __forceinline__ double eval_dihedral(const V &v, int i, int j, int kk, int l,
                                                 const DiheParam &dp, double fscale) {
   const Vec3 ab = v.disp(i, j);
   const Vec3 bc = v.disp(j, kk);
@@ -889,8 +912,8 @@ __device__ __forceinline__ double eval_dihedral(const ValenceCtx &v, int i, int 
 }
 
 // Attenuated 1:4 pair, src/Potential/valence_potential.tpp:1288-1419.
-template <bool FORCES, bool LJTABLE>
-__device__ __forceinline__ void eval_14(const ValenceCtx &v, const MMKit &k, int2 lj, int i,
+template <bool FORCES, bool LJTABLE, typename V>
+__device__ __forceinline__ void eval_14(const V &v, const MMKit &k, int2 lj, int i,
                                         int l, double inv_scee, double inv_scnb, double fscale,
                                         double &ele, double &vdw) {
   const int gi = v.gidx[i], gl = v.gidx[l];
@@ -935,12 +958,12 @@ template <typename T, bool FORCES, bool LJTABLE>
 __device__ void stage_valence(StageCtx<T> &c, const MMKit &k) {
   const double fscale = ldexp(1.0, k.frc_bits);
   const double nrg_scale = ldexp(1.0, k.nrg_bits);
-  ValenceCtx v;
-  v.xc = k.xcrd;
-  v.yc = k.ycrd;
-  v.zc = k.zcrd;
+  ValenceCtxT<typename Prec<T>::crd_t> v;
+  v.cx = c.L.cx;
+  v.cy = c.L.cy;
+  v.cz = c.L.cz;
   v.gidx = c.L.gidx;
-  v.inv_gpos = ldexp(1.0, -k.gpos_bits);
+  v.crd_scale = static_cast<double>(c.crd_scale);
   v.fx = c.L.fx;
   v.fy = c.L.fy;
   v.fz = c.L.fz;

@@ -38,6 +38,7 @@ constexpr int kMaxMinimizeBlock = 512;
 // Entries of each wavefront's LDS queue of near-field Born-integral pairs.  A wave drains 64
 // entries whenever the queue holds at least 64, and one rotation step adds at most 64.
 constexpr int kNearQueue = 128;
+constexpr int kStageCounters = 3;  // work-item counters of the psi, derivative and valence stages
 
 // StateVariable slots, src/Potential/energy_enumerators.h:36-75.
 enum NrgSlot : int {

@@ -203,6 +203,7 @@ template <typename T> struct WuLds {
   int *gidx, *slot;
   WuSys *sys;
   int *opre, *dpre, *vpre;  // per slot: first off-diagonal task, diagonal task, valence term
+  int *work;                // per-stage work-item counters (kStageCounters)
   unsigned *nearq;          // kNearQueue packed pair entries per wavefront
 
   static __host__ __device__ size_t bytes(int maxa, int waves) {
@@ -210,8 +211,8 @@ template <typename T> struct WuLds {
     return (5 * m * sizeof(long long)) + (kMaxWuSystems * 16 * sizeof(long long)) +
            (3 * kMaxWuSystems * sizeof(long long)) + (8 * m * sizeof(T)) +
            (3 * m * sizeof(crd_t)) + (2 * m * sizeof(int)) + (kMaxWuSystems * sizeof(WuSys)) +
-           (3 * (kMaxWuSystems + 1) * sizeof(int)) + (static_cast<size_t>(waves) * kNearQueue *
-                                                     sizeof(unsigned));
+           (3 * (kMaxWuSystems + 1) * sizeof(int)) + (kStageCounters * sizeof(int)) +
+           (static_cast<size_t>(waves) * kNearQueue * sizeof(unsigned));
   }
 
   __device__ WuLds(unsigned char *base, int maxa, int waves) {
@@ -246,6 +247,7 @@ template <typename T> struct WuLds {
     opre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
     dpre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
     vpre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
+    work = reinterpret_cast<int *>(take(kStageCounters * sizeof(int)));
     nearq = reinterpret_cast<unsigned *>(take(static_cast<size_t>(waves) * kNearQueue *
                                               sizeof(unsigned)));
   }

@@ -102,6 +102,9 @@ __device__ void stage_load_coords(StageCtx<T> &c, const MMKit &k) {
   for (int e = c.tid; e < wu.sys_count * 16; e += c.nthr) {
     L.nrg[e] = 0;
   }
+  if (c.tid < kStageCounters) {
+    L.work[c.tid] = 0;
+  }
   __syncthreads();
   for (int a = c.tid; a < wu.lds_atoms; a += c.nthr) {
     const int g = L.gidx[a];
@@ -263,6 +266,34 @@ __device__ __forceinline__ Task wave_task(const StageCtx<T> &c, const MMKit &k, 
   return make_int4(0, 1 << 8, -1, 0);
 }
 
+// Work items of a stage are taken by the wavefronts of the block one at a time from a counter in
+// LDS, so a wavefront that finishes early takes more items (the result does not depend on which
+// wavefront took which item: every contribution goes to integer accumulators).
+__device__ __forceinline__ int next_item(int *counter, int lane) {
+  int item = 0;
+  if (lane == 0) {
+    item = atomicAdd(counter, 1);
+  }
+  return __builtin_amdgcn_readfirstlane(item);
+}
+
+// Item i of a stage holding `quads` tile-task quads and `chunks` chunks of 64 valence terms, the
+// chunks spread evenly among the quads so that latency-bound valence work overlaps the
+// arithmetic of the tiles on the other wavefronts.  Returns the chunk index for a valence item
+// and -1 - (quad index) for a tile item.
+__device__ __forceinline__ int stage_item(int i, int quads, int chunks) {
+  if (chunks == 0) {
+    return -1 - i;
+  }
+  const unsigned total = static_cast<unsigned>(quads + chunks);
+  const int c0 = static_cast<int>((static_cast<unsigned>(i) * chunks) / total);
+  const int c1 = static_cast<int>((static_cast<unsigned>(i + 1) * chunks) / total);
+  return (c1 > c0) ? c0 : (-1 - (i - c0));
+}
+
+template <typename T, bool FORCES>
+__device__ void valence_term(StageCtx<T> &c, const MMKit &k, int t);
+
 // Near-field pair queue of a wavefront.  Entries pack the LDS indices of the pair (9 bits each)
 // and which of the two directions (onto i, onto j) lies within 4 screened radii.
 constexpr unsigned kNearI = 1u << 18;
@@ -341,19 +372,26 @@ __device__ __forceinline__ void psi_drain(StageCtx<T> &c, const unsigned *q, int
 // direction is within 4 screened radii; a full wavefront of queued pairs is evaluated at once.
 // The series terms of one task are summed in the working precision (at most 16 small terms per
 // lane) and added to the fixed-precision accumulators once per task.
-template <typename T>
-__device__ void stage_psi(StageCtx<T> &c, const MMKit &k) {
+// Chunks of valence terms (`val_chunks`, possibly 0) are taken as work items alongside the tiles.
+template <typename T, bool FORCES>
+__device__ void stage_psi(StageCtx<T> &c, const MMKit &k, int val_chunks) {
   using P = Prec<T>;
   WuLds<T> &L = c.L;
   const int il = c.tid & 15;
   const int lane = c.tid & 63;
   const int wave = c.tid >> 6;
-  const int nwaves = c.nthr >> 6;
   unsigned *q = L.nearq + (wave * kNearQueue);
   int qn = 0;
   const double psi_scale = ldexp(1.0, kPsiBits);
-  for (int tb = 4 * wave; tb < c.wu.task_count; tb += 4 * nwaves) {
-    const TileTask tt(wave_task(c, k, tb + ((c.tid >> 4) & 3)), il);
+  const int quads = (c.wu.task_count + 3) >> 2;
+  const int items = quads + val_chunks;
+  for (int item = next_item(&L.work[0], lane); item < items; item = next_item(&L.work[0], lane)) {
+    const int it = stage_item(item, quads, val_chunks);
+    if (it >= 0) {
+      valence_term<T, FORCES>(c, k, (it << 6) + lane);
+      continue;
+    }
+    const TileTask tt(wave_task(c, k, ((-1 - it) << 2) + ((c.tid >> 4) & 3)), il);
     T xi, yi, zi, xj, yj, zj;
     tile_coords(c, tt, xi, yi, zi, xj, yj, zj);
     const T si = L.scr[tt.i];
@@ -685,12 +723,12 @@ __device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
   const int il = c.tid & 15;
   const int lane = c.tid & 63;
   const int wave = c.tid >> 6;
-  const int nwaves = c.nthr >> 6;
   unsigned *q = L.nearq + (wave * kNearQueue);
   int qn = 0;
   const T frc_scale = T(ldexp(1.0, k.frc_bits));
-  for (int tb = 4 * wave; tb < c.wu.task_count; tb += 4 * nwaves) {
-    const TileTask tt(wave_task(c, k, tb + ((c.tid >> 4) & 3)), il);
+  const int quads = (c.wu.task_count + 3) >> 2;
+  for (int item = next_item(&L.work[1], lane); item < quads; item = next_item(&L.work[1], lane)) {
+    const TileTask tt(wave_task(c, k, (item << 2) + ((c.tid >> 4) & 3)), il);
     T xi, yi, zi, xj, yj, zj;
     tile_coords(c, tt, xi, yi, zi, xj, yj, zj);
     const T si = L.scr[tt.i], sdi = L.sd[tt.i];
@@ -912,10 +950,10 @@ __device__ __forceinline__ double eval_dihedral(const V &v, int i, int j, int kk
 }
 
 // Attenuated 1:4 pair, src/Potential/valence_potential.tpp:1288-1419.
-template <bool FORCES, bool LJTABLE, typename V>
-__device__ __forceinline__ void eval_14(const V &v, const MMKit &k, int2 lj, int i,
-                                        int l, double inv_scee, double inv_scnb, double fscale,
-                                        double &ele, double &vdw) {
+template <bool FORCES, typename V>
+__device__ __forceinline__ void eval_14(const V &v, const MMKit &k, bool lj_table, int2 lj,
+                                        int i, int l, double inv_scee, double inv_scnb,
+                                        double fscale, double &ele, double &vdw) {
   const int gi = v.gidx[i], gl = v.gidx[l];
   const Vec3 d = v.disp(i, l);
   const double r2 = dot(d, d);
@@ -924,7 +962,7 @@ __device__ __forceinline__ void eval_14(const V &v, const MMKit &k, int2 lj, int
   const double qiqj = k.q[gi] * k.q[gl] * inv_scee;
   ele = qiqj * invr;
   double fmag = 0.0;
-  if (LJTABLE) {
+  if (lj_table) {
     const double2 ab = k.ljab14[lj.x + (k.lj_type[gi] * lj.y) + k.lj_type[gl]];
     const double lja = ab.x * inv_scnb;
     const double ljb = ab.y * inv_scnb;
@@ -954,8 +992,12 @@ __device__ __forceinline__ void eval_14(const V &v, const MMKit &k, int2 lj, int
   }
 }
 
-template <typename T, bool FORCES, bool LJTABLE>
-__device__ void stage_valence(StageCtx<T> &c, const MMKit &k) {
+// Valence term t of the work unit (nothing when t is past the unit's last term).
+template <typename T, bool FORCES>
+__device__ void valence_term(StageCtx<T> &c, const MMKit &k, int t) {
+  if (t >= c.wu.term_count) {
+    return;
+  }
   const double fscale = ldexp(1.0, k.frc_bits);
   const double nrg_scale = ldexp(1.0, k.nrg_bits);
   ValenceCtxT<typename Prec<T>::crd_t> v;
@@ -967,36 +1009,44 @@ __device__ void stage_valence(StageCtx<T> &c, const MMKit &k) {
   v.fx = c.L.fx;
   v.fy = c.L.fy;
   v.fz = c.L.fz;
-  for (int t = c.tid; t < c.wu.term_count; t += c.nthr) {
-    const int slot = find_slot(c.L.vpre, c.wu.sys_count, t);
-    const WuSys &ws = c.L.sys[slot];
-    const int4 vt = k.vterms[ws.term_first + (t - c.L.vpre[slot])];
-    const int kind = vt.x;
-    const int i = ws.lds_base + (vt.y & 0xffff), j = ws.lds_base + (vt.y >> 16);
-    const int kk = ws.lds_base + (vt.z & 0xffff), l = ws.lds_base + (vt.z >> 16);
-    long long *nrg = &c.L.nrg[slot * 16];
-    if (kind == kVBond) {
-      const double2 p = k.bond_par[vt.w];
-      lds_add(&nrg[kBond], fix64(eval_bond<FORCES>(v, i, j, p.x, p.y, fscale) * nrg_scale));
+  const int slot = find_slot(c.L.vpre, c.wu.sys_count, t);
+  const WuSys &ws = c.L.sys[slot];
+  const int4 vt = k.vterms[ws.term_first + (t - c.L.vpre[slot])];
+  const int kind = vt.x;
+  const int i = ws.lds_base + (vt.y & 0xffff), j = ws.lds_base + (vt.y >> 16);
+  const int kk = ws.lds_base + (vt.z & 0xffff), l = ws.lds_base + (vt.z >> 16);
+  long long *nrg = &c.L.nrg[slot * 16];
+  if (kind == kVBond) {
+    const double2 p = k.bond_par[vt.w];
+    lds_add(&nrg[kBond], fix64(eval_bond<FORCES>(v, i, j, p.x, p.y, fscale) * nrg_scale));
+  }
+  else if (kind == kVAngle) {
+    const double2 p = k.angl_par[vt.w];
+    lds_add(&nrg[kAngle], fix64(eval_angle<FORCES>(v, i, j, kk, p.x, p.y, fscale) * nrg_scale));
+  }
+  else {
+    const DiheParam p = k.dihe_par[vt.w];
+    const double e = eval_dihedral<FORCES>(v, i, j, kk, l, p, fscale);
+    lds_add(&nrg[p.improper ? kImproperDihedral : kProperDihedral], fix64(e * nrg_scale));
+    const double2 a14 = k.dihe14[vt.w];
+    if (a14.x > 0.0) {
+      double ele, vdw;
+      eval_14<FORCES>(v, k, c.wu.lj_table != 0, make_int2(ws.lj_off, ws.lj_nt), i, l, a14.x,
+                      a14.y, fscale, ele, vdw);
+      lds_add(&nrg[kElec14], fix64(ele * nrg_scale));
+      lds_add(&nrg[kVdw14], fix64(vdw * nrg_scale));
     }
-    else if (kind == kVAngle) {
-      const double2 p = k.angl_par[vt.w];
-      lds_add(&nrg[kAngle],
-              fix64(eval_angle<FORCES>(v, i, j, kk, p.x, p.y, fscale) * nrg_scale));
-    }
-    else {
-      const DiheParam p = k.dihe_par[vt.w];
-      const double e = eval_dihedral<FORCES>(v, i, j, kk, l, p, fscale);
-      lds_add(&nrg[p.improper ? kImproperDihedral : kProperDihedral], fix64(e * nrg_scale));
-      const double2 a14 = k.dihe14[vt.w];
-      if (a14.x > 0.0) {
-        double ele, vdw;
-        eval_14<FORCES, LJTABLE>(v, k, make_int2(ws.lj_off, ws.lj_nt), i, l, a14.x, a14.y,
-                                 fscale, ele, vdw);
-        lds_add(&nrg[kElec14], fix64(ele * nrg_scale));
-        lds_add(&nrg[kVdw14], fix64(vdw * nrg_scale));
-      }
-    }
+  }
+}
+
+// Valence terms on their own (evaluations without pair tiles), in chunks of 64 per wavefront.
+template <typename T, bool FORCES>
+__device__ void stage_valence(StageCtx<T> &c, const MMKit &k) {
+  const int lane = c.tid & 63;
+  const int chunks = (c.wu.term_count + 63) >> 6;
+  for (int item = next_item(&c.L.work[2], lane); item < chunks;
+       item = next_item(&c.L.work[2], lane)) {
+    valence_term<T, FORCES>(c, k, (item << 6) + lane);
   }
   __syncthreads();
 }
@@ -1008,8 +1058,10 @@ template <typename T, bool FORCES>
 __device__ void evaluate_wu(StageCtx<T> &c, const MMKit &k, int terms) {
   const bool lj_table = (c.wu.lj_table != 0);
   stage_load_coords<T>(c, k);
+  // With GB the valence terms ride along with the Born-integral tiles.
+  const bool val = (terms & kTermValence) != 0;
   if (c.gb) {
-    stage_psi<T>(c, k);
+    stage_psi<T, FORCES>(c, k, val ? ((c.wu.term_count + 63) >> 6) : 0);
     stage_radii<T, FORCES>(c, k, 0.0);
   }
   if (terms & kTermNonbonded) {
@@ -1034,13 +1086,8 @@ __device__ void evaluate_wu(StageCtx<T> &c, const MMKit &k, int terms) {
     stage_chain<T>(c, k);
     stage_deriv<T>(c, k);
   }
-  if (terms & kTermValence) {
-    if (lj_table) {
-      stage_valence<T, FORCES, true>(c, k);
-    }
-    else {
-      stage_valence<T, FORCES, false>(c, k);
-    }
+  if (val && !c.gb) {
+    stage_valence<T, FORCES>(c, k);
   }
 }
 

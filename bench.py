@@ -134,7 +134,8 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu):
         fixed = [np.rint(np.asarray(c) * 2.0 ** 32).astype(np.int64) for c in coords[:nsample]]
 
         def one(i):
-            oracle.minimize(topos[i], fixed[i], 32, 23, 26, cycles, sd, dx0, dp_mode=False)
+            oracle.minimize(topos[i], fixed[i], 32, 23, 26, cycles, sd, dx0, dp_mode=False,
+                            clash_cycles=min(25, cycles))
 
         t0 = time.perf_counter()
         with ThreadPoolExecutor(threads) as ex:

@@ -148,12 +148,18 @@ typedef struct {
 } stormm_ps_writer;
 
 /* Minimization controls, the subset of MinimizeControls this path reads
- * (src/Namelists/nml_minimize.h:21-30). */
+ * (src/Namelists/nml_minimize.h:21-30).  The reference's defaults are maxcyc 200, ncyc 50,
+ * cdcyc 25, dx0 0.01, clash_r0 0.2, clash_vdw_ratio 0.5.  During the first cdcyc cycles the
+ * evaluations damp clashes with a floor and ratio shrinking linearly to zero
+ * (src/MolecularMechanics/hpc_minimization.cu:269-281); 0 <= cdcyc <= maxcyc. */
 typedef struct {
   int32_t total_cycles;            /* maxcyc                                                    */
   int32_t steepest_descent_cycles; /* ncyc                                                      */
   double initial_step;             /* dx0 (A)                                                   */
   double convergence_rms_force;    /* stop a system once its RMS force <= this (0 = never)      */
+  int32_t clash_damping_cycles;    /* cdcyc                                                     */
+  double clash_minimum_distance;   /* clash_r0 (A): Coulomb is softened inside this distance     */
+  double clash_vdw_ratio;          /* clash_vdw_ratio: LJ is softened inside ratio * sigma_ij    */
 } stormm_minimize_controls;
 
 const char *stormm_last_error(void);
@@ -230,19 +236,25 @@ int stormm_line_min_download(const stormm_line_min *lm, double *values9xN);
 /* ---- Launch entry points ------------------------------------------------------------------ */
 /* Coulomb + Lennard-Jones + (if the synthesis has an implicit-solvent model) Born radii, GB pair
  * energies and the radius-derivative pass.  Forces are added into the PhaseSpaceSynthesis force
- * arrays and energies into the ScoreCard instantaneous accumulators. */
+ * arrays and energies into the ScoreCard instantaneous accumulators.  clash_minimum_distance
+ * and clash_ratio are the reference's trailing arguments (hpc_nonbonded_potential.h:131-157):
+ * when either exceeds 10^-6, Coulomb is softened inside that distance and Lennard-Jones inside
+ * that multiple of sigma_ij (pass 0, 0 for the plain potentials). */
 int stormm_launch_nonbonded(int32_t precision, const stormm_synthesis *ag,
                             stormm_mm_controls *ctrl, stormm_phasespace *ps,
                             stormm_scorecard *sc, int32_t eval_force, int32_t eval_energy,
-                            void *stream);
-/* Bonds, angles, cosine dihedrals and attenuated 1:4 pairs. */
+                            double clash_minimum_distance, double clash_ratio, void *stream);
+/* Bonds, angles, cosine dihedrals and attenuated 1:4 pairs (clash damping of the 1:4 pairs as
+ * in hpc_valence_potential.h:356-378). */
 int stormm_launch_valence(int32_t precision, const stormm_synthesis *ag,
                           stormm_mm_controls *ctrl, stormm_phasespace *ps, stormm_scorecard *sc,
-                          int32_t eval_force, int32_t eval_energy, void *stream);
+                          int32_t eval_force, int32_t eval_energy, double clash_minimum_distance,
+                          double clash_ratio, void *stream);
 /* All terms in one fused launch. */
 int stormm_launch_mm_evaluation(int32_t precision, const stormm_synthesis *ag,
                                 stormm_mm_controls *ctrl, stormm_phasespace *ps,
                                 stormm_scorecard *sc, int32_t eval_force, int32_t eval_energy,
+                                double clash_minimum_distance, double clash_ratio,
                                 void *stream);
 /* Polak-Ribiere direction mixing and normalization of the force arrays per system. */
 int stormm_launch_conjugate_gradient(int32_t precision, const stormm_synthesis *ag,

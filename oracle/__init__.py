@@ -82,6 +82,10 @@ def lib():
         L.orc_eval_mm.argtypes = [ctypes.POINTER(OrcSystem), vp, vp, vp, vp, vp, vp,
                                   ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcEnergy)]
         L.orc_eval_mm.restype = None
+        L.orc_eval_mm_clash.argtypes = [ctypes.POINTER(OrcSystem), vp, vp, vp, vp, vp, vp,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, ctypes.POINTER(OrcEnergy)]
+        L.orc_eval_mm_clash.restype = None
         L.orc_eval_terms.argtypes = [ctypes.POINTER(OrcSystem), vp, vp, vp, vp, vp, vp,
                                      ctypes.c_uint, ctypes.c_int, ctypes.POINTER(OrcEnergy)]
         L.orc_eval_terms.restype = None
@@ -96,6 +100,7 @@ def lib():
         L.orc_cubic_move.restype = ctypes.c_double
         L.orc_minimize.argtypes = [ctypes.POINTER(OrcSystem), vp, vp, vp, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_int, vp]
         L.orc_minimize.restype = None
         _lib = L
@@ -176,9 +181,10 @@ class SystemHandle:
 
 
 def eval_mm(topo, xyz: np.ndarray, eval_force: bool = True, nrg_bits: int = 26,
-            handle: Optional[SystemHandle] = None) -> Tuple[Dict[str, float], Dict[str, int],
-                                                            np.ndarray]:
-    """One FP64 evaluation; returns (double energies, fixed-point energies, forces (N, 3))."""
+            handle: Optional[SystemHandle] = None, clash_distance: float = 0.0,
+            clash_ratio: float = 0.0) -> Tuple[Dict[str, float], Dict[str, int], np.ndarray]:
+    """One FP64 evaluation; returns (double energies, fixed-point energies, forces (N, 3)).
+    A positive clash_distance or clash_ratio softens Coulomb, Lennard-Jones and 1:4 pairs."""
     h = handle or SystemHandle(topo)
     xyz = np.asarray(xyz, dtype=np.float64)
     x = np.ascontiguousarray(xyz[:, 0])
@@ -187,8 +193,9 @@ def eval_mm(topo, xyz: np.ndarray, eval_force: bool = True, nrg_bits: int = 26,
     n = topo.natom
     fx, fy, fz = np.zeros(n), np.zeros(n), np.zeros(n)
     e = OrcEnergy()
-    lib().orc_eval_mm(ctypes.byref(h.struct), _ptr(x), _ptr(y), _ptr(z), _ptr(fx), _ptr(fy),
-                      _ptr(fz), int(eval_force), nrg_bits, ctypes.byref(e))
+    lib().orc_eval_mm_clash(ctypes.byref(h.struct), _ptr(x), _ptr(y), _ptr(z), _ptr(fx),
+                            _ptr(fy), _ptr(fz), int(eval_force), nrg_bits, float(clash_distance),
+                            float(clash_ratio), ctypes.byref(e))
     nrg = {name: e.nrg[k] for k, name in enumerate(SLOT_NAMES)}
     acc = {name: e.acc[k] for k, name in enumerate(SLOT_NAMES)}
     return nrg, acc, np.stack([fx, fy, fz], axis=1)
@@ -253,8 +260,11 @@ def cubic_move(evec: Sequence[float], mvec: Sequence[float]):
 
 def minimize(topo, xyz_fixed: np.ndarray, gpos_bits: int, frc_bits: int, nrg_bits: int,
              nsteps: int, sd_cycles: int = 50, dx0: float = 0.01, dp_mode: bool = False,
-             handle: Optional[SystemHandle] = None):
-    """GPU-semantics minimization on fixed-point coordinates; returns (coords, energy trace)."""
+             handle: Optional[SystemHandle] = None, clash_cycles: int = 0,
+             clash_r0: float = 0.2, clash_ratio: float = 0.5):
+    """GPU-semantics minimization on fixed-point coordinates; returns (coords, energy trace).
+    The first clash_cycles cycles run with clash damping shrinking from (clash_r0,
+    clash_ratio) to zero."""
     h = handle or SystemHandle(topo)
     c = np.array(xyz_fixed, dtype=np.int64).reshape(-1, 3)
     x = np.ascontiguousarray(c[:, 0])
@@ -262,7 +272,8 @@ def minimize(topo, xyz_fixed: np.ndarray, gpos_bits: int, frc_bits: int, nrg_bit
     z = np.ascontiguousarray(c[:, 2])
     trace = np.zeros(nsteps + 1)
     lib().orc_minimize(ctypes.byref(h.struct), _ptr(x), _ptr(y), _ptr(z), gpos_bits, frc_bits,
-                       nrg_bits, nsteps, sd_cycles, dx0, int(dp_mode), _ptr(trace))
+                       nrg_bits, nsteps, sd_cycles, dx0, int(clash_cycles), float(clash_r0),
+                       float(clash_ratio), int(dp_mode), _ptr(trace))
     return np.stack([x, y, z], axis=1), trace
 
 

@@ -36,14 +36,84 @@ static const double PI_LF = 3.14159265358979323846;
 
 static inline int imin(int a, int b) { return a < b ? a : b; }
 
+/* constants::tiny, src/Constants/scaling.h:26 (the radius below which a softened force is not
+ * applied). */
+static const double TINY_LF = 1.0e-10;
+
 /* ----------------------------------------------------------------------------------------- */
-/* Coulomb + Lennard-Jones, src/Potential/nonbonded_potential.tpp:9-425 (the branches without  */
-/* clash damping: lines 174-231 and 312-373).                                                  */
+/* Clash damping: softened cores that replace Coulomb inside clash_distance and Lennard-Jones  */
+/* inside clash_ratio * sigma_ij, each matching the plain potential and its slope at the       */
+/* boundary.  src/Potential/soft_core_potentials.tpp:32-55 (quadratic electrostatic core) and  */
+/* :145-188 (quartic Lennard-Jones core).  fmag receives dE/dr / r, the convention of the pair */
+/* loops (force on i = fmag * (r_j - r_i)).                                                     */
+/* ----------------------------------------------------------------------------------------- */
+static void quadratic_core_coulomb(double r, double clash_distance, double qiqj, double *e,
+                                   double *fmag) {
+  if (r < clash_distance) {
+    const double aparm = -0.5 / (clash_distance * clash_distance * (clash_distance + 1.0));
+    const double bparm = (1.0 / clash_distance) -
+                         (aparm * (clash_distance + 1.0) * (clash_distance + 1.0));
+    *e += qiqj * ((aparm * (r + 1.0) * (r + 1.0)) + bparm);
+    if (fmag != NULL && r > TINY_LF) {
+      *fmag += qiqj * ((2.0 * aparm) + (2.0 * aparm / r));
+    }
+  }
+  else {
+    const double invr = 1.0 / r;
+    const double invr2 = invr * invr;
+    *e += qiqj * invr;
+    if (fmag != NULL) {
+      *fmag += -(qiqj * invr * invr2);
+    }
+  }
+}
+
+static void quartic_core_lj(double r, double clash_ratio, double lja, double ljb, double sigma,
+                            double *e, double *fmag) {
+  const double rlimit = clash_ratio * sigma;
+  if (r < rlimit) {
+    const double invrlim = 1.0 / rlimit;
+    const double invrlim2 = invrlim * invrlim;
+    const double invrlim6 = invrlim2 * invrlim2 * invrlim2;
+    const double aparm = (((6.0 * ljb) - (12.0 * lja * invrlim6)) * invrlim * invrlim6) /
+                         ((((((4.0 * rlimit) + 12.0) * rlimit) + 12.0) * rlimit) + 4.0);
+    const double rlimit_plus_one = rlimit + 1.0;
+    const double arlimit_po_four = aparm * (rlimit_plus_one * rlimit_plus_one *
+                                            rlimit_plus_one * rlimit_plus_one);
+    const double bparm = (((lja * invrlim6) - ljb) * invrlim6) - arlimit_po_four;
+    const double r_plus_one = r + 1.0;
+    const double arpo_three = aparm * r_plus_one * r_plus_one * r_plus_one;
+    *e += (arpo_three * r_plus_one) + bparm;
+    if (fmag != NULL && r > TINY_LF) {
+      *fmag += 4.0 * arpo_three / r;
+    }
+  }
+  else {
+    const double invr2 = 1.0 / (r * r);
+    const double invr4 = invr2 * invr2;
+    *e += (lja * invr4 * invr4 * invr4) - (ljb * invr4 * invr2);
+    if (fmag != NULL) {
+      *fmag += ((6.0 * ljb) - (12.0 * lja * invr4 * invr2)) * invr4 * invr4;
+    }
+  }
+}
+
+/* Pairwise sigma from the A and B coefficients, src/Topology/atomgraph_detailers.cpp:268-284. */
+static inline double pair_sigma(double lja, double ljb) {
+  return (ljb > 1.0e-6) ? sqrt(cbrt(lja / ljb)) : 0.0;
+}
+
+/* ----------------------------------------------------------------------------------------- */
+/* Coulomb + Lennard-Jones, src/Potential/nonbonded_potential.tpp:9-425: the plain branches     */
+/* (lines 174-231 and 312-373) and, when clash_distance or clash_ratio is positive, the         */
+/* clash-damped branches (lines 108-172 and 244-311).                                           */
 /* ----------------------------------------------------------------------------------------- */
 static void nonbonded_impl(const orc_system *s, const double *x, const double *y,
                            const double *z, double *fx, double *fy, double *fz, int frc_elec,
-                           int frc_vdw, double nrg_scale, orc_energy *e) {
+                           int frc_vdw, double nrg_scale, double clash_distance,
+                           double clash_ratio, orc_energy *e) {
   const int eval_force = (frc_elec || frc_vdw);
+  const int clash_check = (clash_distance > 0.0 || clash_ratio > 0.0);
   const int n = s->natom;
   const int nst = (n + STILE_LEN - 1) / STILE_LEN;
   const int nt = s->n_lj_types;
@@ -96,6 +166,24 @@ static void nonbonded_impl(const orc_system *s, const double *x, const double *y
               const double dx = cjx[j] - xi;
               const double dy = cjy[j] - yi;
               const double dz = cjz[j] - zi;
+              if (clash_check) {
+                const double r = sqrt((dx * dx) + (dy * dy) + (dz * dz));
+                const double qiqj = qi * cjq[j];
+                const int ljt_j = cjlj[j];
+                const size_t lij = ((size_t)ljt_j * nt) + ljt_i;
+                double fmag = 0.0;
+                quadratic_core_coulomb(r, clash_distance, qiqj, &ele_c,
+                                       frc_elec ? &fmag : NULL);
+                quartic_core_lj(r, clash_ratio, s->lja[lij], s->ljb[lij],
+                                pair_sigma(s->lja[lij], s->ljb[lij]), &vdw_c,
+                                frc_vdw ? &fmag : NULL);
+                if (eval_force) {
+                  const double fdx = fmag * dx, fdy = fmag * dy, fdz = fmag * dz;
+                  cifx[i] += fdx; cify[i] += fdy; cifz[i] += fdz;
+                  cjfx[j] -= fdx; cjfy[j] -= fdy; cjfz[j] -= fdz;
+                }
+                continue;
+              }
               const double invr2 = 1.0 / ((dx * dx) + (dy * dy) + (dz * dz));
               const double invr = sqrt(invr2);
               const double invr4 = invr2 * invr2;
@@ -143,7 +231,7 @@ static void nonbonded_impl(const orc_system *s, const double *x, const double *y
 void orc_nonbonded(const orc_system *s, const double *x, const double *y, const double *z,
                    double *fx, double *fy, double *fz, int eval_force, double nrg_scale,
                    orc_energy *e) {
-  nonbonded_impl(s, x, y, z, fx, fy, fz, eval_force, eval_force, nrg_scale, e);
+  nonbonded_impl(s, x, y, z, fx, fy, fz, eval_force, eval_force, nrg_scale, 0.0, 0.0, e);
 }
 
 /* ----------------------------------------------------------------------------------------- */
@@ -607,11 +695,13 @@ static double dihedral_twist(int i, int j, int k, int l, double amp, double phas
   return amp * (1.0 + cos(sangle));
 }
 
-/* src/Potential/valence_potential.tpp:1288-1419 (standard branch, no clash damping) */
+/* src/Potential/valence_potential.tpp:1288-1419: the standard branch and, when clash_distance or
+ * clash_ratio is positive, the clash-damped branch (lines 1337-1360), whose sigma comes from the
+ * unattenuated 1:4 coefficients. */
 static void attenuated_14(const orc_system *s, int i, int l, double scee, double scnb,
                           const double *x, const double *y, const double *z, double *fx,
-                          double *fy, double *fz, int frc_elec, int frc_vdw, double *ele,
-                          double *vdw) {
+                          double *fy, double *fz, int frc_elec, int frc_vdw,
+                          double clash_distance, double clash_ratio, double *ele, double *vdw) {
   const double dx = x[l] - x[i];
   const double dy = y[l] - y[i];
   const double dz = z[l] - z[i];
@@ -621,6 +711,20 @@ static void attenuated_14(const orc_system *s, int i, int l, double scee, double
   const double lja = s->lja14[ij] / scnb;
   const double ljb = s->ljb14[ij] / scnb;
   const double qiqj = (s->coulomb * s->charge[i] * s->charge[l]) / scee;
+  if (clash_distance > 0.0 || clash_ratio > 0.0) {
+    const double r = sqrt((dx * dx) + (dy * dy) + (dz * dz));
+    double fmag = 0.0;
+    *ele = 0.0;
+    *vdw = 0.0;
+    quadratic_core_coulomb(r, clash_distance, qiqj, ele, frc_elec ? &fmag : NULL);
+    quartic_core_lj(r, clash_ratio, lja, ljb, pair_sigma(s->lja14[ij], s->ljb14[ij]), vdw,
+                    frc_vdw ? &fmag : NULL);
+    if (frc_elec || frc_vdw) {
+      fx[i] += fmag * dx; fy[i] += fmag * dy; fz[i] += fmag * dz;
+      fx[l] -= fmag * dx; fy[l] -= fmag * dy; fz[l] -= fmag * dz;
+    }
+    return;
+  }
   const double invr2 = 1.0 / ((dx * dx) + (dy * dy) + (dz * dz));
   const double invr = sqrt(invr2);
   const double invr4 = invr2 * invr2;
@@ -638,7 +742,8 @@ static void attenuated_14(const orc_system *s, int i, int l, double scee, double
 
 static void valence_impl(const orc_system *s, const double *x, const double *y,
                          const double *z, double *fx, double *fy, double *fz, unsigned fmask,
-                         double nrg_scale, orc_energy *e) {
+                         double nrg_scale, double clash_distance, double clash_ratio,
+                         orc_energy *e) {
   int eval_force = (fmask & ORC_TERM_BOND) != 0;
   /* Bonds, src/Potential/valence_potential.tpp:100-140 (bond_leq is taken in absolute value). */
   double sum = 0.0;
@@ -698,7 +803,7 @@ static void valence_impl(const orc_system *s, const double *x, const double *y,
     double ele, vdw;
     attenuated_14(s, s->dihe_ijkl[4 * p], s->dihe_ijkl[4 * p + 3], s->dihe_scee[p],
                   s->dihe_scnb[p], x, y, z, fx, fy, fz, (fmask & ORC_TERM_QQ14) != 0,
-                  (fmask & ORC_TERM_LJ14) != 0, &ele, &vdw);
+                  (fmask & ORC_TERM_LJ14) != 0, clash_distance, clash_ratio, &ele, &vdw);
     esum += ele;
     vsum += vdw;
     eacc += llround(ele * nrg_scale);
@@ -711,7 +816,7 @@ static void valence_impl(const orc_system *s, const double *x, const double *y,
     double ele, vdw;
     attenuated_14(s, s->infr14_il[2 * p], s->infr14_il[2 * p + 1], s->infr14_scee[p],
                   s->infr14_scnb[p], x, y, z, fx, fy, fz, (fmask & ORC_TERM_QQ14) != 0,
-                  (fmask & ORC_TERM_LJ14) != 0, &ele, &vdw);
+                  (fmask & ORC_TERM_LJ14) != 0, clash_distance, clash_ratio, &ele, &vdw);
     esum += ele;
     vsum += vdw;
     eacc += llround(ele * nrg_scale);
@@ -726,7 +831,7 @@ static void valence_impl(const orc_system *s, const double *x, const double *y,
 void orc_valence(const orc_system *s, const double *x, const double *y, const double *z,
                  double *fx, double *fy, double *fz, int eval_force, double nrg_scale,
                  orc_energy *e) {
-  valence_impl(s, x, y, z, fx, fy, fz, eval_force ? ORC_TERM_ALL : 0u, nrg_scale, e);
+  valence_impl(s, x, y, z, fx, fy, fz, eval_force ? ORC_TERM_ALL : 0u, nrg_scale, 0.0, 0.0, e);
 }
 
 void orc_eval_terms(const orc_system *s, const double *x, const double *y, const double *z,
@@ -735,8 +840,8 @@ void orc_eval_terms(const orc_system *s, const double *x, const double *y, const
   const double nrg_scale = ldexp(1.0, nrg_bits);
   memset(e, 0, sizeof(*e));
   nonbonded_impl(s, x, y, z, fx, fy, fz, (force_mask & ORC_TERM_QQ) != 0,
-                 (force_mask & ORC_TERM_LJ) != 0, nrg_scale, e);
-  valence_impl(s, x, y, z, fx, fy, fz, force_mask, nrg_scale, e);
+                 (force_mask & ORC_TERM_LJ) != 0, nrg_scale, 0.0, 0.0, e);
+  valence_impl(s, x, y, z, fx, fy, fz, force_mask, nrg_scale, 0.0, 0.0, e);
   if ((force_mask & ORC_TERM_GB) || !(force_mask & ORC_TERM_ALL)) {
     orc_gb(s, x, y, z, fx, fy, fz, (force_mask & ORC_TERM_GB) != 0, nrg_scale, e, NULL, NULL,
            NULL);
@@ -746,14 +851,22 @@ void orc_eval_terms(const orc_system *s, const double *x, const double *y, const
   }
 }
 
+void orc_eval_mm_clash(const orc_system *s, const double *x, const double *y, const double *z,
+                       double *fx, double *fy, double *fz, int eval_force, int nrg_bits,
+                       double clash_distance, double clash_ratio, orc_energy *e) {
+  const double nrg_scale = ldexp(1.0, nrg_bits);
+  memset(e, 0, sizeof(*e));
+  nonbonded_impl(s, x, y, z, fx, fy, fz, eval_force, eval_force, nrg_scale, clash_distance,
+                 clash_ratio, e);
+  valence_impl(s, x, y, z, fx, fy, fz, eval_force ? ORC_TERM_ALL : 0u, nrg_scale,
+               clash_distance, clash_ratio, e);
+  orc_gb(s, x, y, z, fx, fy, fz, eval_force, nrg_scale, e, NULL, NULL, NULL);
+}
+
 void orc_eval_mm(const orc_system *s, const double *x, const double *y, const double *z,
                  double *fx, double *fy, double *fz, int eval_force, int nrg_bits,
                  orc_energy *e) {
-  const double nrg_scale = ldexp(1.0, nrg_bits);
-  memset(e, 0, sizeof(*e));
-  orc_nonbonded(s, x, y, z, fx, fy, fz, eval_force, nrg_scale, e);
-  orc_valence(s, x, y, z, fx, fy, fz, eval_force, nrg_scale, e);
-  orc_gb(s, x, y, z, fx, fy, fz, eval_force, nrg_scale, e, NULL, NULL, NULL);
+  orc_eval_mm_clash(s, x, y, z, fx, fy, fz, eval_force, nrg_bits, 0.0, 0.0, e);
 }
 
 /* ----------------------------------------------------------------------------------------- */
@@ -991,8 +1104,8 @@ static double line_energy(const orc_energy *e, int nrg_bits) {
 
 static void eval_fixed(const orc_system *s, const long long *xc, const long long *yc,
                        const long long *zc, int gpos_bits, int frc_bits, int nrg_bits,
-                       int eval_force, long long *ifx, long long *ify, long long *ifz,
-                       double *work, orc_energy *e) {
+                       int eval_force, double clash_distance, double clash_ratio, long long *ifx,
+                       long long *ify, long long *ifz, double *work, orc_energy *e) {
   const int n = s->natom;
   double *x = work, *y = work + n, *z = work + 2 * n;
   double *fx = work + 3 * n, *fy = work + 4 * n, *fz = work + 5 * n;
@@ -1003,7 +1116,8 @@ static void eval_fixed(const orc_system *s, const long long *xc, const long long
     z[i] = (double)zc[i] * inv_gpos;
     fx[i] = fy[i] = fz[i] = 0.0;
   }
-  orc_eval_mm(s, x, y, z, fx, fy, fz, eval_force, nrg_bits, e);
+  orc_eval_mm_clash(s, x, y, z, fx, fy, fz, eval_force, nrg_bits, clash_distance, clash_ratio,
+                    e);
   if (eval_force) {
     const double fscale = ldexp(1.0, frc_bits);
     for (int i = 0; i < n; i++) {
@@ -1014,9 +1128,28 @@ static void eval_fixed(const orc_system *s, const long long *xc, const long long
   }
 }
 
+/* Clash-damping parameters of cycle i, src/MolecularMechanics/hpc_minimization.cu:269-274: the
+ * floor and ratio shrink linearly to zero over the first clash_cycles cycles.  The damped
+ * kernels run when either value exceeds 10^-6 (hpc_nonbonded_potential.cu:1476 in double
+ * precision, :1662 in single precision); otherwise the plain potentials apply. */
+static void clash_schedule(int i, int clash_cycles, double clash_r0, double clash_ratio,
+                           int dp_mode, double *d_out, double *r_out) {
+  double d = 0.0, r = 0.0;
+  if (i < clash_cycles) {
+    const double progress = (double)(clash_cycles - i) / (double)clash_cycles;
+    d = clash_r0 * progress;
+    r = clash_ratio * progress;
+  }
+  const int on = dp_mode ? (d >= 1.0e-6 || r >= 1.0e-6)
+                         : ((float)d > 0.000001f || (float)r > 0.000001f);
+  *d_out = on ? d : 0.0;
+  *r_out = on ? r : 0.0;
+}
+
 void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long long *zcrd,
                   int gpos_bits, int frc_bits, int nrg_bits, int nsteps, int sd_cycles,
-                  double dx0, int dp_mode, double *energy_trace) {
+                  double dx0, int clash_cycles, double clash_r0, double clash_ratio,
+                  int dp_mode, double *energy_trace) {
   const int n = s->natom;
   long long *buf = (long long *)calloc((size_t)9 * n, sizeof(long long));
   long long *fx = buf, *fy = buf + n, *fz = buf + 2 * n;
@@ -1036,8 +1169,11 @@ void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long lo
   long long *cgt[3] = {cx, cy, cz};
 
   for (int it = 0; it < nsteps; it++) {
+    double cd, cr;
+    clash_schedule(it, clash_cycles, clash_r0, clash_ratio, dp_mode, &cd, &cr);
     /* Force + energy evaluation (hpc_minimization.cu:264-287). */
-    eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 1, fx, fy, fz, work, &e);
+    eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 1, cd, cr, fx, fy, fz, work,
+               &e);
     step_counter += 1;
 
     /* Conjugate gradient all-reduce (conjugate_gradient.cui:259-370, hpc_reduction.cu:39-244). */
@@ -1086,8 +1222,8 @@ void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long lo
       if (move > 0) {
         long long dummy = 0;
         (void)dummy;
-        eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 0, NULL, NULL, NULL,
-                   work, &e);
+        eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 0, cd, cr, NULL, NULL,
+                   NULL, work, &e);
       }
       const double cur = line_energy(&e, nrg_bits);
       if (move == 0 && energy_trace != NULL) {
@@ -1130,7 +1266,8 @@ void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long lo
     }
   }
   if (energy_trace != NULL) {
-    eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 0, NULL, NULL, NULL, work, &e);
+    eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 0, 0.0, 0.0, NULL, NULL, NULL,
+               work, &e);
     energy_trace[nsteps] = line_energy(&e, nrg_bits);
   }
   (void)nrg_d;

@@ -95,6 +95,13 @@ void orc_valence(const orc_system *s, const double *x, const double *y, const do
 void orc_eval_mm(const orc_system *s, const double *x, const double *y, const double *z,
                  double *fx, double *fy, double *fz, int eval_force, int nrg_bits, orc_energy *e);
 
+/* orc_eval_mm with clash damping of the Coulomb, Lennard-Jones and 1:4 terms (both parameters
+ * zero gives the plain potentials): src/Potential/soft_core_potentials.tpp:32-55,145-188 as
+ * called from nonbonded_potential.tpp:108-172,244-311 and valence_potential.tpp:1337-1360. */
+void orc_eval_mm_clash(const orc_system *s, const double *x, const double *y, const double *z,
+                       double *fx, double *fy, double *fz, int eval_force, int nrg_bits,
+                       double clash_distance, double clash_ratio, orc_energy *e);
+
 /* Evaluate all energies, accumulating forces only from the terms named in force_mask (the
  * per-term force flags of the reference's evaluators, e.g. nonbonded_potential.tpp:13-14). */
 void orc_eval_terms(const orc_system *s, const double *x, const double *y, const double *z,
@@ -121,11 +128,14 @@ double orc_cubic_move(const double evec[4], const double mvec[4], double *new_ba
  * src/MolecularMechanics/line_movement.cui.  Coordinates are fixed-precision integers at
  * gpos_bits; forces are fixed-precision at frc_bits.  dp_mode selects the double-precision
  * model's rounding of the normalized move (doubleToInt95) instead of the single-precision
- * model's truncation.  energy_trace receives the energy (kcal/mol) at the start of each step
- * and, in its last slot, the final energy (length nsteps + 1); may be NULL. */
+ * model's truncation.  The first clash_cycles cycles evaluate with clash damping whose floor
+ * (clash_r0) and ratio shrink linearly to zero (hpc_minimization.cu:269-274).  energy_trace
+ * receives the energy (kcal/mol) at the start of each step and, in its last slot, the final
+ * energy (length nsteps + 1); may be NULL. */
 void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long long *zcrd,
                   int gpos_bits, int frc_bits, int nrg_bits, int nsteps, int sd_cycles,
-                  double dx0, int dp_mode, double *energy_trace);
+                  double dx0, int clash_cycles, double clash_r0, double clash_ratio,
+                  int dp_mode, double *energy_trace);
 
 #ifdef __cplusplus
 }

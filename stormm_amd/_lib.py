@@ -74,7 +74,9 @@ class PsWriter(ctypes.Structure):
 
 class MinimizeControls(ctypes.Structure):
     _fields_ = [("total_cycles", c_int32), ("steepest_descent_cycles", c_int32),
-                ("initial_step", c_double), ("convergence_rms_force", c_double)]
+                ("initial_step", c_double), ("convergence_rms_force", c_double),
+                ("clash_damping_cycles", c_int32), ("clash_minimum_distance", c_double),
+                ("clash_vdw_ratio", c_double)]
 
 
 class PoolConfig(ctypes.Structure):
@@ -131,9 +133,12 @@ _SIGS = {
     "stormm_line_min_create": (c_int32, [c_int32, c_double, P(vp)]),
     "stormm_line_min_destroy": (c_int32, [vp]),
     "stormm_line_min_download": (c_int32, [vp, vp]),
-    "stormm_launch_nonbonded": (c_int32, [c_int32, vp, vp, vp, vp, c_int32, c_int32, vp]),
-    "stormm_launch_valence": (c_int32, [c_int32, vp, vp, vp, vp, c_int32, c_int32, vp]),
-    "stormm_launch_mm_evaluation": (c_int32, [c_int32, vp, vp, vp, vp, c_int32, c_int32, vp]),
+    "stormm_launch_nonbonded": (c_int32, [c_int32, vp, vp, vp, vp, c_int32, c_int32, c_double,
+                                          c_double, vp]),
+    "stormm_launch_valence": (c_int32, [c_int32, vp, vp, vp, vp, c_int32, c_int32, c_double,
+                                        c_double, vp]),
+    "stormm_launch_mm_evaluation": (c_int32, [c_int32, vp, vp, vp, vp, c_int32, c_int32,
+                                              c_double, c_double, vp]),
     "stormm_launch_conjugate_gradient": (c_int32, [c_int32, vp, vp, vp, vp]),
     "stormm_launch_line_advance": (c_int32, [c_int32, vp, vp, vp, vp, c_int32, vp]),
     "stormm_launch_minimization": (c_int32, [c_int32, vp, vp, P(MinimizeControls), vp, vp,

@@ -131,6 +131,15 @@ struct MMKit {
   long long *scorecard;
   int *status;
   int store_overwrite;       // 1: forces and energy rows are written, not added to
+  // Clash damping (src/MolecularMechanics/hpc_minimization.cu:269-281).  clash_variant selects
+  // the kernels that can soften Coulomb, Lennard-Jones and 1:4 pairs.  With clash_cycles == 0
+  // every system takes clash_r0 / clash_ratio as given; otherwise a system at cycle i takes them
+  // scaled by (clash_cycles - i) / clash_cycles while i < clash_cycles, where i is
+  // clash_step[system] + clash_step_offset, or clash_cycle when clash_step is null.
+  int clash_variant;
+  int clash_cycles, clash_cycle, clash_step_offset;
+  const int *clash_step;
+  double clash_r0, clash_ratio;
 };
 
 // Status bits reported by kernels.

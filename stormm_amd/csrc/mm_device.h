@@ -41,6 +41,7 @@ template <> struct Prec<float> {
   static __device__ __forceinline__ float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
   static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
   static __device__ __forceinline__ float sqrt(float x) { return __builtin_sqrtf(x); }
+  static __device__ __forceinline__ float cbrt(float x) { return ::cbrtf(x); }
   static __device__ __forceinline__ float log(float x) {
     return __builtin_amdgcn_logf(x) * 0.693147180559945309417f;
   }
@@ -57,6 +58,7 @@ template <> struct Prec<double> {
   static __device__ __forceinline__ double rsqrt(double x) { return 1.0 / ::sqrt(x); }
   static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
   static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+  static __device__ __forceinline__ double cbrt(double x) { return ::cbrt(x); }
   static __device__ __forceinline__ double log(double x) { return ::log(x); }
   static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
   static __device__ __forceinline__ long long fixed(double x) { return round_to_ll(x); }
@@ -204,6 +206,8 @@ template <typename T> struct WuLds {
   WuSys *sys;
   int *opre, *dpre, *vpre;  // per slot: first off-diagonal task, diagonal task, valence term
   int *work;                // per-stage work-item counters (kStageCounters)
+  T *clash_d, *clash_r;     // per-system clash distance and sigma ratio of this evaluation
+  int *clash_on;            // per-system flag: damped potentials apply
   unsigned *nearq;          // kNearQueue packed pair entries per wavefront
 
   static __host__ __device__ size_t bytes(int maxa, int waves) {
@@ -212,6 +216,7 @@ template <typename T> struct WuLds {
            (3 * kMaxWuSystems * sizeof(long long)) + (8 * m * sizeof(T)) +
            (3 * m * sizeof(crd_t)) + (2 * m * sizeof(int)) + (kMaxWuSystems * sizeof(WuSys)) +
            (3 * (kMaxWuSystems + 1) * sizeof(int)) + (kStageCounters * sizeof(int)) +
+           (2 * kMaxWuSystems * sizeof(T)) + (kMaxWuSystems * sizeof(int)) +
            (static_cast<size_t>(waves) * kNearQueue * sizeof(unsigned));
   }
 
@@ -238,6 +243,8 @@ template <typename T> struct WuLds {
     scr = reinterpret_cast<T *>(take(m * sizeof(T)));
     rad = reinterpret_cast<T *>(take(m * sizeof(T)));
     sd = reinterpret_cast<T *>(take(m * sizeof(T)));
+    clash_d = reinterpret_cast<T *>(take(kMaxWuSystems * sizeof(T)));
+    clash_r = reinterpret_cast<T *>(take(kMaxWuSystems * sizeof(T)));
     cx = reinterpret_cast<crd_t *>(take(m * sizeof(crd_t)));
     cy = reinterpret_cast<crd_t *>(take(m * sizeof(crd_t)));
     cz = reinterpret_cast<crd_t *>(take(m * sizeof(crd_t)));
@@ -248,6 +255,7 @@ template <typename T> struct WuLds {
     dpre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
     vpre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
     work = reinterpret_cast<int *>(take(kStageCounters * sizeof(int)));
+    clash_on = reinterpret_cast<int *>(take(kMaxWuSystems * sizeof(int)));
     nearq = reinterpret_cast<unsigned *>(take(static_cast<size_t>(waves) * kNearQueue *
                                               sizeof(unsigned)));
   }

@@ -21,6 +21,7 @@ template <typename T> struct StageCtx {
   int tid, nthr;
   int crd_shift;     // right shift from global fixed precision to LDS coordinates
   T crd_scale;       // LDS coordinate unit in Angstroms (single precision)
+  int cycle;         // minimization cycle of this evaluation (clash-damping schedule)
   bool gb;
   __device__ StageCtx(unsigned char *smem, int maxa, int waves) : L(smem, maxa, waves) {}
 };
@@ -88,11 +89,18 @@ __device__ void stage_load_params(StageCtx<T> &c, const MMKit &k) {
 }
 
 // Coordinates into LDS (relative to a per-system origin) and all accumulators zeroed.
-template <typename T>
+template <typename T, bool CLASH>
 __device__ void stage_load_coords(StageCtx<T> &c, const MMKit &k) {
   WuLds<T> &L = c.L;
   const WuDesc &wu = c.wu;
   if (c.tid < wu.sys_count) {
+    if (CLASH) {
+      T d, r;
+      const bool on = clash_params<T>(k, L.sys[c.tid].sys_index, c.cycle, d, r);
+      L.clash_d[c.tid] = on ? d : T(0);
+      L.clash_r[c.tid] = on ? r : T(0);
+      L.clash_on[c.tid] = on ? 1 : 0;
+    }
     const int gs = L.sys[c.tid].global_start;
     const long long mask = ~((1LL << c.crd_shift) - 1LL);
     L.org[3 * c.tid + 0] = k.xcrd[gs] & mask;
@@ -151,6 +159,38 @@ __device__ void init_ctx(StageCtx<T> &c, const MMKit &k, int wu_index, int terms
     c.crd_scale = T(1);
   }
   c.gb = (k.igb != 0) && ((terms & kTermNonbonded) != 0);
+  c.cycle = k.clash_cycle;
+}
+
+// Clash-damping parameters of one system for this evaluation
+// (src/MolecularMechanics/hpc_minimization.cu:269-274): the floor and ratio shrink linearly to
+// zero over the first clash_cycles cycles.  The damped potentials apply when either value
+// exceeds 10^-6, the threshold at which the reference launches its damped kernels
+// (src/Potential/hpc_nonbonded_potential.cu:1476 in double precision, :1662 in single).
+template <typename T>
+__device__ __forceinline__ bool clash_params(const MMKit &k, int sys_index, int cycle,
+                                             T &dist, T &ratio) {
+  double d = k.clash_r0, r = k.clash_ratio;
+  if (k.clash_cycles > 0) {
+    const int i = (k.clash_step != nullptr) ? k.clash_step[sys_index] + k.clash_step_offset
+                                            : cycle;
+    if (i < k.clash_cycles) {
+      const double progress = static_cast<double>(k.clash_cycles - i) /
+                              static_cast<double>(k.clash_cycles);
+      d *= progress;
+      r *= progress;
+    }
+    else {
+      d = 0.0;
+      r = 0.0;
+    }
+  }
+  dist = static_cast<T>(d);
+  ratio = static_cast<T>(r);
+  if (sizeof(T) == 8) {
+    return d >= 1.0e-6 || r >= 1.0e-6;
+  }
+  return static_cast<float>(d) > 0.000001f || static_cast<float>(r) > 0.000001f;
 }
 
 // Rotation-step mask of a 16-bit mask over the j atoms of a tile: bit ks of the result is bit
@@ -291,7 +331,7 @@ __device__ __forceinline__ int stage_item(int i, int quads, int chunks) {
   return (c1 > c0) ? c0 : (-1 - (i - c0));
 }
 
-template <typename T, bool FORCES>
+template <typename T, bool FORCES, bool CLASH>
 __device__ void valence_term(StageCtx<T> &c, const MMKit &k, int t);
 
 // Near-field pair queue of a wavefront.  Entries pack the LDS indices of the pair (9 bits each)
@@ -373,7 +413,7 @@ __device__ __forceinline__ void psi_drain(StageCtx<T> &c, const unsigned *q, int
 // The series terms of one task are summed in the working precision (at most 16 small terms per
 // lane) and added to the fixed-precision accumulators once per task.
 // Chunks of valence terms (`val_chunks`, possibly 0) are taken as work items alongside the tiles.
-template <typename T, bool FORCES>
+template <typename T, bool FORCES, bool CLASH>
 __device__ void stage_psi(StageCtx<T> &c, const MMKit &k, int val_chunks) {
   using P = Prec<T>;
   WuLds<T> &L = c.L;
@@ -388,7 +428,7 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k, int val_chunks) {
   for (int item = next_item(&L.work[0], lane); item < items; item = next_item(&L.work[0], lane)) {
     const int it = stage_item(item, quads, val_chunks);
     if (it >= 0) {
-      valence_term<T, FORCES>(c, k, (it << 6) + lane);
+      valence_term<T, FORCES, CLASH>(c, k, (it << 6) + lane);
       continue;
     }
     const TileTask tt(wave_task(c, k, ((-1 - it) << 2) + ((c.tid >> 4) & 3)), il);
@@ -497,7 +537,58 @@ __device__ void stage_radii(StageCtx<T> &c, const MMKit &k, double coulomb_inv_s
 // coefficients, and the squared distance of an absent pair is replaced by 1 to keep the
 // arithmetic finite.  Forces are summed per lane in the working precision and converted to fixed
 // precision once per task.
-template <typename T, bool FORCES, bool LJTABLE, bool GB>
+// Damped Coulomb and Lennard-Jones of one pair at distance r (clash damping,
+// src/Potential/nonbonded_potential_tilegroups.cui:536-602): inside clash distance cd the
+// Coulomb potential becomes the parabola qq (ca (r + 1)^2 + cb), and inside ratio * sigma the
+// Lennard-Jones potential becomes a (r + 1)^4 + b, each meeting the plain potential and its
+// slope at the boundary.  fmag receives dE/dr / r.
+template <typename T, bool FORCES>
+__device__ __forceinline__ void clash_pair(T r, T qq, T lja, T ljb, T sigma, T cd, T ca, T cb,
+                                           T ratio, T &e_qq, T &e_lj, T &fmag) {
+  const T tiny = T(1.0e-10);   // constants::tiny, src/Constants/scaling.h:26
+  if (r < cd) {
+    e_qq += qq * ((ca * (r + T(1)) * (r + T(1))) + cb);
+    if (FORCES && r > tiny) {
+      fmag += qq * ((T(2) * ca) + (T(2) * ca / r));
+    }
+  }
+  else {
+    e_qq += qq / r;
+    if (FORCES) {
+      fmag -= qq / (r * r * r);
+    }
+  }
+  const T vdw_limit = ratio * sigma;
+  if (r < vdw_limit) {
+    const T invrlim = T(1) / vdw_limit;
+    const T invrlim2 = invrlim * invrlim;
+    const T invrlim6 = invrlim2 * invrlim2 * invrlim2;
+    const T aparm = (((T(6) * ljb) - (T(12) * lja * invrlim6)) * invrlim * invrlim6) /
+                    ((((((T(4) * vdw_limit) + T(12)) * vdw_limit) + T(12)) * vdw_limit) + T(4));
+    const T r_plus_one = r + T(1);
+    const T arpo_three = aparm * r_plus_one * r_plus_one * r_plus_one;
+    const T rlimit_plus_one = vdw_limit + T(1);
+    const T arlimit_po_four = aparm * (rlimit_plus_one * rlimit_plus_one * rlimit_plus_one *
+                                       rlimit_plus_one);
+    const T bparm = (((lja * invrlim6) - ljb) * invrlim6) - arlimit_po_four;
+    e_lj += (arpo_three * r_plus_one) + bparm;
+    if (FORCES && r > tiny) {
+      fmag += T(4) * arpo_three / r;
+    }
+  }
+  else {
+    const T invr2 = T(1) / (r * r);
+    const T invr4 = invr2 * invr2;
+    e_lj += ((lja * invr4 * invr2) - ljb) * invr4 * invr2;
+    if (FORCES) {
+      fmag += ((T(6) * ljb) - (T(12) * lja * invr4 * invr2)) * invr4 * invr4;
+    }
+  }
+}
+
+// CLASH selects the kernels that damp Coulomb and Lennard-Jones for the systems whose clash
+// flag is set in this evaluation; the other systems take the plain potentials.
+template <typename T, bool FORCES, bool LJTABLE, bool GB, bool CLASH>
 __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
   using P = Prec<T>;
   WuLds<T> &L = c.L;
@@ -530,6 +621,15 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
     if (LJTABLE) {
       lj_off = L.sys[tt.slot].lj_off;
       lj_nt = L.sys[tt.slot].lj_nt;
+    }
+    bool clash_on = false;
+    T cd = T(0), ca = T(0), cb = T(0), cratio = T(0);
+    if (CLASH) {
+      clash_on = (L.clash_on[tt.slot] != 0);
+      cd = L.clash_d[tt.slot];
+      cratio = L.clash_r[tt.slot];
+      ca = T(-0.5) / (cd * cd * (cd + T(1)));
+      cb = (T(1) / cd) - (ca * (cd + T(1)) * (cd + T(1)));
     }
     for (int ks = 0; ks < tt.nsteps; ks++) {
       const bool valid = step_bit(tt.valid, ks);
@@ -569,34 +669,59 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
         }
       }
       const T qnb = nbok ? qiqj : T(0);
-      const T invr = P::rsqrt(r2);
-      const T invr2 = invr * invr;
-      const T eqq = qnb * invr;
-      e_qq += eqq;
-      if (LJTABLE) {
-        const double2 ab = k.ljab[lj_off + (static_cast<int>(p1i) * lj_nt) +
-                                  static_cast<int>(p1j)];
-        const T lja = nbok ? static_cast<T>(ab.x) : T(0);
-        const T ljb = nbok ? static_cast<T>(ab.y) : T(0);
-        const T invr4 = invr2 * invr2;
-        e_lj += ((lja * invr4 * invr4) - (ljb * invr2)) * invr4;
-        if (FORCES) {
-          fmag += ((T(6) * ljb) - (T(12) * lja * invr4 * invr2)) * invr4 * invr4;
+      if (CLASH && clash_on) {
+        T lja, ljb, sigma;
+        if (LJTABLE) {
+          const double2 ab = k.ljab[lj_off + (static_cast<int>(p1i) * lj_nt) +
+                                    static_cast<int>(p1j)];
+          lja = nbok ? static_cast<T>(ab.x) : T(0);
+          ljb = nbok ? static_cast<T>(ab.y) : T(0);
+          sigma = (ljb > T(1.0e-6)) ? P::sqrt(P::cbrt(lja / ljb)) : T(0);
         }
+        else {
+          const T sig = p1i + p1j;
+          const T sig2 = sig * sig;
+          const T sig6 = sig2 * sig2 * sig2;
+          const T e4 = nbok ? p2i * p2j : T(0);
+          ljb = e4 * sig6;
+          lja = ljb * sig6;
+          sigma = (ljb > T(1.0e-6)) ? sig : T(0);
+        }
+        clash_pair<T, FORCES>(P::sqrt(r2), qnb, lja, ljb, sigma, cd, ca, cb, cratio, e_qq, e_lj,
+                              fmag);
       }
       else {
-        const T sig = p1i + p1j;
-        const T s2 = sig * sig * invr2;
-        const T s6 = s2 * s2 * s2;
-        const T e4 = nbok ? p2i * p2j : T(0);
-        const T e4s6 = e4 * s6;
-        e_lj += e4s6 * (s6 - T(1));
+        const T invr = P::rsqrt(r2);
+        const T invr2 = invr * invr;
+        const T eqq = qnb * invr;
+        e_qq += eqq;
+        if (LJTABLE) {
+          const double2 ab = k.ljab[lj_off + (static_cast<int>(p1i) * lj_nt) +
+                                    static_cast<int>(p1j)];
+          const T lja = nbok ? static_cast<T>(ab.x) : T(0);
+          const T ljb = nbok ? static_cast<T>(ab.y) : T(0);
+          const T invr4 = invr2 * invr2;
+          e_lj += ((lja * invr4 * invr4) - (ljb * invr2)) * invr4;
+          if (FORCES) {
+            fmag += ((T(6) * ljb) - (T(12) * lja * invr4 * invr2)) * invr4 * invr4;
+          }
+        }
+        else {
+          const T sig = p1i + p1j;
+          const T s2 = sig * sig * invr2;
+          const T s6 = s2 * s2 * s2;
+          const T e4 = nbok ? p2i * p2j : T(0);
+          const T e4s6 = e4 * s6;
+          e_lj += e4s6 * (s6 - T(1));
+          if (FORCES) {
+            fmag += e4s6 * (T(6) - (T(12) * s6)) * invr2;
+          }
+        }
         if (FORCES) {
-          fmag += e4s6 * (T(6) - (T(12) * s6)) * invr2;
+          fmag -= eqq * invr2;
         }
       }
       if (FORCES) {
-        fmag -= eqq * invr2;
         fxi += fmag * dx;
         fyi += fmag * dy;
         fzi += fmag * dz;
@@ -949,41 +1074,53 @@ __device__ __forceinline__ double eval_dihedral(const V &v, int i, int j, int kk
   return dp.amp * (1.0 + cos_sangle);
 }
 
-// Attenuated 1:4 pair, src/Potential/valence_potential.tpp:1288-1419.
-template <bool FORCES, typename V>
+// Attenuated 1:4 pair, src/Potential/valence_potential.tpp:1288-1419; with clash damping on,
+// the damped potentials of src/Potential/valence_potential.cui:930-1040, whose sigma comes from
+// the attenuated coefficients.
+template <bool FORCES, bool CLASH, typename V>
 __device__ __forceinline__ void eval_14(const V &v, const MMKit &k, bool lj_table, int2 lj,
                                         int i, int l, double inv_scee, double inv_scnb,
-                                        double fscale, double &ele, double &vdw) {
+                                        double fscale, bool clash_on, double cd, double cratio,
+                                        double &ele, double &vdw) {
   const int gi = v.gidx[i], gl = v.gidx[l];
   const Vec3 d = v.disp(i, l);
   const double r2 = dot(d, d);
-  const double invr2 = 1.0 / r2;
-  const double invr = sqrt(invr2);
   const double qiqj = k.q[gi] * k.q[gl] * inv_scee;
-  ele = qiqj * invr;
-  double fmag = 0.0;
+  double lja, ljb, sigma_lb = 0.0;
   if (lj_table) {
     const double2 ab = k.ljab14[lj.x + (k.lj_type[gi] * lj.y) + k.lj_type[gl]];
-    const double lja = ab.x * inv_scnb;
-    const double ljb = ab.y * inv_scnb;
+    lja = ab.x * inv_scnb;
+    ljb = ab.y * inv_scnb;
+  }
+  else {
+    sigma_lb = 0.5 * (k.ljp1[gi] + k.ljp1[gl]);
+    const double sig2 = sigma_lb * sigma_lb;
+    const double sig6 = sig2 * sig2 * sig2;
+    ljb = k.ljp2[gi] * k.ljp2[gl] * inv_scnb * sig6;
+    lja = ljb * sig6;
+  }
+  double fmag = 0.0;
+  if (CLASH && clash_on) {
+    ele = 0.0;
+    vdw = 0.0;
+    const double sigma = (ljb > 1.0e-6) ? (lj_table ? sqrt(cbrt(lja / ljb)) : sigma_lb) : 0.0;
+    const double ca = -0.5 / (cd * cd * (cd + 1.0));
+    const double cb = (1.0 / cd) - (ca * (cd + 1.0) * (cd + 1.0));
+    clash_pair<double, FORCES>(sqrt(r2), qiqj, lja, ljb, sigma, cd, ca, cb, cratio, ele, vdw,
+                               fmag);
+  }
+  else {
+    const double invr2 = 1.0 / r2;
+    const double invr = sqrt(invr2);
     const double invr4 = invr2 * invr2;
+    ele = qiqj * invr;
     vdw = (lja * invr4 * invr4 * invr4) - (ljb * invr4 * invr2);
     if (FORCES) {
       fmag = ((6.0 * ljb) - (12.0 * lja * invr4 * invr2)) * invr4 * invr4;
-    }
-  }
-  else {
-    const double sig = 0.5 * (k.ljp1[gi] + k.ljp1[gl]);
-    const double s2 = sig * sig * invr2;
-    const double s6 = s2 * s2 * s2;
-    const double e4 = k.ljp2[gi] * k.ljp2[gl] * inv_scnb;
-    vdw = e4 * ((s6 * s6) - s6);
-    if (FORCES) {
-      fmag = e4 * ((6.0 * s6) - (12.0 * s6 * s6)) * invr2;
+      fmag -= qiqj * invr * invr2;
     }
   }
   if (FORCES) {
-    fmag -= qiqj * invr * invr2;
     const long long ix = fix64(fmag * d.x * fscale);
     const long long iy = fix64(fmag * d.y * fscale);
     const long long iz = fix64(fmag * d.z * fscale);
@@ -993,7 +1130,7 @@ __device__ __forceinline__ void eval_14(const V &v, const MMKit &k, bool lj_tabl
 }
 
 // Valence term t of the work unit (nothing when t is past the unit's last term).
-template <typename T, bool FORCES>
+template <typename T, bool FORCES, bool CLASH>
 __device__ void valence_term(StageCtx<T> &c, const MMKit &k, int t) {
   if (t >= c.wu.term_count) {
     return;
@@ -1031,8 +1168,11 @@ __device__ void valence_term(StageCtx<T> &c, const MMKit &k, int t) {
     const double2 a14 = k.dihe14[vt.w];
     if (a14.x > 0.0) {
       double ele, vdw;
-      eval_14<FORCES>(v, k, c.wu.lj_table != 0, make_int2(ws.lj_off, ws.lj_nt), i, l, a14.x,
-                      a14.y, fscale, ele, vdw);
+      const bool con = CLASH && (c.L.clash_on[slot] != 0);
+      eval_14<FORCES, CLASH>(v, k, c.wu.lj_table != 0, make_int2(ws.lj_off, ws.lj_nt), i, l,
+                             a14.x, a14.y, fscale, con,
+                             con ? static_cast<double>(c.L.clash_d[slot]) : 0.0,
+                             con ? static_cast<double>(c.L.clash_r[slot]) : 0.0, ele, vdw);
       lds_add(&nrg[kElec14], fix64(ele * nrg_scale));
       lds_add(&nrg[kVdw14], fix64(vdw * nrg_scale));
     }
@@ -1040,13 +1180,13 @@ __device__ void valence_term(StageCtx<T> &c, const MMKit &k, int t) {
 }
 
 // Valence terms on their own (evaluations without pair tiles), in chunks of 64 per wavefront.
-template <typename T, bool FORCES>
+template <typename T, bool FORCES, bool CLASH>
 __device__ void stage_valence(StageCtx<T> &c, const MMKit &k) {
   const int lane = c.tid & 63;
   const int chunks = (c.wu.term_count + 63) >> 6;
   for (int item = next_item(&c.L.work[2], lane); item < chunks;
        item = next_item(&c.L.work[2], lane)) {
-    valence_term<T, FORCES>(c, k, (item << 6) + lane);
+    valence_term<T, FORCES, CLASH>(c, k, (item << 6) + lane);
   }
   __syncthreads();
 }
@@ -1054,31 +1194,31 @@ __device__ void stage_valence(StageCtx<T> &c, const MMKit &k) {
 // ------------------------------------------------------------------------------------------
 // One evaluation of the selected terms on the coordinates currently in global memory, leaving
 // forces (FORCES) and energies in the LDS accumulators.
-template <typename T, bool FORCES>
+template <typename T, bool FORCES, bool CLASH>
 __device__ void evaluate_wu(StageCtx<T> &c, const MMKit &k, int terms) {
   const bool lj_table = (c.wu.lj_table != 0);
-  stage_load_coords<T>(c, k);
+  stage_load_coords<T, CLASH>(c, k);
   // With GB the valence terms ride along with the Born-integral tiles.
   const bool val = (terms & kTermValence) != 0;
   if (c.gb) {
-    stage_psi<T, FORCES>(c, k, val ? ((c.wu.term_count + 63) >> 6) : 0);
+    stage_psi<T, FORCES, CLASH>(c, k, val ? ((c.wu.term_count + 63) >> 6) : 0);
     stage_radii<T, FORCES>(c, k, 0.0);
   }
   if (terms & kTermNonbonded) {
     if (lj_table) {
       if (c.gb) {
-        stage_pairs<T, FORCES, true, true>(c, k);
+        stage_pairs<T, FORCES, true, true, CLASH>(c, k);
       }
       else {
-        stage_pairs<T, FORCES, true, false>(c, k);
+        stage_pairs<T, FORCES, true, false, CLASH>(c, k);
       }
     }
     else {
       if (c.gb) {
-        stage_pairs<T, FORCES, false, true>(c, k);
+        stage_pairs<T, FORCES, false, true, CLASH>(c, k);
       }
       else {
-        stage_pairs<T, FORCES, false, false>(c, k);
+        stage_pairs<T, FORCES, false, false, CLASH>(c, k);
       }
     }
   }
@@ -1087,7 +1227,7 @@ __device__ void evaluate_wu(StageCtx<T> &c, const MMKit &k, int terms) {
     stage_deriv<T>(c, k);
   }
   if (val && !c.gb) {
-    stage_valence<T, FORCES>(c, k);
+    stage_valence<T, FORCES, CLASH>(c, k);
   }
 }
 

@@ -6,22 +6,24 @@
 
 namespace stormm_amd {
 
-template <typename T, bool FORCES>
+// CLASH instantiations carry the clash-damped Coulomb / Lennard-Jones / 1:4 potentials used in
+// the first cycles of a minimization; the others never pay for them.
+template <typename T, bool FORCES, bool CLASH>
 __global__ void __launch_bounds__(EvalBlockLimit<T>::value)
     k_mm_eval(MMKit k, int terms, int eval_energy) {
   extern __shared__ __align__(16) unsigned char smem[];
   StageCtx<T> c(smem, k.maxa, static_cast<int>(blockDim.x >> 6));
   init_ctx<T>(c, k, blockIdx.x, terms);
   stage_load_params<T>(c, k);
-  evaluate_wu<T, FORCES>(c, k, terms);
+  evaluate_wu<T, FORCES, CLASH>(c, k, terms);
   stage_store<T, FORCES>(c, k, eval_energy != 0);
 }
 
-template <typename T, bool FORCES>
+template <typename T, bool FORCES, bool CLASH>
 static hipError_t launch_eval_t(const MMKit &k, int terms, int eval_energy, hipStream_t stream) {
   const int block = std::min(k.block, EvalBlockLimit<T>::value);
   const size_t lds = WuLds<T>::bytes(k.maxa, block / 64);
-  auto fn = k_mm_eval<T, FORCES>;
+  auto fn = k_mm_eval<T, FORCES, CLASH>;
   hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        static_cast<int>(lds));
@@ -37,12 +39,21 @@ hipError_t launch_mm_eval(int precision, const MMKit &k, int terms, int eval_for
   if (k.nwu == 0) {
     return hipSuccess;
   }
+  const bool clash = (k.clash_variant != 0);
   if (precision == 0) {
-    return eval_force ? launch_eval_t<double, true>(k, terms, eval_energy, stream) :
-                        launch_eval_t<double, false>(k, terms, eval_energy, stream);
+    if (clash) {
+      return eval_force ? launch_eval_t<double, true, true>(k, terms, eval_energy, stream) :
+                          launch_eval_t<double, false, true>(k, terms, eval_energy, stream);
+    }
+    return eval_force ? launch_eval_t<double, true, false>(k, terms, eval_energy, stream) :
+                        launch_eval_t<double, false, false>(k, terms, eval_energy, stream);
   }
-  return eval_force ? launch_eval_t<float, true>(k, terms, eval_energy, stream) :
-                      launch_eval_t<float, false>(k, terms, eval_energy, stream);
+  if (clash) {
+    return eval_force ? launch_eval_t<float, true, true>(k, terms, eval_energy, stream) :
+                        launch_eval_t<float, false, true>(k, terms, eval_energy, stream);
+  }
+  return eval_force ? launch_eval_t<float, true, false>(k, terms, eval_energy, stream) :
+                      launch_eval_t<float, false, false>(k, terms, eval_energy, stream);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -280,7 +280,14 @@ __global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
     if (all_done) {
       break;
     }
-    evaluate_wu<T, true>(c, k, terms);
+    c.cycle = cyc;
+    const bool clash = (k.clash_variant != 0) && (cyc < k.clash_cycles);
+    if (clash) {
+      evaluate_wu<T, true, true>(c, k, terms);
+    }
+    else {
+      evaluate_wu<T, true, false>(c, k, terms);
+    }
 
     // Convergence test and conjugate-gradient direction, one wave per system.  The three
     // Cartesian components are handled by the same code on their own arrays.
@@ -366,7 +373,12 @@ __global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
     __syncthreads();
     apply_moves<T>(c, k, M);
     for (int move = 1; move < 4; move++) {
-      evaluate_wu<T, false>(c, k, terms);
+      if (clash) {
+        evaluate_wu<T, false, true>(c, k, terms);
+      }
+      else {
+        evaluate_wu<T, false, false>(c, k, terms);
+      }
       for (int s = c.tid; s < nsys; s += c.nthr) {
         if (M->done[s] == 0) {
           LineState ls = M->ls[s];
@@ -383,7 +395,7 @@ __global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
     }
   }
   // Final energies of every system (hpc_minimization.cu:395-405).
-  evaluate_wu<T, false>(c, k, terms);
+  evaluate_wu<T, false, false>(c, k, terms);
   stage_store<T, false>(c, k, true);
   for (int s = c.tid; s < nsys; s += c.nthr) {
     const int si = c.L.sys[s].sys_index;

@@ -110,7 +110,9 @@ static int check_config(const stormm_pool_config *c) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "pool needs a positive budget and slot count");
   }
   if (c->minimize.total_cycles < 0 || c->minimize.steepest_descent_cycles < 0 ||
-      !(c->minimize.initial_step > 0.0)) {
+      !(c->minimize.initial_step > 0.0) || c->minimize.clash_damping_cycles < 0 ||
+      c->minimize.clash_damping_cycles > c->minimize.total_cycles ||
+      c->minimize.clash_minimum_distance < 0.0 || c->minimize.clash_vdw_ratio < 0.0) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "invalid minimization controls");
   }
   return STORMM_OK;
@@ -517,9 +519,11 @@ extern "C" int stormm_pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *
   HIP_TRY(hipMemcpy(done.data(), p->done.ptr, p->cap_slots * sizeof(int), hipMemcpyDeviceToHost));
   const int max_steps = p->cfg.minimize.total_cycles;
   std::vector<int> act;
+  int min_step = max_steps;
   for (int s = 0; s < p->cap_slots; s++) {
     if (p->slot_live[s] && !done[s] && (max_steps == 0 || steps[s] < max_steps)) {
       act.push_back(s);
+      min_step = std::min(min_step, steps[s]);
     }
   }
   if (running) {
@@ -631,15 +635,30 @@ extern "C" int stormm_pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *
   lk.dp_mode = cg.dp_mode;
   const int terms = kTermNonbonded | kTermValence;
   const int sd = p->cfg.minimize.steepest_descent_cycles;
+  // Clash damping follows each system's own cycle count (hpc_minimization.cu:269-281): the
+  // damped kernels read it from the device (before the conjugate-gradient kernel advances it
+  // for the force evaluation, after for the energy evaluations) and give the plain potentials
+  // to systems past their damping cycles.  They run while the least advanced system may still
+  // be damped.
+  const int cdcyc = p->cfg.minimize.clash_damping_cycles;
+  MMKit mf_clash = mm, me_clash = mm;
+  mf_clash.clash_variant = 1;
+  mf_clash.clash_cycles = cdcyc;
+  mf_clash.clash_step = p->steps.as<int>();
+  mf_clash.clash_r0 = p->cfg.minimize.clash_minimum_distance;
+  mf_clash.clash_ratio = p->cfg.minimize.clash_vdw_ratio;
+  me_clash = mf_clash;
+  me_clash.clash_step_offset = -1;
   // The reference's per-cycle sequence (src/MolecularMechanics/hpc_minimization.cu:262-405):
   // force + energy evaluation, conjugate-gradient direction, then four line moves separated by
   // energy evaluations.  Stores overwrite, so no force or ScoreCard zeroing is launched.
   for (int c = 0; c < cycles; c++) {
-    HIP_TRY(launch_mm_eval(prec, mm, terms, 1, 1, st));
+    const bool damp = (min_step + c < cdcyc);
+    HIP_TRY(launch_mm_eval(prec, damp ? mf_clash : mm, terms, 1, 1, st));
     HIP_TRY(launch_conjugate_gradient(cg, 0, sd, st));
     for (int move = 0; move < 4; move++) {
       if (move > 0) {
-        HIP_TRY(launch_mm_eval(prec, mm, terms, 0, 1, st));
+        HIP_TRY(launch_mm_eval(prec, damp ? me_clash : mm, terms, 0, 1, st));
       }
       HIP_TRY(launch_line_advance(lk, move, st));
     }

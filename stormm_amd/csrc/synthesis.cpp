@@ -519,6 +519,9 @@ extern "C" int stormm_mm_controls_create(const stormm_minimize_controls *mincon,
     c->mc.steepest_descent_cycles = 50;
     c->mc.initial_step = 0.01;
     c->mc.convergence_rms_force = 0.0;
+    c->mc.clash_damping_cycles = 25;
+    c->mc.clash_minimum_distance = 0.2;
+    c->mc.clash_vdw_ratio = 0.5;
   }
   *out = c;
   return STORMM_OK;
@@ -605,12 +608,58 @@ static MMKit make_kit(const stormm_synthesis *ag, stormm_phasespace *ps, stormm_
   k.yfrc = ps->yfrc.as<long long>();
   k.zfrc = ps->zfrc.as<long long>();
   k.scorecard = (sc != nullptr) ? sc->inst.as<long long>() : nullptr;
+  k.clash_variant = 0;
+  k.clash_cycles = 0;
+  k.clash_cycle = 0;
+  k.clash_step_offset = 0;
+  k.clash_step = nullptr;
+  k.clash_r0 = 0.0;
+  k.clash_ratio = 0.0;
   return k;
+}
+
+// Whether the damped kernels run for clash parameters (d, r): the reference launches them when
+// either exceeds 10^-6 (src/Potential/hpc_nonbonded_potential.cu:1476, :1662 in single
+// precision).
+static bool clash_active(int32_t precision, double d, double r) {
+  if (precision == STORMM_PREC_DOUBLE) {
+    return d >= 1.0e-6 || r >= 1.0e-6;
+  }
+  return static_cast<float>(d) > 0.000001f || static_cast<float>(r) > 0.000001f;
+}
+
+// Kit fields for one evaluation with fixed clash parameters (both zero: plain potentials).
+static void set_clash(MMKit &k, int32_t precision, double d, double r) {
+  const bool on = clash_active(precision, d, r);
+  k.clash_variant = on ? 1 : 0;
+  k.clash_cycles = 0;
+  k.clash_cycle = 0;
+  k.clash_step = nullptr;
+  k.clash_step_offset = 0;
+  k.clash_r0 = on ? d : 0.0;
+  k.clash_ratio = on ? r : 0.0;
+}
+
+// Validated minimization controls (src/Namelists/nml_minimize.cpp:227-260, DIE policy).
+static int check_mincon(const stormm_minimize_controls *mc) {
+  if (mc->total_cycles < 0) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "negative cycle count");
+  }
+  if (mc->clash_damping_cycles < 0 || mc->clash_damping_cycles > mc->total_cycles) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT,
+                "clash damping cycles (" + std::to_string(mc->clash_damping_cycles) +
+                    ") must lie between 0 and the total cycle count (" +
+                    std::to_string(mc->total_cycles) + ")");
+  }
+  if (mc->clash_minimum_distance < 0.0 || mc->clash_vdw_ratio < 0.0) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "clash parameters must be non-negative");
+  }
+  return STORMM_OK;
 }
 
 static int launch_terms(int32_t precision, const stormm_synthesis *ag, stormm_phasespace *ps,
                         stormm_scorecard *sc, int terms, int32_t eval_force, int32_t eval_energy,
-                        void *stream) {
+                        double clash_d, double clash_r, void *stream) {
   int rc = check_objects(ag, ps, sc);
   if (rc != STORMM_OK) {
     return rc;
@@ -622,7 +671,11 @@ static int launch_terms(int32_t precision, const stormm_synthesis *ag, stormm_ph
   if (sc == nullptr && eval_energy) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "energy evaluation needs a ScoreCard");
   }
-  const MMKit k = make_kit(ag, ps, sc);
+  if (!(clash_d >= 0.0) || !(clash_r >= 0.0)) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "clash parameters must be non-negative");
+  }
+  MMKit k = make_kit(ag, ps, sc);
+  set_clash(k, precision, clash_d, clash_r);
   HIP_TRY(launch_mm_eval(precision == STORMM_PREC_DOUBLE ? 0 : 1, k, terms, eval_force ? 1 : 0,
                          eval_energy ? 1 : 0, static_cast<hipStream_t>(stream)));
   return STORMM_OK;
@@ -631,26 +684,31 @@ static int launch_terms(int32_t precision, const stormm_synthesis *ag, stormm_ph
 extern "C" int stormm_launch_nonbonded(int32_t precision, const stormm_synthesis *ag,
                                        stormm_mm_controls *ctrl, stormm_phasespace *ps,
                                        stormm_scorecard *sc, int32_t eval_force,
-                                       int32_t eval_energy, void *stream) {
+                                       int32_t eval_energy, double clash_minimum_distance,
+                                       double clash_ratio, void *stream) {
   (void)ctrl;
-  return launch_terms(precision, ag, ps, sc, kTermNonbonded, eval_force, eval_energy, stream);
+  return launch_terms(precision, ag, ps, sc, kTermNonbonded, eval_force, eval_energy,
+                      clash_minimum_distance, clash_ratio, stream);
 }
 
 extern "C" int stormm_launch_valence(int32_t precision, const stormm_synthesis *ag,
                                      stormm_mm_controls *ctrl, stormm_phasespace *ps,
                                      stormm_scorecard *sc, int32_t eval_force,
-                                     int32_t eval_energy, void *stream) {
+                                     int32_t eval_energy, double clash_minimum_distance,
+                                     double clash_ratio, void *stream) {
   (void)ctrl;
-  return launch_terms(precision, ag, ps, sc, kTermValence, eval_force, eval_energy, stream);
+  return launch_terms(precision, ag, ps, sc, kTermValence, eval_force, eval_energy,
+                      clash_minimum_distance, clash_ratio, stream);
 }
 
 extern "C" int stormm_launch_mm_evaluation(int32_t precision, const stormm_synthesis *ag,
                                            stormm_mm_controls *ctrl, stormm_phasespace *ps,
                                            stormm_scorecard *sc, int32_t eval_force,
-                                           int32_t eval_energy, void *stream) {
+                                           int32_t eval_energy, double clash_minimum_distance,
+                                           double clash_ratio, void *stream) {
   (void)ctrl;
   return launch_terms(precision, ag, ps, sc, kTermNonbonded | kTermValence, eval_force,
-                      eval_energy, stream);
+                      eval_energy, clash_minimum_distance, clash_ratio, stream);
 }
 
 static CgKit make_cg_kit(const stormm_synthesis *ag, stormm_phasespace *ps, int precision) {
@@ -760,7 +818,16 @@ static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag, stor
   MMKit k = make_kit(ag, ps, sc);
   k.store_overwrite = 1;
   const int terms = kTermNonbonded | kTermValence;
+  const int cdcyc = mc->clash_damping_cycles;
   for (int cyc = 0; cyc < mc->total_cycles; cyc++) {
+    // Clash damping of this cycle (hpc_minimization.cu:269-274) for all four evaluations.
+    double cd = 0.0, cr = 0.0;
+    if (cyc < cdcyc) {
+      const double progress = static_cast<double>(cdcyc - cyc) / static_cast<double>(cdcyc);
+      cd = mc->clash_minimum_distance * progress;
+      cr = mc->clash_vdw_ratio * progress;
+    }
+    set_clash(k, precision, cd, cr);
     HIP_TRY(launch_mm_eval(prec, k, terms, 1, 1, stream));
     ctrl.step += 1;
     if ((rc = stormm_launch_conjugate_gradient(precision, ag, ps, &ctrl, stream)) != STORMM_OK) {
@@ -776,6 +843,7 @@ static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag, stor
       }
     }
   }
+  set_clash(k, precision, 0.0, 0.0);
   HIP_TRY(launch_mm_eval(prec, k, terms, 0, 1, stream));
   return STORMM_OK;
 }
@@ -783,7 +851,39 @@ static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag, stor
 // The per-cycle launch sequence captured once in a HIP graph and replayed for every cycle.  The
 // conjugate-gradient kernel reads each system's cycle count from device memory (so one captured
 // cycle serves them all) and applies the RMS-force stop of the persistent minimizer; evaluations
-// overwrite, so the graph holds kernels only.
+// overwrite, so the graph holds kernels only.  With clash damping a second graph, whose
+// evaluations take each system's damping schedule from the same cycle counts, serves the first
+// clash_damping_cycles cycles.
+static hipError_t capture_cycle(int prec, const MMKit &kf, const MMKit &ke, const CgKit &cg,
+                                const LineKit &lk, int sd, hipStream_t cap,
+                                hipGraph_t *graph, hipGraphExec_t *exec) {
+  const int terms = kTermNonbonded | kTermValence;
+  hipError_t err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (err != hipSuccess) {
+    return err;
+  }
+  hipError_t e2 = launch_mm_eval(prec, kf, terms, 1, 1, cap);
+  if (e2 == hipSuccess) {
+    e2 = launch_conjugate_gradient(cg, 0, sd, cap);
+  }
+  for (int move = 0; move < 4 && e2 == hipSuccess; move++) {
+    if (move > 0) {
+      e2 = launch_mm_eval(prec, ke, terms, 0, 1, cap);
+    }
+    if (e2 == hipSuccess) {
+      e2 = launch_line_advance(lk, move, cap);
+    }
+  }
+  err = hipStreamEndCapture(cap, graph);
+  if (err == hipSuccess) {
+    err = e2;
+  }
+  if (err == hipSuccess) {
+    err = hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0);
+  }
+  return err;
+}
+
 static int minimize_graph(int32_t precision, const stormm_synthesis *ag_c, stormm_phasespace *ps,
                           const stormm_minimize_controls *mc, stormm_scorecard *sc,
                           stormm_line_min *lm, hipStream_t stream) {
@@ -799,6 +899,7 @@ static int minimize_graph(int32_t precision, const stormm_synthesis *ag_c, storm
   const int prec = (precision == STORMM_PREC_DOUBLE) ? 0 : 1;
   MMKit k = make_kit(ag, ps, sc);
   k.store_overwrite = 1;
+  set_clash(k, precision, 0.0, 0.0);
   CgKit cg = make_cg_kit(ag, ps, precision);
   cg.steps = ag->cycles_run.as<int>();
   cg.done = ag->stop_flags.as<int>();
@@ -809,34 +910,28 @@ static int minimize_graph(int32_t precision, const stormm_synthesis *ag_c, storm
   lk.done = ag->stop_flags.as<int>();
   const int terms = kTermNonbonded | kTermValence;
   const int sd = mc->steepest_descent_cycles;
+  const int cdcyc = std::min(mc->clash_damping_cycles, mc->total_cycles);
+  // Damped evaluations: the force evaluation reads the cycle count before the conjugate-gradient
+  // kernel advances it, the three energy evaluations after.
+  MMKit kf_clash = k, ke_clash = k;
+  kf_clash.clash_variant = 1;
+  kf_clash.clash_cycles = cdcyc;
+  kf_clash.clash_step = ag->cycles_run.as<int>();
+  kf_clash.clash_step_offset = 0;
+  kf_clash.clash_r0 = mc->clash_minimum_distance;
+  kf_clash.clash_ratio = mc->clash_vdw_ratio;
+  ke_clash = kf_clash;
+  ke_clash.clash_step_offset = -1;
   hipStream_t cap = nullptr;
   HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  hipError_t err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-  if (err == hipSuccess) {
-    hipError_t e2 = launch_mm_eval(prec, k, terms, 1, 1, cap);
-    if (e2 == hipSuccess) {
-      e2 = launch_conjugate_gradient(cg, 0, sd, cap);
-    }
-    for (int move = 0; move < 4 && e2 == hipSuccess; move++) {
-      if (move > 0) {
-        e2 = launch_mm_eval(prec, k, terms, 0, 1, cap);
-      }
-      if (e2 == hipSuccess) {
-        e2 = launch_line_advance(lk, move, cap);
-      }
-    }
-    err = hipStreamEndCapture(cap, &graph);
-    if (err == hipSuccess) {
-      err = e2;
-    }
-  }
-  if (err == hipSuccess) {
-    err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraph_t graph = nullptr, graph_clash = nullptr;
+  hipGraphExec_t exec = nullptr, exec_clash = nullptr;
+  hipError_t err = capture_cycle(prec, k, k, cg, lk, sd, cap, &graph, &exec);
+  if (err == hipSuccess && cdcyc > 0) {
+    err = capture_cycle(prec, kf_clash, ke_clash, cg, lk, sd, cap, &graph_clash, &exec_clash);
   }
   for (int c = 0; c < mc->total_cycles && err == hipSuccess; c++) {
-    err = hipGraphLaunch(exec, stream);
+    err = hipGraphLaunch((c < cdcyc) ? exec_clash : exec, stream);
   }
   if (err == hipSuccess) {
     err = launch_mm_eval(prec, k, terms, 0, 1, stream);
@@ -844,11 +939,15 @@ static int minimize_graph(int32_t precision, const stormm_synthesis *ag_c, storm
   if (err == hipSuccess) {
     err = hipStreamSynchronize(stream);
   }
-  if (exec != nullptr) {
-    (void)hipGraphExecDestroy(exec);
+  for (hipGraphExec_t e : {exec, exec_clash}) {
+    if (e != nullptr) {
+      (void)hipGraphExecDestroy(e);
+    }
   }
-  if (graph != nullptr) {
-    (void)hipGraphDestroy(graph);
+  for (hipGraph_t g : {graph, graph_clash}) {
+    if (g != nullptr) {
+      (void)hipGraphDestroy(g);
+    }
   }
   (void)hipStreamDestroy(cap);
   HIP_TRY(err);
@@ -868,8 +967,8 @@ extern "C" int stormm_launch_minimization(int32_t precision, const stormm_synthe
     return fail(STORMM_ERR_INVALID_ARGUMENT, "minimization needs controls, a ScoreCard and a "
                                              "matching LineMinimization record");
   }
-  if (mincon->total_cycles < 0) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "negative cycle count");
+  if ((rc = check_mincon(mincon)) != STORMM_OK) {
+    return rc;
   }
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (persistent == 0) {
@@ -880,6 +979,15 @@ extern "C" int stormm_launch_minimization(int32_t precision, const stormm_synthe
   }
   MinKit k;
   k.mm = make_kit(ag, ps, sc);
+  // The persistent kernel switches to its damped evaluations for cycles below clash_cycles and
+  // takes the schedule from its own cycle counter.
+  set_clash(k.mm, precision, 0.0, 0.0);
+  if (mincon->clash_damping_cycles > 0) {
+    k.mm.clash_variant = 1;
+    k.mm.clash_cycles = mincon->clash_damping_cycles;
+    k.mm.clash_r0 = mincon->clash_minimum_distance;
+    k.mm.clash_ratio = mincon->clash_vdw_ratio;
+  }
   k.xprv = ps->xprv.as<long long>();
   k.yprv = ps->yprv.as<long long>();
   k.zprv = ps->zprv.as<long long>();

@@ -28,7 +28,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _lib
-from ._lib import MinimizeControls, PoolConfig, StormmError, check
+from ._lib import PoolConfig, StormmError, check
 from .synthesis import _BatchArrays, _shape, precision_code
 from .topology import IGB_OBC2, Topology
 
@@ -40,7 +40,10 @@ class MinimizationPool:
                  max_systems: int = 16384, total_cycles: int = 300, sd_cycles: int = 50,
                  dx0: float = 0.01, conv_rms: float = 0.1, igb: int = IGB_OBC2,
                  dielectric: float = 78.5, salt: float = 0.0, coulomb: float = 332.0522172900,
-                 shape=None, bits: Optional[Tuple[int, int, int]] = None):
+                 shape=None, bits: Optional[Tuple[int, int, int]] = None,
+                 clash_cycles: Optional[int] = None, clash_r0: float = 0.2,
+                 clash_vdw_ratio: float = 0.5):
+        from . import synthesis as S
         self._L = _lib.load()
         if bits is None:
             bits = (32, 23, 26) if precision == "single" else (32, 40, 32)
@@ -53,8 +56,9 @@ class MinimizationPool:
         cfg.dielectric = dielectric
         cfg.salt_concentration = salt
         cfg.coulomb_constant = coulomb
-        cfg.minimize = MinimizeControls(int(total_cycles), int(sd_cycles), float(dx0),
-                                        float(conv_rms))
+        cfg.minimize = S.minimize_controls(int(total_cycles), int(sd_cycles), float(dx0),
+                                           float(conv_rms), clash_cycles, clash_r0,
+                                           clash_vdw_ratio)
         sh = _shape(shape)
         if sh is not None:
             cfg.shape = sh

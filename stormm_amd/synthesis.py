@@ -348,13 +348,19 @@ class ScoreCard:
 
 
 def minimize_controls(total_cycles: int = 200, sd_cycles: int = 50, dx0: float = 0.01,
-                      rms_force: float = 0.0) -> MinimizeControls:
-    """Defaults of the reference's &minimize namelist (src/Namelists/nml_minimize.h:21-30)."""
+                      rms_force: float = 0.0, clash_cycles: Optional[int] = None,
+                      clash_r0: float = 0.2, clash_vdw_ratio: float = 0.5) -> MinimizeControls:
+    """Defaults of the reference's &minimize namelist (src/Namelists/nml_minimize.h:21-30):
+    maxcyc 200, ncyc 50, cdcyc 25 (capped at maxcyc), dx0 0.01, clash_r0 0.2,
+    clash_vdw_ratio 0.5."""
     m = MinimizeControls()
     m.total_cycles = total_cycles
     m.steepest_descent_cycles = sd_cycles
     m.initial_step = dx0
     m.convergence_rms_force = rms_force
+    m.clash_damping_cycles = min(25, total_cycles) if clash_cycles is None else clash_cycles
+    m.clash_minimum_distance = clash_r0
+    m.clash_vdw_ratio = clash_vdw_ratio
     return m
 
 
@@ -421,23 +427,28 @@ def _h(obj):
     return None if obj is None else obj.handle
 
 
-def launch_nonbonded(prec, ag, ctrl, ps, sc, eval_force=True, eval_energy=True, stream=None):
+def launch_nonbonded(prec, ag, ctrl, ps, sc, eval_force=True, eval_energy=True, stream=None,
+                     clash_distance=0.0, clash_ratio=0.0):
     check(_lib.load().stormm_launch_nonbonded(precision_code(prec), ag.handle, _h(ctrl),
                                               ps.handle, _h(sc), int(eval_force),
-                                              int(eval_energy), _stream_ptr(stream)))
+                                              int(eval_energy), float(clash_distance),
+                                              float(clash_ratio), _stream_ptr(stream)))
 
 
-def launch_valence(prec, ag, ctrl, ps, sc, eval_force=True, eval_energy=True, stream=None):
+def launch_valence(prec, ag, ctrl, ps, sc, eval_force=True, eval_energy=True, stream=None,
+                   clash_distance=0.0, clash_ratio=0.0):
     check(_lib.load().stormm_launch_valence(precision_code(prec), ag.handle, _h(ctrl),
                                             ps.handle, _h(sc), int(eval_force),
-                                            int(eval_energy), _stream_ptr(stream)))
+                                            int(eval_energy), float(clash_distance),
+                                            float(clash_ratio), _stream_ptr(stream)))
 
 
 def launch_mm_evaluation(prec, ag, ctrl, ps, sc, eval_force=True, eval_energy=True,
-                         stream=None):
+                         stream=None, clash_distance=0.0, clash_ratio=0.0):
     check(_lib.load().stormm_launch_mm_evaluation(precision_code(prec), ag.handle, _h(ctrl),
                                                   ps.handle, _h(sc), int(eval_force),
-                                                  int(eval_energy), _stream_ptr(stream)))
+                                                  int(eval_energy), float(clash_distance),
+                                                  float(clash_ratio), _stream_ptr(stream)))
 
 
 def launch_conjugate_gradient(prec, ag, ps, ctrl, stream=None):

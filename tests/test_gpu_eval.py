@@ -113,6 +113,56 @@ def test_separate_launches_equal_fused_launch(prec):
     assert all(np.array_equal(a, b) for a, b in zip(fused["Ffix"], parts["Ffix"]))
 
 
+def compressed(coords, factor):
+    """Coordinates pulled toward each molecule's centre so that many pairs clash."""
+    out = []
+    for x in coords:
+        x = np.asarray(x, dtype=np.float64)
+        c = x.mean(axis=0)
+        out.append(c + factor * (x - c))
+    return out
+
+
+@pytest.mark.parametrize("prec", ["double", "single"])
+def test_clash_damped_evaluation_matches_oracle(prec):
+    """launchNonbonded / launchValence with clash_minimum_distance and clash_ratio
+    (src/Potential/hpc_nonbonded_potential.h:131-157) against the oracle's damped potentials, on
+    molecules squeezed to 55 % of their size, for combining-rule and stored-table LJ."""
+    topos, coords = synthetic.config1(nsys=12, seed=4)
+    trp, xt = golden_system("trpcage", igb=IGB_OBC2)
+    topos, coords = topos + [trp], coords + [xt]
+    coords = compressed(coords, 0.55)
+    m = MODES[prec]
+    d, ratio = 0.2, 0.5
+
+    def fused(p, ag, ctrl, ps, sc, ef, ee):
+        S.launch_mm_evaluation(p, ag, ctrl, ps, sc, ef, ee, clash_distance=d, clash_ratio=ratio)
+
+    def split(p, ag, ctrl, ps, sc, ef, ee):
+        S.launch_nonbonded(p, ag, ctrl, ps, sc, ef, ee, clash_distance=d, clash_ratio=ratio)
+        S.launch_valence(p, ag, ctrl, ps, sc, ef, ee, clash_distance=d, clash_ratio=ratio)
+
+    res = evaluate(topos, coords, prec, launch=fused)
+    parts = evaluate(topos, coords, prec, launch=split)
+    assert np.array_equal(res["E"], parts["E"])
+    assert all(np.array_equal(a, b) for a, b in zip(res["Ffix"], parts["Ffix"]))
+    plain = evaluate(topos, coords, prec)
+    worst = 0.0
+    for s, t in enumerate(topos):
+        nrg, _, f = oracle.eval_mm(t, res["X"][s], True, m["nrg"], clash_distance=d,
+                                   clash_ratio=ratio)
+        eo = np.array([nrg[k] for k in oracle.SLOT_NAMES])
+        eg = res["E"][s].astype(np.float64) * 2.0 ** -m["nrg"]
+        scale = np.abs(eo).sum()
+        assert np.abs(eg - eo).max() <= m["e_tol"] * scale, (s, np.abs(eg - eo).max() / scale)
+        rel = np.sqrt(((res["F"][s] - f) ** 2).sum() / (f ** 2).sum())
+        assert rel <= m["f_tol"], (s, rel)
+        worst = max(worst, rel)
+        # The damping changed the clashing molecule's Lennard-Jones energy.
+        assert plain["E"][s][7] != res["E"][s][7]
+    print(f"[{prec}] clash-damped evaluation: worst force relative RMS {worst:.2e}")
+
+
 @pytest.mark.parametrize("prec", ["double", "single"])
 def test_nonbonded_and_valence_launches_match_their_terms(prec):
     topos, coords = synthetic.config1(nsys=10, seed=3)

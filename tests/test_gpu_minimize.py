@@ -67,12 +67,13 @@ def test_graph_replay_equals_per_step_and_persistent(prec):
 def test_double_precision_minimizer_tracks_the_oracle():
     topos, coords = synthetic.config2(nsys=6, seed=13)
     gpos, frc, nrg = MODES["double"]
-    for cycles in (1, 4, 12):
+    # The reference's default schedule damps clashes for min(25, cycles) cycles.
+    for cycles in (1, 4, 12, 30):
         res = minimize(topos, coords, "double", cycles, persistent=True)
         for s, t in enumerate(topos):
             fixed = np.rint(np.asarray(coords[s]) * 2.0 ** gpos).astype(np.int64)
             out, trace = oracle.minimize(t, fixed, gpos, frc, nrg, cycles, 50, 0.01,
-                                         dp_mode=True)
+                                         dp_mode=True, clash_cycles=min(25, cycles))
             drift = np.abs(res["X"][s] - out).max() * 2.0 ** -gpos
             assert drift < 1e-6, (cycles, s, drift)
             assert res["E"][s] == pytest.approx(trace[-1], rel=1e-8, abs=1e-6)

@@ -229,7 +229,10 @@ __device__ double wu_energy(const StageCtx<T> &c, int s) {
   return static_cast<double>(acc);
 }
 
-template <typename T>
+// CLASH instantiations serve minimizations with clash-damping cycles: every evaluation takes
+// each system's damping from the cycle counter (off once the damped cycles are over, which gives
+// the plain potentials), so one inlined evaluation of each kind serves the whole run.
+template <typename T, bool CLASH>
 __global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
   extern __shared__ __align__(16) unsigned char smem[];
   const MMKit &k = mk.mm;
@@ -281,13 +284,7 @@ __global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
       break;
     }
     c.cycle = cyc;
-    const bool clash = (k.clash_variant != 0) && (cyc < k.clash_cycles);
-    if (clash) {
-      evaluate_wu<T, true, true>(c, k, terms);
-    }
-    else {
-      evaluate_wu<T, true, false>(c, k, terms);
-    }
+    evaluate_wu<T, true, CLASH>(c, k, terms);
 
     // Convergence test and conjugate-gradient direction, one wave per system.  The three
     // Cartesian components are handled by the same code on their own arrays.
@@ -373,12 +370,7 @@ __global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
     __syncthreads();
     apply_moves<T>(c, k, M);
     for (int move = 1; move < 4; move++) {
-      if (clash) {
-        evaluate_wu<T, false, true>(c, k, terms);
-      }
-      else {
-        evaluate_wu<T, false, false>(c, k, terms);
-      }
+      evaluate_wu<T, false, CLASH>(c, k, terms);
       for (int s = c.tid; s < nsys; s += c.nthr) {
         if (M->done[s] == 0) {
           LineState ls = M->ls[s];
@@ -394,8 +386,9 @@ __global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
       apply_moves<T>(c, k, M);
     }
   }
-  // Final energies of every system (hpc_minimization.cu:395-405).
-  evaluate_wu<T, false, false>(c, k, terms);
+  // Final energies of every system (hpc_minimization.cu:395-405), undamped.
+  c.cycle = k.clash_cycles;
+  evaluate_wu<T, false, CLASH>(c, k, terms);
   stage_store<T, false>(c, k, true);
   for (int s = c.tid; s < nsys; s += c.nthr) {
     const int si = c.L.sys[s].sys_index;
@@ -415,7 +408,7 @@ template <typename T>
 static hipError_t launch_min_t(const MinKit &k, hipStream_t stream) {
   const int block = std::min(k.mm.block, kMaxMinimizeBlock);
   const size_t lds = WuLds<T>::bytes(k.mm.maxa, block / 64) + sizeof(MinLds);
-  auto fn = k_minimize<T>;
+  auto fn = (k.mm.clash_variant != 0) ? k_minimize<T, true> : k_minimize<T, false>;
   hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        static_cast<int>(lds));

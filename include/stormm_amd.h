@@ -68,6 +68,28 @@ extern "C" {
 #define STORMM_NRG_POTENTIAL_ENERGY 28
 #define STORMM_SCORECARD_STRIDE 32
 
+/* Flat-bottom restraints: one record holds the fields of RestraintApparatus / RestraintKit
+ * (src/Restraints/restraint_apparatus.h) for one restraint.  Positional restraints hold atom 0
+ * to a target point, distance restraints atoms 0-1, angle restraints 0-1-2 and dihedral
+ * restraints 0-1-2-3 (atom indices local to the system).  k[0], k[1] are the stiffnesses k2, k3
+ * and r[0..3] the bounds r1 <= r2 <= r3 <= r4 (A, or radians for angles and dihedrals; dihedral
+ * bounds span at most 2 pi): the penalty is quadratic between r1 and r2 and between r3 and r4,
+ * zero between r2 and r3, linear outside.  Parameters slide linearly from init_* to final_*
+ * between init_step and final_step of the controls' step counter; before init_step the
+ * restraint is off (src/Restraints/restraint_util.tpp). */
+#define STORMM_RESTRAINT_POSITION 1
+#define STORMM_RESTRAINT_DISTANCE 2
+#define STORMM_RESTRAINT_ANGLE 3
+#define STORMM_RESTRAINT_DIHEDRAL 4
+typedef struct {
+  int32_t kind;
+  int32_t atoms[4];
+  int32_t init_step, final_step;
+  double init_k[2], final_k[2];
+  double init_r[4], final_r[4];
+  double init_target[3], final_target[3];   /* positional restraints (A)                       */
+} stormm_restraint;
+
 /* Many molecules' parameters, concatenated system after system.  Atom indices inside the term
  * arrays are local to their system.  This is the input the reference assembles from a list of
  * AtomGraph objects in the AtomGraphSynthesis constructor
@@ -108,6 +130,10 @@ typedef struct {
   const int32_t *lj_combining;     /* [nsys] or null                                            */
   const double *lj_sigma;          /* A                                                         */
   const double *lj_epsilon;        /* kcal/mol                                                  */
+  /* Restraints of each system, the reference's per-system RestraintApparatus list handed to the
+   * synthesis; they feed the RESTRAINT ScoreCard slot.  Null restraint_counts: none. */
+  const int32_t *restraint_counts; /* [nsys] or null                                            */
+  const stormm_restraint *restraints; /* [sum counts], system after system                      */
 } stormm_topology_batch;
 
 /* Work-unit shaping, chosen by the autotuner (or fixed by the caller).  Zero = default. */
@@ -268,7 +294,8 @@ int stormm_launch_line_advance(int32_t precision, const stormm_synthesis *ag,
  * the systems' state held in LDS; persistent == 0 issues the reference's per-step launch
  * sequence; persistent == 2 captures that sequence for one cycle in a HIP graph and replays it
  * for every cycle (returns when done).  Modes 1 and 2 stop a system once its RMS force reaches
- * convergence_rms_force (if > 0).  Final energies land in the ScoreCard. */
+ * convergence_rms_force (if > 0).  Final energies land in the ScoreCard.  A synthesis with
+ * restraints is minimized in modes 0 and 2; mode 1 returns STORMM_ERR_STATE for it. */
 int stormm_launch_minimization(int32_t precision, const stormm_synthesis *ag,
                                stormm_phasespace *ps, const stormm_minimize_controls *mincon,
                                stormm_scorecard *sc, stormm_line_min *lm, int32_t persistent,

@@ -62,7 +62,19 @@ class OrcSystem(ctypes.Structure):
         ("pb_radii", ctypes.c_void_p), ("gb_screen", ctypes.c_void_p),
         ("gb_alpha", ctypes.c_void_p), ("gb_beta", ctypes.c_void_p),
         ("gb_gamma", ctypes.c_void_p),
+        ("nrst", ctypes.c_int), ("rst", ctypes.c_void_p),
     ]
+
+
+# orc_restraint (stormm_oracle.h): kind, atoms[4], init_step, final_step, k[2] x 2, r[4] x 2,
+# target[3] x 2.
+ORC_RESTRAINT = np.dtype([
+    ("kind", np.int32), ("atoms", np.int32, (4,)),
+    ("init_step", np.int32), ("final_step", np.int32),
+    ("init_k", np.float64, (2,)), ("final_k", np.float64, (2,)),
+    ("init_r", np.float64, (4,)), ("final_r", np.float64, (4,)),
+    ("init_target", np.float64, (3,)), ("final_target", np.float64, (3,)),
+], align=True)
 
 
 class OrcEnergy(ctypes.Structure):
@@ -86,6 +98,10 @@ def lib():
                                         ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_double, ctypes.POINTER(OrcEnergy)]
         L.orc_eval_mm_clash.restype = None
+        L.orc_eval_mm_ext.argtypes = [ctypes.POINTER(OrcSystem), vp, vp, vp, vp, vp, vp,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_int, ctypes.POINTER(OrcEnergy)]
+        L.orc_eval_mm_ext.restype = None
         L.orc_eval_terms.argtypes = [ctypes.POINTER(OrcSystem), vp, vp, vp, vp, vp, vp,
                                      ctypes.c_uint, ctypes.c_int, ctypes.POINTER(OrcEnergy)]
         L.orc_eval_terms.restype = None
@@ -112,9 +128,10 @@ def _ptr(a: np.ndarray) -> int:
 
 
 class SystemHandle:
-    """Keeps the numpy arrays behind one orc_system alive."""
+    """Keeps the numpy arrays behind one orc_system alive.  `restraints` is a record array with
+    the fields of ORC_RESTRAINT (atom indices local to the system) or None."""
 
-    def __init__(self, topo):
+    def __init__(self, topo, restraints=None):
         n = topo.natom
         keep: List[np.ndarray] = []
 
@@ -175,6 +192,16 @@ class SystemHandle:
         s.gb_alpha = _ptr(f64(a))
         s.gb_beta = _ptr(f64(b))
         s.gb_gamma = _ptr(f64(g))
+        s.nrst = 0
+        s.rst = None
+        if restraints is not None and len(restraints):
+            src = np.asarray(restraints).reshape(-1)
+            rst = np.zeros(len(src), dtype=ORC_RESTRAINT)
+            for name in ORC_RESTRAINT.names:
+                rst[name] = src[name]
+            keep.append(rst)
+            s.nrst = len(rst)
+            s.rst = _ptr(rst)
         self.struct = s
         self.keep = keep
         self.natom = n
@@ -182,10 +209,12 @@ class SystemHandle:
 
 def eval_mm(topo, xyz: np.ndarray, eval_force: bool = True, nrg_bits: int = 26,
             handle: Optional[SystemHandle] = None, clash_distance: float = 0.0,
-            clash_ratio: float = 0.0) -> Tuple[Dict[str, float], Dict[str, int], np.ndarray]:
+            clash_ratio: float = 0.0, restraints=None,
+            step: int = 0) -> Tuple[Dict[str, float], Dict[str, int], np.ndarray]:
     """One FP64 evaluation; returns (double energies, fixed-point energies, forces (N, 3)).
-    A positive clash_distance or clash_ratio softens Coulomb, Lennard-Jones and 1:4 pairs."""
-    h = handle or SystemHandle(topo)
+    A positive clash_distance or clash_ratio softens Coulomb, Lennard-Jones and 1:4 pairs;
+    restraints (or those of `handle`) are mixed at step number `step`."""
+    h = handle or SystemHandle(topo, restraints)
     xyz = np.asarray(xyz, dtype=np.float64)
     x = np.ascontiguousarray(xyz[:, 0])
     y = np.ascontiguousarray(xyz[:, 1])
@@ -193,9 +222,9 @@ def eval_mm(topo, xyz: np.ndarray, eval_force: bool = True, nrg_bits: int = 26,
     n = topo.natom
     fx, fy, fz = np.zeros(n), np.zeros(n), np.zeros(n)
     e = OrcEnergy()
-    lib().orc_eval_mm_clash(ctypes.byref(h.struct), _ptr(x), _ptr(y), _ptr(z), _ptr(fx),
-                            _ptr(fy), _ptr(fz), int(eval_force), nrg_bits, float(clash_distance),
-                            float(clash_ratio), ctypes.byref(e))
+    lib().orc_eval_mm_ext(ctypes.byref(h.struct), _ptr(x), _ptr(y), _ptr(z), _ptr(fx),
+                          _ptr(fy), _ptr(fz), int(eval_force), nrg_bits, float(clash_distance),
+                          float(clash_ratio), int(step), ctypes.byref(e))
     nrg = {name: e.nrg[k] for k, name in enumerate(SLOT_NAMES)}
     acc = {name: e.acc[k] for k, name in enumerate(SLOT_NAMES)}
     return nrg, acc, np.stack([fx, fy, fz], axis=1)
@@ -261,11 +290,11 @@ def cubic_move(evec: Sequence[float], mvec: Sequence[float]):
 def minimize(topo, xyz_fixed: np.ndarray, gpos_bits: int, frc_bits: int, nrg_bits: int,
              nsteps: int, sd_cycles: int = 50, dx0: float = 0.01, dp_mode: bool = False,
              handle: Optional[SystemHandle] = None, clash_cycles: int = 0,
-             clash_r0: float = 0.2, clash_ratio: float = 0.5):
+             clash_r0: float = 0.2, clash_ratio: float = 0.5, restraints=None):
     """GPU-semantics minimization on fixed-point coordinates; returns (coords, energy trace).
     The first clash_cycles cycles run with clash damping shrinking from (clash_r0,
-    clash_ratio) to zero."""
-    h = handle or SystemHandle(topo)
+    clash_ratio) to zero; restraints follow the minimizer's step counters."""
+    h = handle or SystemHandle(topo, restraints)
     c = np.array(xyz_fixed, dtype=np.int64).reshape(-1, 3)
     x = np.ascontiguousarray(c[:, 0])
     y = np.ascontiguousarray(c[:, 1])

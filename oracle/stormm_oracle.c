@@ -851,6 +851,212 @@ void orc_eval_terms(const orc_system *s, const double *x, const double *y, const
   }
 }
 
+/* ----------------------------------------------------------------------------------------- */
+/* Flat-bottom restraints, src/Restraints/restraint_util.tpp:9-77 and                          */
+/* src/Potential/valence_potential.tpp:1555-2160 (non-periodic: no imaging of displacements).   */
+/* ----------------------------------------------------------------------------------------- */
+/* Weights of the initial and final parameters at step_number (restraint_util.tpp:47-77). */
+static void restraint_mixture(int step, int init_step, int final_step, double w[2]) {
+  if (step < init_step) {
+    w[0] = 0.0; w[1] = 0.0;
+  }
+  else if (init_step == final_step) {
+    w[0] = 1.0; w[1] = 0.0;
+  }
+  else if (step < final_step) {
+    const double wslide = (double)(step - init_step) / (double)(final_step - init_step);
+    w[0] = 1.0 - wslide; w[1] = wslide;
+  }
+  else {
+    w[0] = 0.0; w[1] = 1.0;
+  }
+}
+
+/* Stiffness, displacement beyond the flat bottom and energy at coordinate value dr
+ * (restraint_util.tpp:8-45): quadratic between r1 and r2 and between r3 and r4, flat between r2
+ * and r3, linear beyond r1 and r4. */
+static void restraint_delta(const orc_restraint *t, const double w[2], double dr, double *keq,
+                            double *dl, double *du) {
+  const double r1 = (w[0] * t->init_r[0]) + (w[1] * t->final_r[0]);
+  const double r2 = (w[0] * t->init_r[1]) + (w[1] * t->final_r[1]);
+  const double r3 = (w[0] * t->init_r[2]) + (w[1] * t->final_r[2]);
+  const double r4 = (w[0] * t->init_r[3]) + (w[1] * t->final_r[3]);
+  const double k2 = (w[0] * t->init_k[0]) + (w[1] * t->final_k[0]);
+  const double k3 = (w[0] * t->init_k[1]) + (w[1] * t->final_k[1]);
+  if (dr < r1) {
+    *dl = r1 - r2;
+    *du = k2 * ((*dl * *dl) + (2.0 * *dl * (dr - r1)));
+    *keq = k2;
+  }
+  else if (dr < r2) {
+    *dl = dr - r2;
+    *du = k2 * *dl * *dl;
+    *keq = k2;
+  }
+  else if (dr < r3) {
+    *dl = 0.0;
+    *du = 0.0;
+    *keq = 0.0;
+  }
+  else if (dr < r4) {
+    *dl = dr - r3;
+    *du = k3 * *dl * *dl;
+    *keq = k3;
+  }
+  else {
+    *dl = r4 - r3;
+    *du = k3 * ((*dl * *dl) + (2.0 * *dl * (dr - r4)));
+    *keq = k3;
+  }
+}
+
+/* Minimum image of an angle over 2 pi (src/Structure/local_arrangement.tpp:42-58). */
+static double image_angle(double x) {
+  const double range = 2.0 * PI_LF;
+  double xf = x / range;
+  xf -= ((xf >= 0.5) ? ceil(xf - 0.5) : 0.0) + ((xf < -0.5) ? floor(xf + 0.5) : 0.0);
+  return (xf - ((xf >= 0.5) ? 1.0 : 0.0)) * range;
+}
+
+static double eval_restraint(const orc_restraint *t, int step, const double *x, const double *y,
+                             const double *z, double *fx, double *fy, double *fz,
+                             int eval_force) {
+  double w[2];
+  restraint_mixture(step, t->init_step, t->final_step, w);
+  double keq, dl, du;
+  const int *a = t->atoms;
+  if (t->kind == ORC_RST_POSITION) {
+    /* valence_potential.tpp:1555-1656 */
+    const int p = a[0];
+    const double dx = x[p] - ((w[0] * t->init_target[0]) + (w[1] * t->final_target[0]));
+    const double dy = y[p] - ((w[0] * t->init_target[1]) + (w[1] * t->final_target[1]));
+    const double dz = z[p] - ((w[0] * t->init_target[2]) + (w[1] * t->final_target[2]));
+    const double dr = sqrt((dx * dx) + (dy * dy) + (dz * dz));
+    restraint_delta(t, w, dr, &keq, &dl, &du);
+    if (eval_force) {
+      if (dr < 1.0e-10) {
+        const double fmag = 2.0 * keq * dl / sqrt(3.0);
+        fx[p] -= fmag; fy[p] -= fmag; fz[p] -= fmag;
+      }
+      else {
+        const double fmag = 2.0 * keq * dl / dr;
+        fx[p] -= fmag * dx; fy[p] -= fmag * dy; fz[p] -= fmag * dz;
+      }
+    }
+    return du;
+  }
+  if (t->kind == ORC_RST_DISTANCE) {
+    /* valence_potential.tpp:1658-1739 */
+    const int i = a[0], j = a[1];
+    const double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
+    const double dr = sqrt((dx * dx) + (dy * dy) + (dz * dz));
+    restraint_delta(t, w, dr, &keq, &dl, &du);
+    if (eval_force) {
+      const double fmag = 2.0 * keq * dl / dr;
+      fx[i] += fmag * dx; fy[i] += fmag * dy; fz[i] += fmag * dz;
+      fx[j] -= fmag * dx; fy[j] -= fmag * dy; fz[j] -= fmag * dz;
+    }
+    return du;
+  }
+  if (t->kind == ORC_RST_ANGLE) {
+    /* valence_potential.tpp:1741-1864 */
+    const int i = a[0], j = a[1], k = a[2];
+    const double ba[3] = {x[i] - x[j], y[i] - y[j], z[i] - z[j]};
+    const double bc[3] = {x[k] - x[j], y[k] - y[j], z[k] - z[j]};
+    const double mgba = (ba[0] * ba[0]) + (ba[1] * ba[1]) + (ba[2] * ba[2]);
+    const double mgbc = (bc[0] * bc[0]) + (bc[1] * bc[1]) + (bc[2] * bc[2]);
+    const double invbabc = 1.0 / sqrt(mgba * mgbc);
+    double costheta = ((ba[0] * bc[0]) + (ba[1] * bc[1]) + (ba[2] * bc[2])) * invbabc;
+    costheta = (costheta < -1.0) ? -1.0 : (costheta > 1.0) ? 1.0 : costheta;
+    const double theta = acos(costheta);
+    restraint_delta(t, w, theta, &keq, &dl, &du);
+    if (eval_force) {
+      const double dA = -2.0 * keq * dl / sqrt(1.0 - (costheta * costheta));
+      const double sqba = dA / mgba, sqbc = dA / mgbc, mbabc = dA * invbabc;
+      double adf[3], cdf[3];
+      for (int m = 0; m < 3; m++) {
+        adf[m] = (bc[m] * mbabc) - (costheta * ba[m] * sqba);
+        cdf[m] = (ba[m] * mbabc) - (costheta * bc[m] * sqbc);
+      }
+      fx[i] -= adf[0]; fy[i] -= adf[1]; fz[i] -= adf[2];
+      fx[j] += adf[0] + cdf[0]; fy[j] += adf[1] + cdf[1]; fz[j] += adf[2] + cdf[2];
+      fx[k] -= cdf[0]; fy[k] -= cdf[1]; fz[k] -= cdf[2];
+    }
+    return du;
+  }
+  /* Dihedral, valence_potential.tpp:1866-2088. */
+  const int i = a[0], j = a[1], k = a[2], l = a[3];
+  const double ab[3] = {x[j] - x[i], y[j] - y[i], z[j] - z[i]};
+  const double bc[3] = {x[k] - x[j], y[k] - y[j], z[k] - z[j]};
+  const double cd[3] = {x[l] - x[k], y[l] - y[k], z[l] - z[k]};
+  double crabbc[3], crbccd[3], scr[3];
+  cross3(ab, bc, crabbc);
+  cross3(bc, cd, crbccd);
+  double costheta = (crabbc[0] * crbccd[0]) + (crabbc[1] * crbccd[1]) + (crabbc[2] * crbccd[2]);
+  costheta /= sqrt(((crabbc[0] * crabbc[0]) + (crabbc[1] * crabbc[1]) + (crabbc[2] * crabbc[2])) *
+                   ((crbccd[0] * crbccd[0]) + (crbccd[1] * crbccd[1]) + (crbccd[2] * crbccd[2])));
+  cross3(crabbc, crbccd, scr);
+  costheta = (costheta < -1.0) ? -1.0 : (costheta > 1.0) ? 1.0 : costheta;
+  double theta = ((scr[0] * bc[0]) + (scr[1] * bc[1]) + (scr[2] * bc[2]) > 0.0) ? acos(costheta)
+                                                                               : -acos(costheta);
+  const double midpoint = 0.5 * ((w[0] * (t->init_r[1] + t->init_r[2])) +
+                                 (w[1] * (t->final_r[1] + t->final_r[2])));
+  const double midpoint_delta = image_angle(theta - midpoint);
+  theta += midpoint_delta - (theta - midpoint);
+  restraint_delta(t, w, theta, &keq, &dl, &du);
+  if (eval_force) {
+    const double fr = -2.0 * keq * dl;
+    const double mgab = sqrt((ab[0] * ab[0]) + (ab[1] * ab[1]) + (ab[2] * ab[2]));
+    const double mgbc = sqrt((bc[0] * bc[0]) + (bc[1] * bc[1]) + (bc[2] * bc[2]));
+    const double mgcd = sqrt((cd[0] * cd[0]) + (cd[1] * cd[1]) + (cd[2] * cd[2]));
+    const double invab = 1.0 / mgab, invbc = 1.0 / mgbc, invcd = 1.0 / mgcd;
+    const double cosb = -((ab[0] * bc[0]) + (ab[1] * bc[1]) + (ab[2] * bc[2])) * invab * invbc;
+    const double cosc = -((bc[0] * cd[0]) + (bc[1] * cd[1]) + (bc[2] * cd[2])) * invbc * invcd;
+    const double isinb2 = (cosb * cosb < ASYMPTOTIC_TO_ONE_LF) ? fr / (1.0 - (cosb * cosb))
+                                                               : fr * INV_ONE_MINUS_ASYMPTOTE_LF;
+    const double isinc2 = (cosc * cosc < ASYMPTOTIC_TO_ONE_LF) ? fr / (1.0 - (cosc * cosc))
+                                                               : fr * INV_ONE_MINUS_ASYMPTOTE_LF;
+    const double invabc = invab * invbc, invbcd = invbc * invcd;
+    for (int m = 0; m < 3; m++) {
+      crabbc[m] *= invabc;
+      crbccd[m] *= invbcd;
+    }
+    const double fa = -invab * isinb2;
+    const double fb1 = (mgbc - (mgab * cosb)) * invabc * isinb2;
+    const double fb2 = cosc * invbc * isinc2;
+    const double fd = -invcd * isinc2;
+    double *f[3] = {fx, fy, fz};
+    for (int m = 0; m < 3; m++) {
+      const double fi = crabbc[m] * fa;
+      const double fj = (fb1 * crabbc[m]) - (fb2 * crbccd[m]);
+      const double fl = -fd * crbccd[m];
+      f[m][i] += fi;
+      f[m][j] += fj;
+      f[m][k] -= fi + fj + fl;
+      f[m][l] += fl;
+    }
+  }
+  return du;
+}
+
+void orc_eval_mm_ext(const orc_system *s, const double *x, const double *y, const double *z,
+                     double *fx, double *fy, double *fz, int eval_force, int nrg_bits,
+                     double clash_distance, double clash_ratio, int step_number, orc_energy *e) {
+  orc_eval_mm_clash(s, x, y, z, fx, fy, fz, eval_force, nrg_bits, clash_distance, clash_ratio,
+                    e);
+  /* evaluateRestraints, valence_potential.tpp:2090-2160 */
+  const double nrg_scale = ldexp(1.0, nrg_bits);
+  double sum = 0.0;
+  long long acc = 0;
+  for (int r = 0; r < s->nrst; r++) {
+    const double du = eval_restraint(&s->rst[r], step_number, x, y, z, fx, fy, fz, eval_force);
+    sum += du;
+    acc += llround(du * nrg_scale);
+  }
+  e->nrg[ORC_RESTRAINT] += sum;
+  e->acc[ORC_RESTRAINT] += acc;
+}
+
 void orc_eval_mm_clash(const orc_system *s, const double *x, const double *y, const double *z,
                        double *fx, double *fy, double *fz, int eval_force, int nrg_bits,
                        double clash_distance, double clash_ratio, orc_energy *e) {
@@ -866,7 +1072,7 @@ void orc_eval_mm_clash(const orc_system *s, const double *x, const double *y, co
 void orc_eval_mm(const orc_system *s, const double *x, const double *y, const double *z,
                  double *fx, double *fy, double *fz, int eval_force, int nrg_bits,
                  orc_energy *e) {
-  orc_eval_mm_clash(s, x, y, z, fx, fy, fz, eval_force, nrg_bits, 0.0, 0.0, e);
+  orc_eval_mm_ext(s, x, y, z, fx, fy, fz, eval_force, nrg_bits, 0.0, 0.0, 0, e);
 }
 
 /* ----------------------------------------------------------------------------------------- */
@@ -1104,8 +1310,9 @@ static double line_energy(const orc_energy *e, int nrg_bits) {
 
 static void eval_fixed(const orc_system *s, const long long *xc, const long long *yc,
                        const long long *zc, int gpos_bits, int frc_bits, int nrg_bits,
-                       int eval_force, double clash_distance, double clash_ratio, long long *ifx,
-                       long long *ify, long long *ifz, double *work, orc_energy *e) {
+                       int eval_force, double clash_distance, double clash_ratio, int step,
+                       long long *ifx, long long *ify, long long *ifz, double *work,
+                       orc_energy *e) {
   const int n = s->natom;
   double *x = work, *y = work + n, *z = work + 2 * n;
   double *fx = work + 3 * n, *fy = work + 4 * n, *fz = work + 5 * n;
@@ -1116,8 +1323,8 @@ static void eval_fixed(const orc_system *s, const long long *xc, const long long
     z[i] = (double)zc[i] * inv_gpos;
     fx[i] = fy[i] = fz[i] = 0.0;
   }
-  orc_eval_mm_clash(s, x, y, z, fx, fy, fz, eval_force, nrg_bits, clash_distance, clash_ratio,
-                    e);
+  orc_eval_mm_ext(s, x, y, z, fx, fy, fz, eval_force, nrg_bits, clash_distance, clash_ratio,
+                  step, e);
   if (eval_force) {
     const double fscale = ldexp(1.0, frc_bits);
     for (int i = 0; i < n; i++) {
@@ -1146,6 +1353,19 @@ static void clash_schedule(int i, int clash_cycles, double clash_r0, double clas
   *r_out = on ? r : 0.0;
 }
 
+/* Step number seen by the restraints in evaluation `move` (0: the force evaluation, 1..3: the
+ * energy evaluations) of cycle i.  hpc_minimization.cu:262-405 keeps four controls objects,
+ * each with its own step counter: the force-evaluation controls count every cycle (:280, :301)
+ * and the energy-evaluation controls, damped (:318) or plain (:327), count their own launches,
+ * so the damped and the plain energy evaluations each start from 0. */
+static int restraint_step(int i, int move, int clash_cycles) {
+  if (move == 0) {
+    return i;
+  }
+  const int j = (i < clash_cycles) ? i : i - clash_cycles;
+  return (3 * j) + move - 1;
+}
+
 void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long long *zcrd,
                   int gpos_bits, int frc_bits, int nrg_bits, int nsteps, int sd_cycles,
                   double dx0, int clash_cycles, double clash_r0, double clash_ratio,
@@ -1172,8 +1392,8 @@ void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long lo
     double cd, cr;
     clash_schedule(it, clash_cycles, clash_r0, clash_ratio, dp_mode, &cd, &cr);
     /* Force + energy evaluation (hpc_minimization.cu:264-287). */
-    eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 1, cd, cr, fx, fy, fz, work,
-               &e);
+    eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 1, cd, cr,
+               restraint_step(it, 0, clash_cycles), fx, fy, fz, work, &e);
     step_counter += 1;
 
     /* Conjugate gradient all-reduce (conjugate_gradient.cui:259-370, hpc_reduction.cu:39-244). */
@@ -1222,8 +1442,8 @@ void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long lo
       if (move > 0) {
         long long dummy = 0;
         (void)dummy;
-        eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 0, cd, cr, NULL, NULL,
-                   NULL, work, &e);
+        eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 0, cd, cr,
+                   restraint_step(it, move, clash_cycles), NULL, NULL, NULL, work, &e);
       }
       const double cur = line_energy(&e, nrg_bits);
       if (move == 0 && energy_trace != NULL) {
@@ -1266,8 +1486,10 @@ void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long lo
     }
   }
   if (energy_trace != NULL) {
-    eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 0, 0.0, 0.0, NULL, NULL, NULL,
-               work, &e);
+    /* The final evaluation is the plain energy controls' next launch (:395-405). */
+    eval_fixed(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, 0, 0.0, 0.0,
+               restraint_step(nsteps, 1, clash_cycles < nsteps ? clash_cycles : nsteps), NULL,
+               NULL, NULL, work, &e);
     energy_trace[nsteps] = line_energy(&e, nrg_bits);
   }
   (void)nrg_d;

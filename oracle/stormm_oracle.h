@@ -36,6 +36,21 @@ enum {
 /* Implicit solvent models (AMBER igb numbering). */
 enum { ORC_IGB_NONE = 0, ORC_IGB_HCT = 1, ORC_IGB_OBC = 2, ORC_IGB_OBC2 = 5 };
 
+/* One flat-bottom restraint, the fields of RestraintApparatus / RestraintKit
+ * (src/Restraints/restraint_apparatus.h): kind 1 positional (atoms[0] to a target point),
+ * 2 distance (atoms 0-1), 3 angle (0-1-2), 4 dihedral (0-1-2-3).  k[0..1] are k2 and k3, r[0..3]
+ * are r1..r4 (A or radians); values slide from init_* to final_* between the two steps. */
+typedef struct {
+  int kind;
+  int atoms[4];
+  int init_step, final_step;
+  double init_k[2], final_k[2];
+  double init_r[4], final_r[4];
+  double init_target[3], final_target[3];
+} orc_restraint;
+
+enum { ORC_RST_POSITION = 1, ORC_RST_DISTANCE = 2, ORC_RST_ANGLE = 3, ORC_RST_DIHEDRAL = 4 };
+
 typedef struct {
   int natom;
   double coulomb;                 /* Coulomb constant (kcal/mol A / e^2)                        */
@@ -63,6 +78,8 @@ typedef struct {
   int igb;
   double dielectric, kappa, gb_offset;
   const double *pb_radii, *gb_screen, *gb_alpha, *gb_beta, *gb_gamma;
+  int nrst;                       /* restraints (may be 0)                                      */
+  const orc_restraint *rst;
 } orc_system;
 
 /* Energies of one evaluation: plain double sums and the fixed-precision accumulators that the
@@ -102,6 +119,12 @@ void orc_eval_mm_clash(const orc_system *s, const double *x, const double *y, co
                        double *fx, double *fy, double *fz, int eval_force, int nrg_bits,
                        double clash_distance, double clash_ratio, orc_energy *e);
 
+/* orc_eval_mm_clash plus the system's restraints evaluated at step_number
+ * (src/Potential/valence_potential.tpp:2090-2160, src/Restraints/restraint_util.tpp). */
+void orc_eval_mm_ext(const orc_system *s, const double *x, const double *y, const double *z,
+                     double *fx, double *fy, double *fz, int eval_force, int nrg_bits,
+                     double clash_distance, double clash_ratio, int step_number, orc_energy *e);
+
 /* Evaluate all energies, accumulating forces only from the terms named in force_mask (the
  * per-term force flags of the reference's evaluators, e.g. nonbonded_potential.tpp:13-14). */
 void orc_eval_terms(const orc_system *s, const double *x, const double *y, const double *z,
@@ -129,7 +152,9 @@ double orc_cubic_move(const double evec[4], const double mvec[4], double *new_ba
  * gpos_bits; forces are fixed-precision at frc_bits.  dp_mode selects the double-precision
  * model's rounding of the normalized move (doubleToInt95) instead of the single-precision
  * model's truncation.  The first clash_cycles cycles evaluate with clash damping whose floor
- * (clash_r0) and ratio shrink linearly to zero (hpc_minimization.cu:269-274).  energy_trace
+ * (clash_r0) and ratio shrink linearly to zero (hpc_minimization.cu:269-274).  Restraints see
+ * the reference's step counters: the force evaluation of cycle i step i, its three energy
+ * evaluations steps 3i, 3i + 1, 3i + 2, the final evaluation step 3 nsteps.  energy_trace
  * receives the energy (kcal/mol) at the start of each step and, in its last slot, the final
  * energy (length nsteps + 1); may be NULL. */
 void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long long *zcrd,

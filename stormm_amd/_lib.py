@@ -46,7 +46,7 @@ class TopologyBatch(ctypes.Structure):
         ("dihe_scee", vp), ("dihe_scnb", vp), ("exclusion_counts", vp), ("exclusion_pairs", vp),
         ("igb", c_int32), ("dielectric", c_double), ("salt_concentration", c_double),
         ("coulomb_constant", c_double), ("lj_combining", vp), ("lj_sigma", vp),
-        ("lj_epsilon", vp),
+        ("lj_epsilon", vp), ("restraint_counts", vp), ("restraints", vp),
     ]
 
 

@@ -6,6 +6,8 @@
 
 #include <cstdint>
 
+#include "../../include/stormm_amd.h"
+
 namespace stormm_amd {
 
 // Tile edge of the all-pairs work: one 16-lane DPP row of a wave64 owns a 16 x 16 tile of
@@ -43,7 +45,7 @@ constexpr int kStageCounters = 3;  // work-item counters of the psi, derivative 
 // StateVariable slots, src/Potential/energy_enumerators.h:36-75.
 enum NrgSlot : int {
   kBond = 0, kAngle = 1, kProperDihedral = 2, kImproperDihedral = 3, kVdw = 7, kVdw14 = 8,
-  kElec = 9, kElec14 = 10, kGb = 11
+  kElec = 9, kElec14 = 10, kGb = 11, kRestraint = 12
 };
 
 // Term selection for the fused kernel.
@@ -63,7 +65,7 @@ struct WuDesc {
   int lds_atoms;     // atoms staged in LDS (each system padded to kTile)
   int term_count;    // valence terms of all systems
   int lj_table;      // 1 = systems in this unit read their stored LJ tables
-  int pad;
+  int rst_count;     // restraints of all systems
 };
 
 // One system inside a work unit and the ranges of its topology template: the template's tile
@@ -81,7 +83,8 @@ struct WuSys {
   int term_count;
   int lj_off;        // stored LJ table (offset, type count)
   int lj_nt;
-  int pad;
+  int rst_first;     // the system's restraints in the synthesis restraint list
+  int rst_count;
 };
 
 // One 16 x 16 tile task of a template: x = first atom of the i tile | (first atom of the j tile
@@ -140,7 +143,28 @@ struct MMKit {
   int clash_cycles, clash_cycle, clash_step_offset;
   const int *clash_step;
   double clash_r0, clash_ratio;
+  // Restraints (src/Potential/valence_potential.tpp:2090-2160) and the step number their
+  // parameters are mixed at: rst_step, or, when rst_step_sys holds each system's cycle count,
+  // minimizer_step(rst_step_sys[system] + rst_cycle_offset, rst_move, rst_clash_cycles).
+  const stormm_restraint *rst;
+  int rst_step;
+  const int *rst_step_sys;
+  int rst_cycle_offset, rst_move, rst_clash_cycles;
 };
+
+// The step counter a restraint sees in the reference's minimizer
+// (src/MolecularMechanics/hpc_minimization.cu:262-405).  Each of its four evaluation controls
+// counts its own launches: the force evaluation of cycle i runs at step i; the move-th energy
+// evaluation (move 1..3) runs at 3 j + move - 1, where j counts the earlier cycles of the same
+// kind (clash-damped cycles below clash_cycles, plain cycles after); the final energy evaluation
+// after `total` cycles is the plain controls' next launch, minimizer_step(total, 1, clash_cycles).
+__host__ __device__ inline int minimizer_step(int cycle, int move, int clash_cycles) {
+  if (move == 0) {
+    return cycle;
+  }
+  const int j = (cycle < clash_cycles) ? cycle : cycle - clash_cycles;
+  return (3 * j) + move - 1;
+}
 
 // Status bits reported by kernels.
 enum StatusBit : int { kStatusCoordinateRange = 1, kStatusNonFinite = 2 };

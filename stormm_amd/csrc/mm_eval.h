@@ -22,6 +22,7 @@ template <typename T> struct StageCtx {
   int crd_shift;     // right shift from global fixed precision to LDS coordinates
   T crd_scale;       // LDS coordinate unit in Angstroms (single precision)
   int cycle;         // minimization cycle of this evaluation (clash-damping schedule)
+  int rst_step;      // restraint step number when the kit gives no per-system cycle counts
   bool gb;
   __device__ StageCtx(unsigned char *smem, int maxa, int waves) : L(smem, maxa, waves) {}
 };
@@ -160,6 +161,7 @@ __device__ void init_ctx(StageCtx<T> &c, const MMKit &k, int wu_index, int terms
   }
   c.gb = (k.igb != 0) && ((terms & kTermNonbonded) != 0);
   c.cycle = k.clash_cycle;
+  c.rst_step = k.rst_step;
 }
 
 // Clash-damping parameters of one system for this evaluation
@@ -942,6 +944,13 @@ template <typename C> struct ValenceCtxT {
   const int *gidx;
   double crd_scale;
   long long *fx, *fy, *fz;
+  // Absolute position (A) of LDS atom a of a system whose coordinate origin is org (fixed
+  // precision at inv_gpos).
+  __device__ __forceinline__ Vec3 absolute(int a, const long long *org, double inv_gpos) const {
+    return {(static_cast<double>(org[0]) * inv_gpos) + (static_cast<double>(cx[a]) * crd_scale),
+            (static_cast<double>(org[1]) * inv_gpos) + (static_cast<double>(cy[a]) * crd_scale),
+            (static_cast<double>(org[2]) * inv_gpos) + (static_cast<double>(cz[a]) * crd_scale)};
+  }
   __device__ __forceinline__ Vec3 disp(int from, int to) const {
     return {(static_cast<double>(cx[to]) - static_cast<double>(cx[from])) * crd_scale,
             (static_cast<double>(cy[to]) - static_cast<double>(cy[from])) * crd_scale,
@@ -1129,6 +1138,186 @@ __device__ __forceinline__ void eval_14(const V &v, const MMKit &k, bool lj_tabl
   }
 }
 
+// ------------------------------------------------------------------------------------------
+// Flat-bottom restraints, src/Restraints/restraint_util.tpp:8-77 and
+// src/Potential/valence_potential.tpp:1555-2088 (non-periodic systems: no imaging), in double
+// precision like the other valence terms.
+// ------------------------------------------------------------------------------------------
+// Weights of a restraint's initial and final parameters at `step`.
+__device__ __forceinline__ double2 restraint_mixture(int step, int init_step, int final_step) {
+  if (step < init_step) {
+    return make_double2(0.0, 0.0);
+  }
+  if (init_step == final_step) {
+    return make_double2(1.0, 0.0);
+  }
+  if (step < final_step) {
+    const double wslide = static_cast<double>(step - init_step) /
+                          static_cast<double>(final_step - init_step);
+    return make_double2(1.0 - wslide, wslide);
+  }
+  return make_double2(0.0, 1.0);
+}
+
+// Stiffness, displacement beyond the flat bottom and penalty at coordinate value dr.
+__device__ __forceinline__ double3 restraint_delta(const stormm_restraint &t, double2 w,
+                                                   double dr) {
+  const double r1 = (w.x * t.init_r[0]) + (w.y * t.final_r[0]);
+  const double r2 = (w.x * t.init_r[1]) + (w.y * t.final_r[1]);
+  const double r3 = (w.x * t.init_r[2]) + (w.y * t.final_r[2]);
+  const double r4 = (w.x * t.init_r[3]) + (w.y * t.final_r[3]);
+  const double k2 = (w.x * t.init_k[0]) + (w.y * t.final_k[0]);
+  const double k3 = (w.x * t.init_k[1]) + (w.y * t.final_k[1]);
+  if (dr < r1) {
+    const double dl = r1 - r2;
+    return make_double3(k2, dl, k2 * ((dl * dl) + (2.0 * dl * (dr - r1))));
+  }
+  if (dr < r2) {
+    const double dl = dr - r2;
+    return make_double3(k2, dl, k2 * dl * dl);
+  }
+  if (dr < r3) {
+    return make_double3(0.0, 0.0, 0.0);
+  }
+  if (dr < r4) {
+    const double dl = dr - r3;
+    return make_double3(k3, dl, k3 * dl * dl);
+  }
+  const double dl = r4 - r3;
+  return make_double3(k3, dl, k3 * ((dl * dl) + (2.0 * dl * (dr - r4))));
+}
+
+// Minimum image of an angle over 2 pi, src/Structure/local_arrangement.tpp:42-58.
+__device__ __forceinline__ double image_angle(double x) {
+  constexpr double kTwoPi = 6.283185307179586476925286766559;
+  double xf = x / kTwoPi;
+  xf -= ((xf >= 0.5) ? ceil(xf - 0.5) : 0.0) + ((xf < -0.5) ? floor(xf + 0.5) : 0.0);
+  return (xf - ((xf >= 0.5) ? 1.0 : 0.0)) * kTwoPi;
+}
+
+// One restraint of the system at LDS base `base`; returns its penalty.
+template <bool FORCES, typename V>
+__device__ double eval_restraint(const V &v, const stormm_restraint &t, int base,
+                                 const long long *org, double inv_gpos, int step,
+                                 double fscale) {
+  const double2 w = restraint_mixture(step, t.init_step, t.final_step);
+  const int i = base + t.atoms[0], j = base + t.atoms[1];
+  const int kk = base + t.atoms[2], l = base + t.atoms[3];
+  if (t.kind == STORMM_RESTRAINT_POSITION) {
+    const Vec3 p = v.absolute(i, org, inv_gpos);
+    const Vec3 d = {p.x - ((w.x * t.init_target[0]) + (w.y * t.final_target[0])),
+                    p.y - ((w.x * t.init_target[1]) + (w.y * t.final_target[1])),
+                    p.z - ((w.x * t.init_target[2]) + (w.y * t.final_target[2]))};
+    const double dr = sqrt(dot(d, d));
+    const double3 e = restraint_delta(t, w, dr);
+    if (FORCES) {
+      if (dr < 1.0e-10) {
+        // On the target: split the force among the three axes (valence_potential.tpp:1610-1626).
+        const long long f = fix64(2.0 * e.x * e.y / sqrt(3.0) * fscale);
+        v.add(i, -f, -f, -f);
+      }
+      else {
+        const double fmag = 2.0 * e.x * e.y / dr;
+        v.add(i, -fix64(fmag * d.x * fscale), -fix64(fmag * d.y * fscale),
+              -fix64(fmag * d.z * fscale));
+      }
+    }
+    return e.z;
+  }
+  if (t.kind == STORMM_RESTRAINT_DISTANCE) {
+    const Vec3 d = v.disp(i, j);
+    const double dr = sqrt(dot(d, d));
+    const double3 e = restraint_delta(t, w, dr);
+    if (FORCES) {
+      const double fmag = 2.0 * e.x * e.y / dr;
+      const long long fx = fix64(fmag * d.x * fscale);
+      const long long fy = fix64(fmag * d.y * fscale);
+      const long long fz = fix64(fmag * d.z * fscale);
+      v.add(i, fx, fy, fz);
+      v.add(j, -fx, -fy, -fz);
+    }
+    return e.z;
+  }
+  if (t.kind == STORMM_RESTRAINT_ANGLE) {
+    const Vec3 ba = v.disp(j, i);
+    const Vec3 bc = v.disp(j, kk);
+    const double mgba = dot(ba, ba);
+    const double mgbc = dot(bc, bc);
+    const double invbabc = 1.0 / sqrt(mgba * mgbc);
+    double costheta = dot(ba, bc) * invbabc;
+    costheta = (costheta < -1.0) ? -1.0 : ((costheta > 1.0) ? 1.0 : costheta);
+    const double3 e = restraint_delta(t, w, acos(costheta));
+    if (FORCES) {
+      const double dA = -2.0 * e.x * e.y / sqrt(1.0 - (costheta * costheta));
+      const double sqba = dA / mgba;
+      const double sqbc = dA / mgbc;
+      const double mbabc = dA * invbabc;
+      const long long iax = fix64(((bc.x * mbabc) - (costheta * ba.x * sqba)) * fscale);
+      const long long iay = fix64(((bc.y * mbabc) - (costheta * ba.y * sqba)) * fscale);
+      const long long iaz = fix64(((bc.z * mbabc) - (costheta * ba.z * sqba)) * fscale);
+      const long long icx = fix64(((ba.x * mbabc) - (costheta * bc.x * sqbc)) * fscale);
+      const long long icy = fix64(((ba.y * mbabc) - (costheta * bc.y * sqbc)) * fscale);
+      const long long icz = fix64(((ba.z * mbabc) - (costheta * bc.z * sqbc)) * fscale);
+      v.add(i, -iax, -iay, -iaz);
+      v.add(j, iax + icx, iay + icy, iaz + icz);
+      v.add(kk, -icx, -icy, -icz);
+    }
+    return e.z;
+  }
+  // Dihedral: the signed angle from the plane normals, imaged to the restraint's centre.
+  const Vec3 ab = v.disp(i, j);
+  const Vec3 bc = v.disp(j, kk);
+  const Vec3 cd = v.disp(kk, l);
+  Vec3 crabbc = cross(ab, bc);
+  Vec3 crbccd = cross(bc, cd);
+  double costheta = dot(crabbc, crbccd) / sqrt(dot(crabbc, crabbc) * dot(crbccd, crbccd));
+  const Vec3 scr = cross(crabbc, crbccd);
+  costheta = (costheta < -1.0) ? -1.0 : ((costheta > 1.0) ? 1.0 : costheta);
+  double theta = (dot(scr, bc) > 0.0) ? acos(costheta) : -acos(costheta);
+  const double midpoint = 0.5 * ((w.x * (t.init_r[1] + t.init_r[2])) +
+                                 (w.y * (t.final_r[1] + t.final_r[2])));
+  theta += image_angle(theta - midpoint) - (theta - midpoint);
+  const double3 e = restraint_delta(t, w, theta);
+  if (FORCES) {
+    const double fr = -2.0 * e.x * e.y;
+    const double mgab = sqrt(dot(ab, ab));
+    const double mgbc = sqrt(dot(bc, bc));
+    const double mgcd = sqrt(dot(cd, cd));
+    const double invab = 1.0 / mgab;
+    const double invbc = 1.0 / mgbc;
+    const double invcd = 1.0 / mgcd;
+    const double cosb = -dot(ab, bc) * invab * invbc;
+    const double cosc = -dot(bc, cd) * invbc * invcd;
+    // src/Constants/symbol_values.h:72-82 (asymptotic_to_one_lf, inverse_one_minus_asymptote)
+    const double isinb2 = (cosb * cosb < 0.99999904632568359375) ? fr / (1.0 - (cosb * cosb)) :
+                                                                   fr * 1048576.0;
+    const double isinc2 = (cosc * cosc < 0.99999904632568359375) ? fr / (1.0 - (cosc * cosc)) :
+                                                                   fr * 1048576.0;
+    const double invabc = invab * invbc;
+    const double invbcd = invbc * invcd;
+    crabbc = {crabbc.x * invabc, crabbc.y * invabc, crabbc.z * invabc};
+    crbccd = {crbccd.x * invbcd, crbccd.y * invbcd, crbccd.z * invbcd};
+    const double fa = -invab * isinb2;
+    const double fb1 = (mgbc - (mgab * cosb)) * invabc * isinb2;
+    const double fb2 = cosc * invbc * isinc2;
+    const double fd = -invcd * isinc2;
+    const long long fix = fix64(crabbc.x * fa * fscale);
+    const long long fiy = fix64(crabbc.y * fa * fscale);
+    const long long fiz = fix64(crabbc.z * fa * fscale);
+    const long long flx = fix64(-fd * crbccd.x * fscale);
+    const long long fly = fix64(-fd * crbccd.y * fscale);
+    const long long flz = fix64(-fd * crbccd.z * fscale);
+    const long long fjx = fix64(((fb1 * crabbc.x) - (fb2 * crbccd.x)) * fscale);
+    const long long fjy = fix64(((fb1 * crabbc.y) - (fb2 * crbccd.y)) * fscale);
+    const long long fjz = fix64(((fb1 * crabbc.z) - (fb2 * crbccd.z)) * fscale);
+    v.add(i, fix, fiy, fiz);
+    v.add(l, flx, fly, flz);
+    v.add(j, fjx, fjy, fjz);
+    v.add(kk, -(fix + fjx + flx), -(fiy + fjy + fly), -(fiz + fjz + flz));
+  }
+  return e.z;
+}
+
 // Valence term t of the work unit (nothing when t is past the unit's last term).
 template <typename T, bool FORCES, bool CLASH>
 __device__ void valence_term(StageCtx<T> &c, const MMKit &k, int t) {
@@ -1191,10 +1380,45 @@ __device__ void stage_valence(StageCtx<T> &c, const MMKit &k) {
   __syncthreads();
 }
 
+// Restraints of the unit's systems, one thread each, in a stage of their own so that their
+// double-precision code stays out of the register budget of the tile loops.
+template <typename T, bool FORCES>
+__device__ void stage_restraints(StageCtx<T> &c, const MMKit &k) {
+  const double fscale = ldexp(1.0, k.frc_bits);
+  const double nrg_scale = ldexp(1.0, k.nrg_bits);
+  const double inv_gpos = ldexp(1.0, -k.gpos_bits);
+  ValenceCtxT<typename Prec<T>::crd_t> v;
+  v.cx = c.L.cx;
+  v.cy = c.L.cy;
+  v.cz = c.L.cz;
+  v.gidx = c.L.gidx;
+  v.crd_scale = static_cast<double>(c.crd_scale);
+  v.fx = c.L.fx;
+  v.fy = c.L.fy;
+  v.fz = c.L.fz;
+  for (int r = c.tid; r < c.wu.rst_count; r += c.nthr) {
+    int slot = 0, u = r;
+    while (u >= c.L.sys[slot].rst_count) {
+      u -= c.L.sys[slot].rst_count;
+      slot++;
+    }
+    const WuSys &ws = c.L.sys[slot];
+    const int step = (k.rst_step_sys != nullptr) ?
+                         minimizer_step(k.rst_step_sys[ws.sys_index] + k.rst_cycle_offset,
+                                        k.rst_move, k.rst_clash_cycles) :
+                         c.rst_step;
+    const double e = eval_restraint<FORCES>(v, k.rst[ws.rst_first + u], ws.lds_base,
+                                            &c.L.org[3 * slot], inv_gpos, step, fscale);
+    lds_add(&c.L.nrg[slot * 16 + kRestraint], fix64(e * nrg_scale));
+  }
+  __syncthreads();
+}
+
 // ------------------------------------------------------------------------------------------
 // One evaluation of the selected terms on the coordinates currently in global memory, leaving
-// forces (FORCES) and energies in the LDS accumulators.
-template <typename T, bool FORCES, bool CLASH>
+// forces (FORCES) and energies in the LDS accumulators.  RESTRAINTS = false leaves the restraint
+// stage out of the instantiation (for callers that never see restrained work units).
+template <typename T, bool FORCES, bool CLASH, bool RESTRAINTS = true>
 __device__ void evaluate_wu(StageCtx<T> &c, const MMKit &k, int terms) {
   const bool lj_table = (c.wu.lj_table != 0);
   stage_load_coords<T, CLASH>(c, k);
@@ -1228,6 +1452,9 @@ __device__ void evaluate_wu(StageCtx<T> &c, const MMKit &k, int terms) {
   }
   if (val && !c.gb) {
     stage_valence<T, FORCES, CLASH>(c, k);
+  }
+  if (RESTRAINTS && val && c.wu.rst_count > 0) {
+    stage_restraints<T, FORCES>(c, k);
   }
 }
 

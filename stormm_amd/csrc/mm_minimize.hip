@@ -186,7 +186,9 @@ namespace stormm_amd {
 // moves with three energy-only evaluations, exactly the sequence of the reference's host loop
 // (src/MolecularMechanics/hpc_minimization.cu:262-405) without returning to the host.
 // Coordinates stay in the global fixed-precision arrays (read back into LDS at each evaluation);
-// the normalized move direction is kept in the global force arrays as the reference does.
+// the normalized move direction is kept in the global force arrays as the reference does.  The
+// host launches it only for syntheses without restraints, so its evaluations leave the restraint
+// stage out.
 // ------------------------------------------------------------------------------------------
 struct MinLds {
   LineState ls[kMaxWuSystems];
@@ -284,7 +286,7 @@ __global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
       break;
     }
     c.cycle = cyc;
-    evaluate_wu<T, true, CLASH>(c, k, terms);
+    evaluate_wu<T, true, CLASH, false>(c, k, terms);
 
     // Convergence test and conjugate-gradient direction, one wave per system.  The three
     // Cartesian components are handled by the same code on their own arrays.
@@ -370,7 +372,7 @@ __global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
     __syncthreads();
     apply_moves<T>(c, k, M);
     for (int move = 1; move < 4; move++) {
-      evaluate_wu<T, false, CLASH>(c, k, terms);
+      evaluate_wu<T, false, CLASH, false>(c, k, terms);
       for (int s = c.tid; s < nsys; s += c.nthr) {
         if (M->done[s] == 0) {
           LineState ls = M->ls[s];
@@ -388,7 +390,7 @@ __global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
   }
   // Final energies of every system (hpc_minimization.cu:395-405), undamped.
   c.cycle = k.clash_cycles;
-  evaluate_wu<T, false, CLASH>(c, k, terms);
+  evaluate_wu<T, false, CLASH, false>(c, k, terms);
   stage_store<T, false>(c, k, true);
   for (int s = c.tid; s < nsys; s += c.nthr) {
     const int si = c.L.sys[s].sys_index;

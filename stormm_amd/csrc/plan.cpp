@@ -8,6 +8,8 @@
 
 namespace stormm_amd {
 
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
 // Per-type sigma and epsilon when the table follows Lorentz-Berthelot combining.
 static bool lorentz_berthelot(int t, const double *A, const double *B, const double *A14,
                               const double *B14, std::vector<double> &sig,
@@ -538,7 +540,8 @@ int append_work_unit(const std::vector<int> &systems, int lj_table,
                      const std::vector<int> &sys_index, const std::vector<int> &atom_starts,
                      const std::vector<int> &atom_counts, const std::vector<int> &sys_template,
                      const TemplateStore &ts, std::vector<WuDesc> &wu,
-                     std::vector<WuSys> &wu_sys) {
+                     std::vector<WuSys> &wu_sys, const std::vector<int> *rst_first,
+                     const std::vector<int> *rst_count) {
   WuDesc d;
   d.sys_first = static_cast<int>(wu_sys.size());
   d.sys_count = static_cast<int>(systems.size());
@@ -546,7 +549,7 @@ int append_work_unit(const std::vector<int> &systems, int lj_table,
   d.task_count = 0;
   d.term_count = 0;
   d.lj_table = lj_table;
-  d.pad = 0;
+  d.rst_count = 0;
   int lds = 0;
   for (int s : systems) {
     const TopoTemplate &tp = ts.templates[sys_template[s]];
@@ -562,16 +565,73 @@ int append_work_unit(const std::vector<int> &systems, int lj_table,
     ws.term_count = tp.term_count;
     ws.lj_off = tp.lj_off;
     ws.lj_nt = tp.lj_nt;
-    ws.pad = 0;
+    ws.rst_first = (rst_first != nullptr) ? (*rst_first)[s] : 0;
+    ws.rst_count = (rst_count != nullptr) ? (*rst_count)[s] : 0;
     wu_sys.push_back(ws);
     d.offd_count += tp.offd_count;
     d.task_count += tp.offd_count + tp.diag_count;
     d.term_count += tp.term_count;
+    d.rst_count += ws.rst_count;
     lds += round_up(atom_counts[s], kTile);
   }
   d.lds_atoms = lds;
   wu.push_back(d);
   return lds;
+}
+
+int collect_restraints(const stormm_topology_batch *b, std::vector<stormm_restraint> &rst,
+                       std::vector<int> &first, std::vector<int> &count) {
+  const int ns = b->system_count;
+  rst.clear();
+  first.assign(ns, 0);
+  count.assign(ns, 0);
+  if (b->restraint_counts == nullptr) {
+    return STORMM_OK;
+  }
+  if (b->restraints == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "restraint counts given without restraints");
+  }
+  static const int natoms_of_kind[5] = {0, 1, 2, 3, 4};
+  size_t pos = 0;
+  for (int s = 0; s < ns; s++) {
+    const int n = b->restraint_counts[s];
+    if (n < 0) {
+      return fail(STORMM_ERR_INVALID_ARGUMENT, "negative restraint count");
+    }
+    first[s] = static_cast<int>(rst.size());
+    count[s] = n;
+    for (int r = 0; r < n; r++) {
+      const stormm_restraint &t = b->restraints[pos++];
+      const std::string where = "restraint " + std::to_string(r) + " of system " +
+                                std::to_string(s);
+      if (t.kind < STORMM_RESTRAINT_POSITION || t.kind > STORMM_RESTRAINT_DIHEDRAL) {
+        return fail(STORMM_ERR_INVALID_ARGUMENT, where + " has an unknown kind");
+      }
+      for (int a = 0; a < natoms_of_kind[t.kind]; a++) {
+        if (t.atoms[a] < 0 || t.atoms[a] >= b->atom_counts[s]) {
+          return fail(STORMM_ERR_INVALID_ARGUMENT, where + " names an atom outside the system");
+        }
+      }
+      // src/Restraints/bounded_restraint.cpp:667-695 (the records arrive with the adjustments
+      // of that routine already made).
+      for (const double *r4 : {t.init_r, t.final_r}) {
+        if (!(r4[0] <= r4[1] && r4[1] <= r4[2] && r4[2] <= r4[3])) {
+          return fail(STORMM_ERR_INVALID_ARGUMENT, where + " needs r1 <= r2 <= r3 <= r4");
+        }
+        if (t.kind == STORMM_RESTRAINT_ANGLE && (r4[1] < 0.0 || r4[2] >= kPi)) {
+          return fail(STORMM_ERR_INVALID_ARGUMENT, where + " needs r2 and r3 in [0, pi)");
+        }
+        if (t.kind == STORMM_RESTRAINT_DIHEDRAL && r4[2] - r4[1] >= 2.0 * kPi) {
+          return fail(STORMM_ERR_INVALID_ARGUMENT, where + " needs r3 - r2 < 2 pi");
+        }
+      }
+      if (t.final_step < t.init_step) {
+        return fail(STORMM_ERR_INVALID_ARGUMENT, where + " ends before it starts");
+      }
+      rst.push_back(t);
+    }
+  }
+  return STORMM_OK;
 }
 
 int build_plan(const stormm_topology_batch *b, const stormm_launch_shape *shape_in,
@@ -601,6 +661,10 @@ int build_plan(const stormm_topology_batch *b, const stormm_launch_shape *shape_
   P.nsys = ns;
   std::vector<SystemTerms> st;
   rc = collect_terms(b, st);
+  if (rc != STORMM_OK) {
+    return rc;
+  }
+  rc = collect_restraints(b, P.rst, P.rst_first, P.rst_count);
   if (rc != STORMM_OK) {
     return rc;
   }
@@ -641,7 +705,7 @@ int build_plan(const stormm_topology_batch *b, const stormm_launch_shape *shape_
       return rc;
     }
     P.task_count += tp.offd_count + tp.diag_count;
-    P.vterm_count += tp.term_count;
+    P.vterm_count += tp.term_count + P.rst_count[s];
   }
 
   // Work units.
@@ -649,7 +713,7 @@ int build_plan(const stormm_topology_batch *b, const stormm_launch_shape *shape_
   std::vector<double> cost(ns);
   std::vector<int> sys_index(ns);
   for (int s = 0; s < ns; s++) {
-    cost[s] = P.ts.templates[P.sys_template[s]].cost;
+    cost[s] = P.ts.templates[P.sys_template[s]].cost + P.rst_count[s];
     sys_index[s] = s;
   }
   std::vector<std::vector<int>> bins;
@@ -660,7 +724,7 @@ int build_plan(const stormm_topology_batch *b, const stormm_launch_shape *shape_
   for (size_t u = 0; u < bins.size(); u++) {
     P.maxa = std::max(P.maxa, append_work_unit(bins[u], bin_mode[u], sys_index, P.atom_starts,
                                                P.atom_counts, P.sys_template, P.ts, P.wu,
-                                               P.wu_sys));
+                                               P.wu_sys, &P.rst_first, &P.rst_count));
   }
   return STORMM_OK;
 }

@@ -95,6 +95,8 @@ struct SynthesisPlan {
   long long task_count = 0;
   long long vterm_count = 0;
   int lj_table_systems = 0;
+  std::vector<stormm_restraint> rst;       // all systems' restraints, system after system
+  std::vector<int> rst_first, rst_count;   // per system
 };
 
 // Validated solvent settings of a topology batch.
@@ -120,12 +122,18 @@ void pack_systems(const std::vector<double> &cost, const std::vector<int> &natom
                   std::vector<std::vector<int>> &bins_out, std::vector<int> &mode_out);
 
 // Appends one work unit holding `systems` (indices into the per-system arrays); returns its
-// LDS atom count.  `sys_index` gives each system's number in the kernels' per-system arrays.
+// LDS atom count.  `sys_index` gives each system's number in the kernels' per-system arrays;
+// rst_first / rst_count (null: no restraints) each system's restraints.
 int append_work_unit(const std::vector<int> &systems, int lj_table,
                      const std::vector<int> &sys_index, const std::vector<int> &atom_starts,
                      const std::vector<int> &atom_counts, const std::vector<int> &sys_template,
                      const TemplateStore &ts, std::vector<WuDesc> &wu,
-                     std::vector<WuSys> &wu_sys);
+                     std::vector<WuSys> &wu_sys, const std::vector<int> *rst_first = nullptr,
+                     const std::vector<int> *rst_count = nullptr);
+
+// Validated restraints of every system of a batch (atoms inside the system, bounds ordered).
+int collect_restraints(const stormm_topology_batch *b, std::vector<stormm_restraint> &rst,
+                       std::vector<int> &first, std::vector<int> &count);
 
 // Whole plan of a batch synthesis.
 int build_plan(const stormm_topology_batch *b, const stormm_launch_shape *shape_in,

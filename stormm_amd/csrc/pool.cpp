@@ -257,6 +257,14 @@ static int admit_entries(stormm_pool *pool, const stormm_topology_batch *batch,
     return fail(STORMM_ERR_INVALID_ARGUMENT, "the batch's implicit-solvent model differs from "
                                              "the pool's");
   }
+  if (batch->restraint_counts != nullptr) {
+    for (int s = 0; s < batch->system_count; s++) {
+      if (batch->restraint_counts[s] != 0) {
+        return fail(STORMM_ERR_INVALID_ARGUMENT, "the pool does not take restrained systems "
+                                                 "(minimize them with a synthesis)");
+      }
+    }
+  }
   stormm_pool *p = pool;
   const int ns = static_cast<int>(sys_of.size());
   if (static_cast<int>(p->free_slots.size()) < ns) {

@@ -43,7 +43,7 @@ struct stormm_synthesis {
   SynthesisPlan plan;
   DevMem atom_starts, atom_counts, q, ljp1, ljp2, lj_type, ljab, ljab14, rho, scr,
       pbr, alpha, beta, gamma, wu, wu_sys, tasks, masks, vterms, bond_par, angl_par, dihe_par,
-      dihe14, status, cycles_run, rms_force, stop_flags;
+      dihe14, status, cycles_run, rms_force, stop_flags, rst;
   MMKit kit;
 };
 
@@ -147,6 +147,7 @@ extern "C" int stormm_synthesis_create(const stormm_topology_batch *batch,
   HIP_TRY(ag->angl_par.upload(P.ts.angl_par));
   HIP_TRY(ag->dihe_par.upload(P.ts.dihe_par));
   HIP_TRY(ag->dihe14.upload(P.ts.dihe14));
+  HIP_TRY(ag->rst.upload(P.rst));
   std::vector<int> zero_status(1, 0);
   HIP_TRY(ag->status.upload(zero_status));
   std::vector<int> zi(P.nsys, 0);
@@ -187,6 +188,7 @@ extern "C" int stormm_synthesis_create(const stormm_topology_batch *batch,
   k.gb_beta = ag->beta.as<double>();
   k.gb_gamma = ag->gamma.as<double>();
   k.status = ag->status.as<int>();
+  k.rst = ag->rst.as<stormm_restraint>();
   *out = ag.release();
   return STORMM_OK;
 }
@@ -615,6 +617,11 @@ static MMKit make_kit(const stormm_synthesis *ag, stormm_phasespace *ps, stormm_
   k.clash_step = nullptr;
   k.clash_r0 = 0.0;
   k.clash_ratio = 0.0;
+  k.rst_step = 0;
+  k.rst_step_sys = nullptr;
+  k.rst_cycle_offset = 0;
+  k.rst_move = 0;
+  k.rst_clash_cycles = 0;
   return k;
 }
 
@@ -659,7 +666,7 @@ static int check_mincon(const stormm_minimize_controls *mc) {
 
 static int launch_terms(int32_t precision, const stormm_synthesis *ag, stormm_phasespace *ps,
                         stormm_scorecard *sc, int terms, int32_t eval_force, int32_t eval_energy,
-                        double clash_d, double clash_r, void *stream) {
+                        double clash_d, double clash_r, int step, void *stream) {
   int rc = check_objects(ag, ps, sc);
   if (rc != STORMM_OK) {
     return rc;
@@ -676,6 +683,7 @@ static int launch_terms(int32_t precision, const stormm_synthesis *ag, stormm_ph
   }
   MMKit k = make_kit(ag, ps, sc);
   set_clash(k, precision, clash_d, clash_r);
+  k.rst_step = step;
   HIP_TRY(launch_mm_eval(precision == STORMM_PREC_DOUBLE ? 0 : 1, k, terms, eval_force ? 1 : 0,
                          eval_energy ? 1 : 0, static_cast<hipStream_t>(stream)));
   return STORMM_OK;
@@ -686,9 +694,9 @@ extern "C" int stormm_launch_nonbonded(int32_t precision, const stormm_synthesis
                                        stormm_scorecard *sc, int32_t eval_force,
                                        int32_t eval_energy, double clash_minimum_distance,
                                        double clash_ratio, void *stream) {
-  (void)ctrl;
   return launch_terms(precision, ag, ps, sc, kTermNonbonded, eval_force, eval_energy,
-                      clash_minimum_distance, clash_ratio, stream);
+                      clash_minimum_distance, clash_ratio,
+                      (ctrl != nullptr) ? ctrl->step : 0, stream);
 }
 
 extern "C" int stormm_launch_valence(int32_t precision, const stormm_synthesis *ag,
@@ -696,9 +704,9 @@ extern "C" int stormm_launch_valence(int32_t precision, const stormm_synthesis *
                                      stormm_scorecard *sc, int32_t eval_force,
                                      int32_t eval_energy, double clash_minimum_distance,
                                      double clash_ratio, void *stream) {
-  (void)ctrl;
   return launch_terms(precision, ag, ps, sc, kTermValence, eval_force, eval_energy,
-                      clash_minimum_distance, clash_ratio, stream);
+                      clash_minimum_distance, clash_ratio,
+                      (ctrl != nullptr) ? ctrl->step : 0, stream);
 }
 
 extern "C" int stormm_launch_mm_evaluation(int32_t precision, const stormm_synthesis *ag,
@@ -706,9 +714,9 @@ extern "C" int stormm_launch_mm_evaluation(int32_t precision, const stormm_synth
                                            stormm_scorecard *sc, int32_t eval_force,
                                            int32_t eval_energy, double clash_minimum_distance,
                                            double clash_ratio, void *stream) {
-  (void)ctrl;
   return launch_terms(precision, ag, ps, sc, kTermNonbonded | kTermValence, eval_force,
-                      eval_energy, clash_minimum_distance, clash_ratio, stream);
+                      eval_energy, clash_minimum_distance, clash_ratio,
+                      (ctrl != nullptr) ? ctrl->step : 0, stream);
 }
 
 static CgKit make_cg_kit(const stormm_synthesis *ag, stormm_phasespace *ps, int precision) {
@@ -828,6 +836,7 @@ static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag, stor
       cr = mc->clash_vdw_ratio * progress;
     }
     set_clash(k, precision, cd, cr);
+    k.rst_step = minimizer_step(cyc, 0, cdcyc);
     HIP_TRY(launch_mm_eval(prec, k, terms, 1, 1, stream));
     ctrl.step += 1;
     if ((rc = stormm_launch_conjugate_gradient(precision, ag, ps, &ctrl, stream)) != STORMM_OK) {
@@ -835,6 +844,7 @@ static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag, stor
     }
     for (int move = 0; move < 4; move++) {
       if (move > 0) {
+        k.rst_step = minimizer_step(cyc, move, cdcyc);
         HIP_TRY(launch_mm_eval(prec, k, terms, 0, 1, stream));
       }
       if ((rc = stormm_launch_line_advance(precision, ag, ps, sc, lm, move, stream)) !=
@@ -844,6 +854,7 @@ static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag, stor
     }
   }
   set_clash(k, precision, 0.0, 0.0);
+  k.rst_step = minimizer_step(mc->total_cycles, 1, cdcyc);
   HIP_TRY(launch_mm_eval(prec, k, terms, 0, 1, stream));
   return STORMM_OK;
 }
@@ -854,7 +865,7 @@ static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag, stor
 // overwrite, so the graph holds kernels only.  With clash damping a second graph, whose
 // evaluations take each system's damping schedule from the same cycle counts, serves the first
 // clash_damping_cycles cycles.
-static hipError_t capture_cycle(int prec, const MMKit &kf, const MMKit &ke, const CgKit &cg,
+static hipError_t capture_cycle(int prec, const MMKit &kf, const MMKit *ke, const CgKit &cg,
                                 const LineKit &lk, int sd, hipStream_t cap,
                                 hipGraph_t *graph, hipGraphExec_t *exec) {
   const int terms = kTermNonbonded | kTermValence;
@@ -868,7 +879,7 @@ static hipError_t capture_cycle(int prec, const MMKit &kf, const MMKit &ke, cons
   }
   for (int move = 0; move < 4 && e2 == hipSuccess; move++) {
     if (move > 0) {
-      e2 = launch_mm_eval(prec, ke, terms, 0, 1, cap);
+      e2 = launch_mm_eval(prec, ke[move - 1], terms, 0, 1, cap);
     }
     if (e2 == hipSuccess) {
       e2 = launch_line_advance(lk, move, cap);
@@ -922,19 +933,37 @@ static int minimize_graph(int32_t precision, const stormm_synthesis *ag_c, storm
   kf_clash.clash_ratio = mc->clash_vdw_ratio;
   ke_clash = kf_clash;
   ke_clash.clash_step_offset = -1;
+  // Restraint steps from the same cycle counts: the force evaluation reads cycle i, the energy
+  // evaluations i + 1.
+  k.rst_step_sys = ag->cycles_run.as<int>();
+  k.rst_clash_cycles = cdcyc;
+  kf_clash.rst_step_sys = k.rst_step_sys;
+  kf_clash.rst_clash_cycles = cdcyc;
+  MMKit ke[3], ke_cl[3];
+  for (int m = 0; m < 3; m++) {
+    ke[m] = k;
+    ke_cl[m] = ke_clash;
+    ke[m].rst_cycle_offset = ke_cl[m].rst_cycle_offset = -1;
+    ke[m].rst_move = ke_cl[m].rst_move = m + 1;
+    ke_cl[m].rst_step_sys = k.rst_step_sys;
+    ke_cl[m].rst_clash_cycles = cdcyc;
+  }
   hipStream_t cap = nullptr;
   HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
   hipGraph_t graph = nullptr, graph_clash = nullptr;
   hipGraphExec_t exec = nullptr, exec_clash = nullptr;
-  hipError_t err = capture_cycle(prec, k, k, cg, lk, sd, cap, &graph, &exec);
+  hipError_t err = capture_cycle(prec, k, ke, cg, lk, sd, cap, &graph, &exec);
   if (err == hipSuccess && cdcyc > 0) {
-    err = capture_cycle(prec, kf_clash, ke_clash, cg, lk, sd, cap, &graph_clash, &exec_clash);
+    err = capture_cycle(prec, kf_clash, ke_cl, cg, lk, sd, cap, &graph_clash, &exec_clash);
   }
   for (int c = 0; c < mc->total_cycles && err == hipSuccess; c++) {
     err = hipGraphLaunch((c < cdcyc) ? exec_clash : exec, stream);
   }
   if (err == hipSuccess) {
-    err = launch_mm_eval(prec, k, terms, 0, 1, stream);
+    MMKit kfin = k;
+    kfin.rst_step_sys = nullptr;
+    kfin.rst_step = minimizer_step(mc->total_cycles, 1, cdcyc);
+    err = launch_mm_eval(prec, kfin, terms, 0, 1, stream);
   }
   if (err == hipSuccess) {
     err = hipStreamSynchronize(stream);
@@ -976,6 +1005,11 @@ extern "C" int stormm_launch_minimization(int32_t precision, const stormm_synthe
   }
   if (persistent == 2) {
     return minimize_graph(precision, ag, ps, mincon, sc, lm, st);
+  }
+  if (!ag->plan.rst.empty()) {
+    return fail(STORMM_ERR_STATE, "the persistent minimizer does not evaluate restraints; "
+                                  "minimize a restrained synthesis with the per-step (0) or "
+                                  "graph (2) mode");
   }
   MinKit k;
   k.mm = make_kit(ag, ps, sc);

@@ -24,8 +24,9 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import (LaunchShape, MinimizeControls, PREC_DOUBLE, PREC_SINGLE, SynthesisInfo,
-                   TopologyBatch, check)
+from . import restraints as R
+from ._lib import (LaunchShape, MinimizeControls, PREC_DOUBLE, PREC_SINGLE, StormmError,
+                   SynthesisInfo, TopologyBatch, check)
 from .topology import Topology
 
 SCORECARD_STRIDE = 32
@@ -60,7 +61,7 @@ def _stream_ptr(stream) -> Optional[int]:
 class _BatchArrays:
     """Concatenated per-system topology arrays behind one TopologyBatch struct."""
 
-    def __init__(self, topos: Sequence[Topology]):
+    def __init__(self, topos: Sequence[Topology], restraints=None):
         if len(topos) == 0:
             raise ValueError("a synthesis needs at least one topology")
         igbs = {t.igb for t in topos}
@@ -125,6 +126,20 @@ class _BatchArrays:
         b.dielectric = topos[0].dielectric
         b.salt_concentration = topos[0].salt
         b.coulomb_constant = topos[0].coulomb
+        # Per-system restraint records (None: no restraints anywhere).
+        b.restraint_counts = None
+        b.restraints = None
+        if restraints is not None:
+            if len(restraints) != len(topos):
+                raise ValueError("give one restraint array (or None) per system")
+            recs = [R.empty() if r is None else np.asarray(r, dtype=R.RESTRAINT_DTYPE).reshape(-1)
+                    for r in restraints]
+            allr = np.ascontiguousarray(np.concatenate(recs) if recs else R.empty())
+            if allr.size == 0:
+                allr = R.empty(1)
+            keep.append(allr)
+            b.restraint_counts = _ptr(counts([len(r) for r in recs]))
+            b.restraints = allr.ctypes.data
         self.struct = b
         self.keep = keep
 
@@ -141,10 +156,10 @@ def _shape(shape) -> Optional[LaunchShape]:
     return s
 
 
-def plan_synthesis(topos: Sequence[Topology], shape=None) -> SynthesisInfo:
+def plan_synthesis(topos: Sequence[Topology], shape=None, restraints=None) -> SynthesisInfo:
     """Host-only work-unit plan (no device memory): counts of units, tasks, pairs, terms."""
     L = _lib.load()
-    arrs = _BatchArrays(topos)
+    arrs = _BatchArrays(topos, restraints)
     info = SynthesisInfo()
     sh = _shape(shape)
     check(L.stormm_synthesis_plan(ctypes.byref(arrs.struct),
@@ -173,12 +188,15 @@ def autotune(topos: Sequence[Topology], precision="single", coords=None):
 
 
 class AtomGraphSynthesis:
-    """Many topologies batched into one device-resident parameter set with work units."""
+    """Many topologies batched into one device-resident parameter set with work units.
+    `restraints`, when given, holds one restraint record array (stormm_amd.restraints) or None
+    per system, the counterpart of the RestraintApparatus list the reference's synthesis takes
+    (src/Synthesis/atomgraph_synthesis.h:58-104)."""
 
-    def __init__(self, topos: Sequence[Topology], shape=None):
+    def __init__(self, topos: Sequence[Topology], shape=None, restraints=None):
         self._L = _lib.load()
         self.topologies = list(topos)
-        arrs = _BatchArrays(self.topologies)
+        arrs = _BatchArrays(self.topologies, restraints)
         sh = _shape(shape)
         h = ctypes.c_void_p()
         check(self._L.stormm_synthesis_create(ctypes.byref(arrs.struct),

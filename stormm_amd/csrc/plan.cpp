@@ -329,12 +329,115 @@ static int add_template(const stormm_topology_batch *b, const SystemTerms &t, in
 }
 
 
-// Longest-processing-time assignment of systems onto about `target_total` work units, within
+// Longest-processing-time assignment of systems onto `groups` x `per_group` work units, within
 // the LDS atom capacity and the per-unit system limit; stored-table and combining-rule systems
-// go to separate units.  Units come out in order of decreasing cost.
-void pack_systems(const std::vector<double> &cost, const std::vector<int> &natom,
-                         const std::vector<char> &table_mode, int cap, int target_total,
+// go to separate units.  Systems go, largest first, to the least loaded group and there to its
+// least loaded unit that can hold them, so that the groups' totals come out even.  Units are
+// emitted member by member: unit g + r * G is member r of group g, so the blocks that the
+// hardware makes co-resident on one compute unit (one from each pass over the device) hold one
+// group, and the compute units' loads are even rather than the units'.  Returns false (nothing
+// emitted) when the capacities need more groups than there are compute units.
+static bool pack_grouped(const std::vector<double> &cost, const std::vector<int> &natom,
+                         const std::vector<char> &table_mode, int cap, int groups, int per_group,
                          std::vector<std::vector<int>> &bins_out, std::vector<int> &mode_out) {
+  const int ns = static_cast<int>(cost.size());
+  double total_cost = 0.0;
+  for (int s = 0; s < ns; s++) {
+    total_cost += cost[s];
+  }
+  struct Bin {
+    double cost = 0.0;
+    int atoms = 0;
+    std::vector<int> systems;
+  };
+  struct Group {
+    double cost = 0.0;
+    int mode = 0;
+    std::vector<Bin> bins;
+  };
+  std::vector<Group> all;
+  for (int mode = 0; mode < 2; mode++) {
+    std::vector<int> order;
+    double mode_cost = 0.0;
+    for (int s = 0; s < ns; s++) {
+      if (table_mode[s] == mode) {
+        order.push_back(s);
+        mode_cost += cost[s];
+      }
+    }
+    if (order.empty()) {
+      continue;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+    const int target = std::max(1, static_cast<int>(std::lround(groups * mode_cost /
+                                                                std::max(total_cost, 1.0))));
+    std::vector<Group> grp;
+    for (int s : order) {
+      const int need = round_up(natom[s], kTile);
+      int best_g = -1, best_b = -1;
+      for (size_t g = 0; g < grp.size(); g++) {
+        if (best_g >= 0 && grp[g].cost >= grp[best_g].cost) {
+          continue;
+        }
+        int b_fit = -1;
+        for (int b = 0; b < per_group; b++) {
+          const Bin &bn = grp[g].bins[b];
+          if (bn.atoms + need <= cap && static_cast<int>(bn.systems.size()) < kMaxWuSystems &&
+              (b_fit < 0 || bn.cost < grp[g].bins[b_fit].cost)) {
+            b_fit = b;
+          }
+        }
+        if (b_fit >= 0) {
+          best_g = static_cast<int>(g);
+          best_b = b_fit;
+        }
+      }
+      if (best_g < 0 || (static_cast<int>(grp.size()) < target && grp[best_g].cost > 0.0)) {
+        Group ng;
+        ng.mode = mode;
+        ng.bins.resize(per_group);
+        grp.push_back(std::move(ng));
+        best_g = static_cast<int>(grp.size()) - 1;
+        best_b = 0;
+      }
+      Bin &bn = grp[best_g].bins[best_b];
+      bn.systems.push_back(s);
+      bn.atoms += need;
+      bn.cost += cost[s];
+      grp[best_g].cost += cost[s];
+    }
+    for (Group &g : grp) {
+      std::stable_sort(g.bins.begin(), g.bins.end(), [](const Bin &a, const Bin &c) {
+        return a.cost > c.cost;
+      });
+      all.push_back(std::move(g));
+    }
+  }
+  if (static_cast<int>(all.size()) > groups) {
+    return false;
+  }
+  std::stable_sort(all.begin(), all.end(), [](const Group &a, const Group &c) {
+    return a.cost > c.cost;
+  });
+  bins_out.clear();
+  mode_out.clear();
+  for (int r = 0; r < per_group; r++) {
+    for (Group &g : all) {
+      if (!g.bins[r].systems.empty()) {
+        bins_out.push_back(std::move(g.bins[r].systems));
+        mode_out.push_back(g.mode);
+      }
+    }
+  }
+  return true;
+}
+
+// Plain longest-processing-time assignment onto about `target_total` units, emitted in order of
+// decreasing cost (for batches whose units outnumber the blocks the device holds at once, which
+// the hardware then schedules as blocks finish).
+static void pack_lpt(const std::vector<double> &cost, const std::vector<int> &natom,
+                     const std::vector<char> &table_mode, int cap, int target_total,
+                     std::vector<std::vector<int>> &bins_out, std::vector<int> &mode_out) {
   const int ns = static_cast<int>(cost.size());
   double total_cost = 0.0;
   for (int s = 0; s < ns; s++) {
@@ -398,6 +501,17 @@ void pack_systems(const std::vector<double> &cost, const std::vector<int> &natom
   }
 }
 
+void pack_systems(const std::vector<double> &cost, const std::vector<int> &natom,
+                  const std::vector<char> &table_mode, int cap, int groups, int per_group,
+                  bool grouped, std::vector<std::vector<int>> &bins_out,
+                  std::vector<int> &mode_out) {
+  groups = std::max(1, groups);
+  per_group = std::max(1, per_group);
+  if (!grouped ||
+      !pack_grouped(cost, natom, table_mode, cap, groups, per_group, bins_out, mode_out)) {
+    pack_lpt(cost, natom, table_mode, cap, groups * per_group, bins_out, mode_out);
+  }
+}
 
 int solvent_model(const stormm_topology_batch *b, SolventModel &sm) {
   if (b->igb != 0 && b->igb != 1 && b->igb != 2 && b->igb != 5) {
@@ -718,8 +832,7 @@ int build_plan(const stormm_topology_batch *b, const stormm_launch_shape *shape_
   }
   std::vector<std::vector<int>> bins;
   std::vector<int> bin_mode;
-  pack_systems(cost, P.atom_counts, P.lj_table_mode, cap, std::max(1, cus * wu_per_cu), bins,
-               bin_mode);
+  pack_systems(cost, P.atom_counts, P.lj_table_mode, cap, cus, wu_per_cu, true, bins, bin_mode);
   P.maxa = 0;
   for (size_t u = 0; u < bins.size(); u++) {
     P.maxa = std::max(P.maxa, append_work_unit(bins[u], bin_mode[u], sys_index, P.atom_starts,

@@ -116,10 +116,14 @@ int fill_atom_parameters(const stormm_topology_batch *b, const SystemTerms &t, i
                          const TopoTemplate &tp, const SolventModel &sm, int start,
                          AtomParams &ap, int s);
 
-// Longest-processing-time assignment of systems onto about `target_total` work units.
+// Longest-processing-time assignment of systems onto about `groups` x `per_group` work units.
+// With `grouped`, and when every unit fits on the device at once, the systems are balanced over
+// `groups` groups (one per compute unit) emitted so that units g, g + G, g + 2G, ... form group
+// g; otherwise the units are balanced individually and emitted by decreasing cost.
 void pack_systems(const std::vector<double> &cost, const std::vector<int> &natom,
-                  const std::vector<char> &table_mode, int cap, int target_total,
-                  std::vector<std::vector<int>> &bins_out, std::vector<int> &mode_out);
+                  const std::vector<char> &table_mode, int cap, int groups, int per_group,
+                  bool grouped, std::vector<std::vector<int>> &bins_out,
+                  std::vector<int> &mode_out);
 
 // Appends one work unit holding `systems` (indices into the per-system arrays); returns its
 // LDS atom count.  `sys_index` gives each system's number in the kernels' per-system arrays;

@@ -569,8 +569,9 @@ extern "C" int stormm_pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *
   const int block = (sh.block_threads > 0) ? sh.block_threads : kEvalBlock;
   std::vector<std::vector<int>> bins;
   std::vector<int> modes;
-  pack_systems(cost, natom, table, cap, std::max(1, cu_count_or_default() * per_cu), bins,
-               modes);
+  // The resident set changes every call; balancing units one by one (largest first) serves
+  // these short launches better than the compute-unit groups of the batch synthesis.
+  pack_systems(cost, natom, table, cap, cu_count_or_default(), per_cu, false, bins, modes);
   std::vector<WuDesc> wu;
   std::vector<WuSys> wu_sys;
   int maxa = 0;

@@ -75,8 +75,8 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu):
     topos, coords = synthetic.config2(nsys=nsys, seed=2)
     sd, dx0 = 50, 0.01
     mc = S.minimize_controls(cycles, sd, dx0, 0.0)
-    # Work-unit shape from the autotuner (timed force evaluations of a 1024-system sample).
-    shape, tuned_ms = S.autotune(topos[:1024], "single", coords[:1024])
+    # Work-unit shape from the autotuner (timed force evaluations of the whole batch).
+    shape, tuned_ms = S.autotune(topos, "single", coords)
 
     def setup(sub_t, sub_c):
         ag = S.AtomGraphSynthesis(sub_t, shape)

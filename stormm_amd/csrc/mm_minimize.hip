@@ -25,6 +25,11 @@ __device__ __forceinline__ double wave_sum64(double v) {
 // With per-system step counters (serving), each system advances its own counter and stops (its
 // `done` flag is set, the direction left untouched and no line move follows) once its RMS force
 // reaches `conv_rms` or its counter reaches `max_steps`: the persistent minimizer's stopping rule.
+// Systems of up to kCgCached x 64 atoms keep their forces and directions in registers from the
+// first pass to the last, so the three passes cost one round trip to memory; larger systems read
+// them again in each pass.  Both forms accumulate in the same order.
+constexpr int kCgCached = 4;
+
 __global__ void __launch_bounds__(64) k_conjugate_gradient(CgKit k, int step, int sd_cycles) {
   const int s = (k.slots != nullptr) ? k.slots[blockIdx.x] : static_cast<int>(blockIdx.x);
   const int lane = threadIdx.x;
@@ -33,19 +38,48 @@ __global__ void __launch_bounds__(64) k_conjugate_gradient(CgKit k, int step, in
   }
   const int start = k.sys_start[s];
   const int n = k.sys_natom[s];
+  const int prior = (k.steps != nullptr) ? k.steps[s] : 0;
   long long *frc[3] = {k.xfrc, k.yfrc, k.zfrc};
   long long *prv[3] = {k.xprv, k.yprv, k.zprv};
   long long *cg[3] = {k.xcg, k.ycg, k.zcg};
+  const bool cached = (n <= 64 * kCgCached);
+  long long rf[kCgCached][3], rp[kCgCached][3], rc[kCgCached][3];
+  if (cached) {
+#pragma unroll
+    for (int it = 0; it < kCgCached; it++) {
+      const int i = lane + (64 * it);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        rf[it][d] = (i < n) ? frc[d][start + i] : 0LL;
+        rp[it][d] = (i < n) ? prv[d][start + i] : 0LL;
+        rc[it][d] = (i < n) ? cg[d][start + i] : 0LL;
+      }
+    }
+  }
   double ff = 0.0, gg = 0.0, dgg = 0.0;
-  for (int i = lane; i < n; i += 64) {
-    for (int d = 0; d < 3; d++) {
-      const long long lf = frc[d][start + i];
-      const long long lp = prv[d][start + i];
-      const double df = static_cast<double>(lf);
-      const double dp = static_cast<double>(lp);
-      ff += df * df;
-      gg += dp * dp;
-      dgg += static_cast<double>(lf - lp) * df;
+  auto accumulate = [&](long long lf, long long lp) {
+    const double df = static_cast<double>(lf);
+    const double dp = static_cast<double>(lp);
+    ff += df * df;
+    gg += dp * dp;
+    dgg += static_cast<double>(lf - lp) * df;
+  };
+  if (cached) {
+#pragma unroll
+    for (int it = 0; it < kCgCached; it++) {
+      if (lane + (64 * it) < n) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          accumulate(rf[it][d], rp[it][d]);
+        }
+      }
+    }
+  }
+  else {
+    for (int i = lane; i < n; i += 64) {
+      for (int d = 0; d < 3; d++) {
+        accumulate(frc[d][start + i], prv[d][start + i]);
+      }
     }
   }
   ff = wave_sum64(ff);
@@ -57,45 +91,86 @@ __global__ void __launch_bounds__(64) k_conjugate_gradient(CgKit k, int step, in
       k.rms[s] = rms;
     }
     const bool stop = (k.conv_rms > 0.0 && rms <= k.conv_rms) ||
-                      (k.max_steps > 0 && k.steps[s] >= k.max_steps);
+                      (k.max_steps > 0 && prior >= k.max_steps);
     if (stop) {
       if (lane == 0) {
         k.done[s] = 1;
       }
       return;
     }
-    step = k.steps[s] + 1;
+    step = prior + 1;
     if (lane == 0) {
       k.steps[s] = step;
     }
   }
   const double gam = (step >= sd_cycles) ? (dgg / gg) : 0.0;
+  const bool use_gam = fabs(gam) > 1.0e-12;
   double msum = 0.0;
-  for (int i = lane; i < n; i += 64) {
-    const int g = start + i;
-    for (int d = 0; d < 3; d++) {
-      const long long ifv = frc[d][g];
-      prv[d][g] = ifv;
-      if (fabs(gam) > 1.0e-12) {
-        const double dfcg = static_cast<double>(ifv) + (gam * static_cast<double>(cg[d][g]));
-        msum += dfcg * dfcg;
-        const long long v = static_cast<long long>(__builtin_rint(dfcg));
-        cg[d][g] = v;
-        frc[d][g] = v;
+  // New direction of one component: the previous force is recorded, the conjugate direction
+  // replaces the force (when gamma is not negligible) and msum collects its square.
+  auto direct = [&](long long ifv, long long &pv, long long &cv, long long &fv) {
+    pv = ifv;
+    if (use_gam) {
+      const double dfcg = static_cast<double>(ifv) + (gam * static_cast<double>(cv));
+      msum += dfcg * dfcg;
+      const long long v = static_cast<long long>(__builtin_rint(dfcg));
+      cv = v;
+      fv = v;
+    }
+    else {
+      msum += static_cast<double>(ifv) * static_cast<double>(ifv);
+      cv = ifv;
+    }
+  };
+  if (cached) {
+#pragma unroll
+    for (int it = 0; it < kCgCached; it++) {
+      if (lane + (64 * it) < n) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          direct(rf[it][d], rp[it][d], rc[it][d], rf[it][d]);
+        }
       }
-      else {
-        msum += static_cast<double>(ifv) * static_cast<double>(ifv);
-        cg[d][g] = ifv;
+    }
+  }
+  else {
+    for (int i = lane; i < n; i += 64) {
+      const int g = start + i;
+      for (int d = 0; d < 3; d++) {
+        long long pv, cv = cg[d][g], fv = frc[d][g];
+        direct(fv, pv, cv, fv);
+        prv[d][g] = pv;
+        cg[d][g] = cv;
+        frc[d][g] = fv;
       }
     }
   }
   msum = wave_sum64(msum);
   const double inv_factor = 1.0 / (sqrt(msum) * k.inv_frc_scale);
-  for (int i = lane; i < n; i += 64) {
-    const int g = start + i;
-    for (int d = 0; d < 3; d++) {
-      const double v = static_cast<double>(frc[d][g]) * inv_factor;
-      frc[d][g] = k.dp_mode ? static_cast<long long>(__builtin_rint(v)) : static_cast<long long>(v);
+  auto normalized = [&](long long fv) {
+    const double v = static_cast<double>(fv) * inv_factor;
+    return k.dp_mode ? static_cast<long long>(__builtin_rint(v)) : static_cast<long long>(v);
+  };
+  if (cached) {
+#pragma unroll
+    for (int it = 0; it < kCgCached; it++) {
+      const int i = lane + (64 * it);
+      if (i < n) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          prv[d][start + i] = rp[it][d];
+          cg[d][start + i] = rc[it][d];
+          frc[d][start + i] = normalized(rf[it][d]);
+        }
+      }
+    }
+  }
+  else {
+    for (int i = lane; i < n; i += 64) {
+      const int g = start + i;
+      for (int d = 0; d < 3; d++) {
+        frc[d][g] = normalized(frc[d][g]);
+      }
     }
   }
 }
@@ -110,19 +185,28 @@ hipError_t launch_conjugate_gradient(const CgKit &k, int step, int sd_cycles,
   return hipGetLastError();
 }
 
-// One line move per system, src/MolecularMechanics/line_movement.cui:4-415.  Stopped systems
-// (serving) keep their coordinates and line state.
-__global__ void __launch_bounds__(256) k_line_advance(LineKit k, int move_number) {
-  __shared__ double sh_move;
-  const int s = (k.slots != nullptr) ? k.slots[blockIdx.x] : static_cast<int>(blockIdx.x);
+// One line move per system, src/MolecularMechanics/line_movement.cui:4-415, one wavefront per
+// system and four systems per block.  Stopped systems (serving) keep their coordinates and line
+// state.
+constexpr int kLineWaves = 4;
+
+__global__ void __launch_bounds__(64 * kLineWaves) k_line_advance(LineKit k, int move_number) {
+  const int lane = threadIdx.x & 63;
+  const int e = (blockIdx.x * kLineWaves) + static_cast<int>(threadIdx.x >> 6);
+  if (e >= k.nsys) {
+    return;
+  }
+  const int s = (k.slots != nullptr) ? k.slots[e] : e;
   if (k.done != nullptr && k.done[s] != 0) {
     return;
   }
-  if (threadIdx.x == 0) {
-    long long acc = 0;
-    for (int c = 0; c <= 12; c++) {
-      acc += k.scorecard[static_cast<size_t>(s) * kScoreStride + c];
-    }
+  // Potential energy: the 13 ScoreCard components, one per lane, summed exactly.
+  long long acc = (lane <= 12) ? k.scorecard[(static_cast<size_t>(s) * kScoreStride) + lane] : 0;
+  for (int m = 32; m >= 1; m >>= 1) {
+    acc += __shfl_xor(acc, m, 64);
+  }
+  double mf = 0.0;
+  if (lane == 0) {
     const double cur = static_cast<double>(acc) * k.inv_nrg_scale;
     LineState ls;
     ls.l_move = k.l_move[s];
@@ -134,7 +218,7 @@ __global__ void __launch_bounds__(256) k_line_advance(LineKit k, int move_number
     ls.nrg_b = k.nrg_b[s];
     ls.nrg_c = k.nrg_c[s];
     ls.nrg_d = k.nrg_d[s];
-    double mf = line_move_factor(ls, move_number, cur);
+    mf = line_move_factor(ls, move_number, cur);
     k.l_move[s] = ls.l_move;
     k.s_move[s] = ls.s_move;
     k.mfac_a[s] = ls.mfac_a;
@@ -150,16 +234,14 @@ __global__ void __launch_bounds__(256) k_line_advance(LineKit k, int move_number
     else {
       mf = 0.0;
     }
-    sh_move = mf;
   }
-  __syncthreads();
-  const double mf = sh_move;
+  mf = __shfl(mf, 0, 64);
   if (mf == 0.0) {
     return;
   }
   const int start = k.sys_start[s];
   const int n = k.sys_natom[s];
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = lane; i < n; i += 64) {
     const int g = start + i;
     k.xcrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.xfrc[g])));
     k.ycrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.yfrc[g])));
@@ -171,7 +253,8 @@ hipError_t launch_line_advance(const LineKit &k, int move_number, hipStream_t st
   if (k.nsys == 0) {
     return hipSuccess;
   }
-  hipLaunchKernelGGL(k_line_advance, dim3(k.nsys), dim3(256), 0, stream, k, move_number);
+  hipLaunchKernelGGL(k_line_advance, dim3((k.nsys + kLineWaves - 1) / kLineWaves),
+                     dim3(64 * kLineWaves), 0, stream, k, move_number);
   return hipGetLastError();
 }
 

@@ -89,6 +89,10 @@ __device__ void stage_load_params(StageCtx<T> &c, const MMKit &k) {
   __syncthreads();
 }
 
+template <typename T>
+__device__ __forceinline__ bool clash_params(const MMKit &k, int sys_index, int cycle, T &d,
+                                             T &r);
+
 // Coordinates into LDS (relative to a per-system origin) and all accumulators zeroed.
 template <typename T, bool CLASH>
 __device__ void stage_load_coords(StageCtx<T> &c, const MMKit &k) {

@@ -78,8 +78,8 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu):
     # Work-unit shape from the autotuner (timed force evaluations of the whole batch).
     shape, tuned_ms = S.autotune(topos, "single", coords)
 
-    def setup(sub_t, sub_c):
-        ag = S.AtomGraphSynthesis(sub_t, shape)
+    def setup(sub_t, sub_c, sub_shape=None):
+        ag = S.AtomGraphSynthesis(sub_t, sub_shape or shape)
         ps = S.PhaseSpaceSynthesis(ag, 32, 23)
         sc = S.ScoreCard(ag.system_count, 26)
         lm = S.LineMinimization(ag.system_count, dx0)
@@ -114,7 +114,9 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu):
     for lo, hi in zip(edges[:-1], edges[1:]):
         idx = [i for i, t in enumerate(topos) if lo <= t.natom < hi]
         if idx:
-            parts.append(setup([topos[i] for i in idx], [coords[i] for i in idx]))
+            # Each class gets the shape tuned for it.
+            sub_t, sub_c = [topos[i] for i in idx], [coords[i] for i in idx]
+            parts.append(setup(sub_t, sub_c, S.autotune(sub_t, "single", sub_c)[0]))
     t_class = run(parts, True)
     out = {"systems": nsys, "cycles": cycles, "sd_cycles": sd, "precision": "fp32",
            "autotuned_shape": shape,

@@ -1112,7 +1112,7 @@ extern "C" int stormm_autotune_launch_shape(const stormm_topology_batch *batch,
     maxn = std::max(maxn, batch->atom_counts[s]);
   }
   const int caps[4] = {128, 256, 384, 512};
-  const int per_cu[3] = {1, 2, 4};
+  const int per_cu[4] = {1, 2, 3, 4};
   const int blocks[3] = {256, 512, 1024};
   double best_t = 1.0e30;
   stormm_launch_shape best_shape = {512, 2, kEvalBlock};

@@ -11,6 +11,8 @@
 // result does not depend on which lanes or work units processed which tiles.
 #pragma once
 
+#include <type_traits>
+
 #include "mm_device.h"
 
 namespace stormm_amd {
@@ -943,10 +945,21 @@ __device__ __forceinline__ double dot(const Vec3 &a, const Vec3 &b) {
   return (a.x * b.x) + (a.y * b.y) + (a.z * b.z);
 }
 
+struct Vec3f {
+  float x, y, z;
+};
+__device__ __forceinline__ Vec3f cross(const Vec3f &a, const Vec3f &b) {
+  return {(a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x)};
+}
+__device__ __forceinline__ float dot(const Vec3f &a, const Vec3f &b) {
+  return (a.x * b.x) + (a.y * b.y) + (a.z * b.z);
+}
+
 template <typename C> struct ValenceCtxT {
   const C *cx, *cy, *cz;
   const int *gidx;
   double crd_scale;
+  float crd_scale_f;
   long long *fx, *fy, *fz;
   // Absolute position (A) of LDS atom a of a system whose coordinate origin is org (fixed
   // precision at inv_gpos).
@@ -954,6 +967,12 @@ template <typename C> struct ValenceCtxT {
     return {(static_cast<double>(org[0]) * inv_gpos) + (static_cast<double>(cx[a]) * crd_scale),
             (static_cast<double>(org[1]) * inv_gpos) + (static_cast<double>(cy[a]) * crd_scale),
             (static_cast<double>(org[2]) * inv_gpos) + (static_cast<double>(cz[a]) * crd_scale)};
+  }
+  // Single-precision displacement: the LDS coordinates' difference is exact, then rounded once.
+  __device__ __forceinline__ Vec3f dispf(int from, int to) const {
+    return {static_cast<float>(cx[to] - cx[from]) * crd_scale_f,
+            static_cast<float>(cy[to] - cy[from]) * crd_scale_f,
+            static_cast<float>(cz[to] - cz[from]) * crd_scale_f};
   }
   __device__ __forceinline__ Vec3 disp(int from, int to) const {
     return {(static_cast<double>(cx[to]) - static_cast<double>(cx[from])) * crd_scale,
@@ -1137,6 +1156,128 @@ __device__ __forceinline__ void eval_14(const V &v, const MMKit &k, bool lj_tabl
     const long long ix = fix64(fmag * d.x * fscale);
     const long long iy = fix64(fmag * d.y * fscale);
     const long long iz = fix64(fmag * d.z * fscale);
+    v.add(i, ix, iy, iz);
+    v.add(l, -ix, -iy, -iz);
+  }
+}
+
+// Single-precision forms of the cosine dihedral and the attenuated 1:4 pair for the
+// single-precision model, as the reference evaluates them there (valence_potential.cui:540-640,
+// 930-1040 with TCALC = float).  Their energies carry no cancellation (a cosine series and
+// pair potentials at 1:4 distances), unlike r - r0 and theta - theta0 of the bonds and angles,
+// which stay in double precision.  Guards: asymptotic_to_one_f and
+// inverse_one_minus_asymptote_f (src/Constants/symbol_values.h:72-82).
+template <bool FORCES, typename V>
+__device__ __forceinline__ float eval_dihedral_f(const V &v, int i, int j, int kk, int l,
+                                                 const DiheParam &dp, int frc_bits) {
+  using P = Prec<float>;
+  const Vec3f ab = v.dispf(i, j);
+  const Vec3f bc = v.dispf(j, kk);
+  const Vec3f cd = v.dispf(kk, l);
+  Vec3f crabbc = cross(ab, bc);
+  Vec3f crbccd = cross(bc, cd);
+  const float inv12 = P::rsqrt(dot(crabbc, crabbc) * dot(crbccd, crbccd));
+  const float invbc = P::rsqrt(dot(bc, bc));
+  const Vec3f scr = cross(crabbc, crbccd);
+  const float cth = dot(crabbc, crbccd) * inv12;
+  const float sth = dot(scr, bc) * inv12 * invbc;
+  float cn = 1.0f, sn = 0.0f;
+  for (int p = 0; p < dp.period; p++) {
+    const float t = (cn * cth) - (sn * sth);
+    sn = (sn * cth) + (cn * sth);
+    cn = t;
+  }
+  const float cph = static_cast<float>(dp.cos_phase);
+  const float sph = static_cast<float>(dp.sin_phase);
+  const float amp = static_cast<float>(dp.amp);
+  const float cos_sangle = (cn * cph) + (sn * sph);
+  if (FORCES) {
+    const float sin_sangle = (sn * cph) - (cn * sph);
+    const float fr = amp * static_cast<float>(dp.period) * sin_sangle;
+    const float invab = P::rsqrt(dot(ab, ab));
+    const float invcd = P::rsqrt(dot(cd, cd));
+    const float mgab = dot(ab, ab) * invab;
+    const float mgbc = dot(bc, bc) * invbc;
+    const float cosb = -dot(ab, bc) * invab * invbc;
+    const float cosc = -dot(bc, cd) * invbc * invcd;
+    constexpr float kAsym = 0.99999904632568359375f;
+    constexpr float kInvAsym = 1048576.0f;
+    const float isinb2 = (cosb * cosb < kAsym) ? fr * P::rcp(1.0f - (cosb * cosb)) : fr * kInvAsym;
+    const float isinc2 = (cosc * cosc < kAsym) ? fr * P::rcp(1.0f - (cosc * cosc)) : fr * kInvAsym;
+    const float invabc = invab * invbc;
+    const float invbcd = invbc * invcd;
+    crabbc = {crabbc.x * invabc, crabbc.y * invabc, crabbc.z * invabc};
+    crbccd = {crbccd.x * invbcd, crbccd.y * invbcd, crbccd.z * invbcd};
+    const float fa = -invab * isinb2;
+    const float fb1 = (mgbc - (mgab * cosb)) * invabc * isinb2;
+    const float fb2 = cosc * invbc * isinc2;
+    const float fd = -invcd * isinc2;
+    const long long fix = fixed_contrib(crabbc.x * fa, frc_bits);
+    const long long fiy = fixed_contrib(crabbc.y * fa, frc_bits);
+    const long long fiz = fixed_contrib(crabbc.z * fa, frc_bits);
+    v.add(i, fix, fiy, fiz);
+    const long long flx = fixed_contrib(-fd * crbccd.x, frc_bits);
+    const long long fly = fixed_contrib(-fd * crbccd.y, frc_bits);
+    const long long flz = fixed_contrib(-fd * crbccd.z, frc_bits);
+    v.add(l, flx, fly, flz);
+    const long long fjx = fixed_contrib((fb1 * crabbc.x) - (fb2 * crbccd.x), frc_bits);
+    const long long fjy = fixed_contrib((fb1 * crabbc.y) - (fb2 * crbccd.y), frc_bits);
+    const long long fjz = fixed_contrib((fb1 * crabbc.z) - (fb2 * crbccd.z), frc_bits);
+    v.add(j, fjx, fjy, fjz);
+    v.add(kk, -(fix + fjx + flx), -(fiy + fjy + fly), -(fiz + fjz + flz));
+  }
+  return amp * (1.0f + cos_sangle);
+}
+
+template <bool FORCES, bool CLASH, typename V>
+__device__ __forceinline__ void eval_14_f(const V &v, const MMKit &k, bool lj_table, int2 lj,
+                                          int i, int l, float inv_scee, float inv_scnb,
+                                          int frc_bits, bool clash_on, float cd, float cratio,
+                                          float &ele, float &vdw) {
+  using P = Prec<float>;
+  const int gi = v.gidx[i], gl = v.gidx[l];
+  const Vec3f d = v.dispf(i, l);
+  const float r2 = dot(d, d);
+  const float qiqj = static_cast<float>(k.q[gi] * k.q[gl]) * inv_scee;
+  float lja, ljb, sigma_lb = 0.0f;
+  if (lj_table) {
+    const double2 ab = k.ljab14[lj.x + (k.lj_type[gi] * lj.y) + k.lj_type[gl]];
+    lja = static_cast<float>(ab.x) * inv_scnb;
+    ljb = static_cast<float>(ab.y) * inv_scnb;
+  }
+  else {
+    sigma_lb = 0.5f * static_cast<float>(k.ljp1[gi] + k.ljp1[gl]);
+    const float sig2 = sigma_lb * sigma_lb;
+    const float sig6 = sig2 * sig2 * sig2;
+    ljb = static_cast<float>(k.ljp2[gi] * k.ljp2[gl]) * inv_scnb * sig6;
+    lja = ljb * sig6;
+  }
+  float fmag = 0.0f;
+  if (CLASH && clash_on) {
+    ele = 0.0f;
+    vdw = 0.0f;
+    const float sigma = (ljb > 1.0e-6f) ? (lj_table ? P::sqrt(P::cbrt(lja / ljb)) : sigma_lb)
+                                        : 0.0f;
+    const float ca = -0.5f / (cd * cd * (cd + 1.0f));
+    const float cb = (1.0f / cd) - (ca * (cd + 1.0f) * (cd + 1.0f));
+    clash_pair<float, FORCES>(P::sqrt(r2), qiqj, lja, ljb, sigma, cd, ca, cb, cratio, ele, vdw,
+                              fmag);
+  }
+  else {
+    const float invr = P::rsqrt(r2);
+    const float invr2 = invr * invr;
+    const float invr4 = invr2 * invr2;
+    ele = qiqj * invr;
+    vdw = (lja * invr4 * invr4 * invr4) - (ljb * invr4 * invr2);
+    if (FORCES) {
+      fmag = ((6.0f * ljb) - (12.0f * lja * invr4 * invr2)) * invr4 * invr4;
+      fmag -= qiqj * invr * invr2;
+    }
+  }
+  if (FORCES) {
+    const long long ix = fixed_contrib(fmag * d.x, frc_bits);
+    const long long iy = fixed_contrib(fmag * d.y, frc_bits);
+    const long long iz = fixed_contrib(fmag * d.z, frc_bits);
     v.add(i, ix, iy, iz);
     v.add(l, -ix, -iy, -iz);
   }
@@ -1336,6 +1477,7 @@ __device__ void valence_term(StageCtx<T> &c, const MMKit &k, int t) {
   v.cz = c.L.cz;
   v.gidx = c.L.gidx;
   v.crd_scale = static_cast<double>(c.crd_scale);
+  v.crd_scale_f = static_cast<float>(c.crd_scale);
   v.fx = c.L.fx;
   v.fy = c.L.fy;
   v.fz = c.L.fz;
@@ -1353,6 +1495,23 @@ __device__ void valence_term(StageCtx<T> &c, const MMKit &k, int t) {
   else if (kind == kVAngle) {
     const double2 p = k.angl_par[vt.w];
     lds_add(&nrg[kAngle], fix64(eval_angle<FORCES>(v, i, j, kk, p.x, p.y, fscale) * nrg_scale));
+  }
+  else if (std::is_same<T, float>::value) {
+    const DiheParam p = k.dihe_par[vt.w];
+    const float e = eval_dihedral_f<FORCES>(v, i, j, kk, l, p, k.frc_bits);
+    lds_add(&nrg[p.improper ? kImproperDihedral : kProperDihedral],
+            fix64(static_cast<double>(e) * nrg_scale));
+    const double2 a14 = k.dihe14[vt.w];
+    if (a14.x > 0.0) {
+      float ele, vdw;
+      const bool con = CLASH && (c.L.clash_on[slot] != 0);
+      eval_14_f<FORCES, CLASH>(v, k, c.wu.lj_table != 0, make_int2(ws.lj_off, ws.lj_nt), i, l,
+                               static_cast<float>(a14.x), static_cast<float>(a14.y), k.frc_bits,
+                               con, con ? static_cast<float>(c.L.clash_d[slot]) : 0.0f,
+                               con ? static_cast<float>(c.L.clash_r[slot]) : 0.0f, ele, vdw);
+      lds_add(&nrg[kElec14], fix64(static_cast<double>(ele) * nrg_scale));
+      lds_add(&nrg[kVdw14], fix64(static_cast<double>(vdw) * nrg_scale));
+    }
   }
   else {
     const DiheParam p = k.dihe_par[vt.w];
@@ -1397,6 +1556,7 @@ __device__ void stage_restraints(StageCtx<T> &c, const MMKit &k) {
   v.cz = c.L.cz;
   v.gidx = c.L.gidx;
   v.crd_scale = static_cast<double>(c.crd_scale);
+  v.crd_scale_f = static_cast<float>(c.crd_scale);
   v.fx = c.L.fx;
   v.fy = c.L.fy;
   v.fz = c.L.fz;

@@ -516,13 +516,21 @@ __device__ void stage_radii(StageCtx<T> &c, const MMKit &k, double coulomb_inv_s
     }
     else {
       const double fipsi = psi * (-rho);
-      R = 1.0 / ((1.0 / rho) - tanh((k.gb_alpha[g] - (k.gb_beta[g] * fipsi) +
-                                     (k.gb_gamma[g] * fipsi * fipsi)) * fipsi) / pbr);
+      const double al = k.gb_alpha[g], be = k.gb_beta[g], ga = k.gb_gamma[g];
+      const double thi = tanh((al - (be * fipsi) + (ga * fipsi * fipsi)) * fipsi);
+      R = 1.0 / ((1.0 / rho) - thi / pbr);
+      if (FORCES) {
+        // dR/dpsi of the OBC model (nonbonded_potential.tpp:985-1017), kept for the chain rule
+        // in the accumulator slot whose psi is no longer needed.
+        const double dr_dpsi = (al - ((2.0 * be) - (3.0 * ga * fipsi)) * fipsi) *
+                               (1.0 - thi * thi) * rho / pbr;
+        L.psi[a] = __double_as_longlong(dr_dpsi);
+      }
     }
     L.rad[a] = static_cast<T>(R);
     // q holds charge * sqrt(Coulomb), so q^2 = Coulomb * charge^2.
     const double qg = k.q[g];
-    const double expmkf = exp(-kGbKscale * k.kappa * R) * inv_diel;
+    const double expmkf = ((k.kappa != 0.0) ? exp(-kGbKscale * k.kappa * R) : 1.0) * inv_diel;
     const double dielfac = 1.0 - expmkf;
     const double atmq2h = 0.5 * qg * qg;
     const double atmqd2h = atmq2h * dielfac;
@@ -803,13 +811,8 @@ __device__ void stage_chain(StageCtx<T> &c, const MMKit &k) {
     }
     double sdi = static_cast<double>(L.dedr[a]) * ldexp(1.0, -kDedrBits);
     if (k.igb != 1) {
-      const double psi = static_cast<double>(L.psi[a]) * ldexp(1.0, -kPsiBits);
-      const double rho = k.rho[g];
-      const double fipsi = psi * (-rho);
-      const double al = k.gb_alpha[g], be = k.gb_beta[g], ga = k.gb_gamma[g];
-      const double thi = tanh((al - (be - (ga * fipsi)) * fipsi) * fipsi);
-      sdi *= (al - ((2.0 * be) - (3.0 * ga * fipsi)) * fipsi) * (1.0 - thi * thi) * rho /
-             k.pbr[g];
+      // The radii stage left dR/dpsi in the psi slot.
+      sdi *= __longlong_as_double(L.psi[a]);
     }
     L.sd[a] = static_cast<T>(sdi);
   }
@@ -850,7 +853,8 @@ __device__ __forceinline__ void deriv_drain(StageCtx<T> &c, const unsigned *q, i
 // with the same far-field / queued near-field split as stage_psi.  Far-field forces are summed
 // per lane in the working precision and converted to fixed precision once per task.
 template <typename T>
-__device__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
+// Always inlined: kept whole inside the kernels that evaluate forces.
+__device__ __forceinline__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
   using P = Prec<T>;
   WuLds<T> &L = c.L;
   const int il = c.tid & 15;
@@ -1626,7 +1630,7 @@ __device__ void evaluate_wu(StageCtx<T> &c, const MMKit &k, int terms) {
 // Results out: forces added to the global fixed-precision arrays (one owning block per atom)
 // and energies added to the ScoreCard rows of the block's systems.
 template <typename T, bool FORCES>
-__device__ void stage_store(StageCtx<T> &c, const MMKit &k, bool eval_energy) {
+__device__ __forceinline__ void stage_store(StageCtx<T> &c, const MMKit &k, bool eval_energy) {
   WuLds<T> &L = c.L;
   if (FORCES) {
     for (int a = c.tid; a < c.wu.lds_atoms; a += c.nthr) {

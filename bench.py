@@ -99,7 +99,11 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu):
     for persistent in (1, 0, 2):
         S.launch_minimization("single", wag, wps, S.minimize_controls(3, 1, dx0, 0.0), wsc, wlm,
                               persistent=persistent)
-    whole = [setup(topos, coords)]
+    # The persistent kernel keeps each unit's block for the whole run, so it wants the units to
+    # fit the device in as few passes as possible: the default shape (512-atom units, two per
+    # CU), not the one tuned for single evaluations.
+    persistent_shape = {"wu_atom_capacity": 512, "target_wu_per_cu": 2, "block_threads": 512}
+    whole = [setup(topos, coords, persistent_shape)]
     t_pers = run(whole, True)
     e_pers = whole[0][2].potential_energies()
     whole = [setup(topos, coords)]
@@ -114,12 +118,11 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu):
     for lo, hi in zip(edges[:-1], edges[1:]):
         idx = [i for i, t in enumerate(topos) if lo <= t.natom < hi]
         if idx:
-            # Each class gets the shape tuned for it.
             sub_t, sub_c = [topos[i] for i in idx], [coords[i] for i in idx]
-            parts.append(setup(sub_t, sub_c, S.autotune(sub_t, "single", sub_c)[0]))
+            parts.append(setup(sub_t, sub_c, persistent_shape))
     t_class = run(parts, True)
     out = {"systems": nsys, "cycles": cycles, "sd_cycles": sd, "precision": "fp32",
-           "autotuned_shape": shape,
+           "autotuned_shape": shape, "persistent_shape": persistent_shape,
            "persistent_s": t_pers, "per_size_class_s": t_class, "per_step_launches_s": t_step,
            "graph_replay_s": t_graph,
            "minimizations_per_s": nsys / min(t_pers, t_graph),

@@ -32,7 +32,7 @@ template <typename T> struct StageCtx {
 // ------------------------------------------------------------------------------------------
 // Work-unit set-up: system table, per-atom parameters, and the coordinate origins.
 template <typename T>
-__device__ void stage_load_params(StageCtx<T> &c, const MMKit &k) {
+__device__ __forceinline__ void stage_load_params(StageCtx<T> &c, const MMKit &k) {
   WuLds<T> &L = c.L;
   const WuDesc &wu = c.wu;
   if (c.tid < wu.sys_count) {
@@ -1587,7 +1587,7 @@ __device__ void stage_restraints(StageCtx<T> &c, const MMKit &k) {
 // forces (FORCES) and energies in the LDS accumulators.  RESTRAINTS = false leaves the restraint
 // stage out of the instantiation (for callers that never see restrained work units).
 template <typename T, bool FORCES, bool CLASH, bool RESTRAINTS = true>
-__device__ void evaluate_wu(StageCtx<T> &c, const MMKit &k, int terms) {
+__device__ __forceinline__ void evaluate_wu(StageCtx<T> &c, const MMKit &k, int terms) {
   const bool lj_table = (c.wu.lj_table != 0);
   stage_load_coords<T, CLASH>(c, k);
   // With GB the valence terms ride along with the Born-integral tiles.

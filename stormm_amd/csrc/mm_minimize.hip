@@ -289,7 +289,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 template <typename T>
-__device__ void apply_moves(StageCtx<T> &c, const MMKit &k, const MinLds *M) {
+__device__ __forceinline__ void apply_moves(StageCtx<T> &c, const MMKit &k, const MinLds *M) {
   for (int a = c.tid; a < c.wu.lds_atoms; a += c.nthr) {
     const int g = c.L.gidx[a];
     if (g < 0) {

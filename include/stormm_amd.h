@@ -227,6 +227,21 @@ int stormm_phasespace_download_forces(const stormm_phasespace *ps, double *fxyz)
 /* Raw fixed-precision copies, [sum atoms][3] int64 */
 int stormm_phasespace_upload_fixed(stormm_phasespace *ps, const int64_t *xyz);
 int stormm_phasespace_download_fixed(const stormm_phasespace *ps, int64_t *xyz, int64_t *fxyz);
+/* Systems [lo, hi) only: PhaseSpaceSynthesis::upload / download(kind, lower, upper, gpu)
+ * (src/Synthesis/phasespace_synthesis.h:649-672, .cpp:2327-2333,2379-2385).  xyz / fxyz hold
+ * those systems' atoms, [atoms][3] double, system after system; fxyz may be NULL.  Invalid or
+ * empty ranges return STORMM_ERR_INVALID_ARGUMENT.  Synchronous. */
+int stormm_phasespace_upload_systems(stormm_phasespace *ps, int32_t lo, int32_t hi,
+                                     const double *xyz);
+int stormm_phasespace_download_systems(const stormm_phasespace *ps, int32_t lo, int32_t hi,
+                                       double *xyz, double *fxyz);
+/* One system: PhaseSpaceSynthesis::importSystem (phasespace_synthesis.h:917-923,
+ * .cpp:3410-3436) written straight to the device, and exportSystem (phasespace_synthesis.h:698,
+ * .cpp:2472-) read straight from it (coordinates and forces; xyz or fxyz may be NULL). */
+int stormm_phasespace_import_system(stormm_phasespace *ps, int32_t system_index,
+                                    const double *xyz);
+int stormm_phasespace_export_system(const stormm_phasespace *ps, int32_t system_index,
+                                    double *xyz, double *fxyz);
 int stormm_phasespace_writer(stormm_phasespace *ps, stormm_ps_writer *w);
 int stormm_phasespace_initialize_forces(stormm_phasespace *ps, void *stream);
 int stormm_phasespace_prime_conjugate_gradient(stormm_phasespace *ps, void *stream);

@@ -116,6 +116,10 @@ _SIGS = {
     "stormm_phasespace_download_forces": (c_int32, [vp, vp]),
     "stormm_phasespace_upload_fixed": (c_int32, [vp, vp]),
     "stormm_phasespace_download_fixed": (c_int32, [vp, vp, vp]),
+    "stormm_phasespace_upload_systems": (c_int32, [vp, c_int32, c_int32, vp]),
+    "stormm_phasespace_download_systems": (c_int32, [vp, c_int32, c_int32, vp, vp]),
+    "stormm_phasespace_import_system": (c_int32, [vp, c_int32, vp]),
+    "stormm_phasespace_export_system": (c_int32, [vp, c_int32, vp, vp]),
     "stormm_phasespace_writer": (c_int32, [vp, P(PsWriter)]),
     "stormm_phasespace_initialize_forces": (c_int32, [vp, vp]),
     "stormm_phasespace_prime_conjugate_gradient": (c_int32, [vp, vp]),

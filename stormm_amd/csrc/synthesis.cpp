@@ -375,6 +375,99 @@ extern "C" int stormm_phasespace_download_forces(const stormm_phasespace *ps, do
   return STORMM_OK;
 }
 
+// Systems [lo, hi) occupy one contiguous span of device atoms, system after system with each
+// system padded to the wave width; the helpers below move that span in one copy per axis.
+static int system_span(const stormm_phasespace *ps, int lo, int hi, int *begin, int *end) {
+  if (ps == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  const SynthesisPlan &P = ps->ag->plan;
+  if (lo < 0 || hi > P.nsys || lo >= hi) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT,
+                "system range [" + std::to_string(lo) + ", " + std::to_string(hi) +
+                ") is empty or outside the synthesis's " + std::to_string(P.nsys) + " systems");
+  }
+  *begin = P.atom_starts[lo];
+  *end = P.atom_starts[hi - 1] + P.atom_counts[hi - 1];
+  return STORMM_OK;
+}
+
+extern "C" int stormm_phasespace_upload_systems(stormm_phasespace *ps, int32_t lo, int32_t hi,
+                                                const double *xyz) {
+  int b = 0, e = 0;
+  int rc = system_span(ps, lo, hi, &b, &e);
+  if (rc != STORMM_OK) {
+    return rc;
+  }
+  if (xyz == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null coordinates");
+  }
+  const SynthesisPlan &P = ps->ag->plan;
+  const double scale = std::ldexp(1.0, ps->gpos_bits);
+  std::vector<long long> hx(e - b, 0), hy(e - b, 0), hz(e - b, 0);
+  size_t c = 0;
+  for (int s = lo; s < hi; s++) {
+    for (int i = 0; i < P.atom_counts[s]; i++) {
+      const int g = P.atom_starts[s] + i - b;
+      hx[g] = std::llround(xyz[3 * c] * scale);
+      hy[g] = std::llround(xyz[3 * c + 1] * scale);
+      hz[g] = std::llround(xyz[3 * c + 2] * scale);
+      c++;
+    }
+  }
+  const size_t n = hx.size() * sizeof(long long);
+  HIP_TRY(hipMemcpy(ps->xcrd.as<long long>() + b, hx.data(), n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ps->ycrd.as<long long>() + b, hy.data(), n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ps->zcrd.as<long long>() + b, hz.data(), n, hipMemcpyHostToDevice));
+  return STORMM_OK;
+}
+
+extern "C" int stormm_phasespace_download_systems(const stormm_phasespace *ps, int32_t lo,
+                                                  int32_t hi, double *xyz, double *fxyz) {
+  int b = 0, e = 0;
+  int rc = system_span(ps, lo, hi, &b, &e);
+  if (rc != STORMM_OK) {
+    return rc;
+  }
+  const SynthesisPlan &P = ps->ag->plan;
+  const size_t n = static_cast<size_t>(e - b) * sizeof(long long);
+  std::vector<long long> a(e - b), bb(e - b), cc(e - b);
+  for (int pass = 0; pass < 2; pass++) {
+    double *dst = (pass == 0) ? xyz : fxyz;
+    if (dst == nullptr) {
+      continue;
+    }
+    const DevMem &dx = (pass == 0) ? ps->xcrd : ps->xfrc;
+    const DevMem &dy = (pass == 0) ? ps->ycrd : ps->yfrc;
+    const DevMem &dz = (pass == 0) ? ps->zcrd : ps->zfrc;
+    HIP_TRY(hipMemcpy(a.data(), dx.as<long long>() + b, n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bb.data(), dy.as<long long>() + b, n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cc.data(), dz.as<long long>() + b, n, hipMemcpyDeviceToHost));
+    const double inv = std::ldexp(1.0, (pass == 0) ? -ps->gpos_bits : -ps->frc_bits);
+    size_t k = 0;
+    for (int s = lo; s < hi; s++) {
+      for (int i = 0; i < P.atom_counts[s]; i++) {
+        const int g = P.atom_starts[s] + i - b;
+        dst[3 * k] = static_cast<double>(a[g]) * inv;
+        dst[3 * k + 1] = static_cast<double>(bb[g]) * inv;
+        dst[3 * k + 2] = static_cast<double>(cc[g]) * inv;
+        k++;
+      }
+    }
+  }
+  return STORMM_OK;
+}
+
+extern "C" int stormm_phasespace_import_system(stormm_phasespace *ps, int32_t system_index,
+                                               const double *xyz) {
+  return stormm_phasespace_upload_systems(ps, system_index, system_index + 1, xyz);
+}
+
+extern "C" int stormm_phasespace_export_system(const stormm_phasespace *ps, int32_t system_index,
+                                               double *xyz, double *fxyz) {
+  return stormm_phasespace_download_systems(ps, system_index, system_index + 1, xyz, fxyz);
+}
+
 extern "C" int stormm_phasespace_writer(stormm_phasespace *ps, stormm_ps_writer *w) {
   if (ps == nullptr || w == nullptr) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");

@@ -295,6 +295,46 @@ class PhaseSpaceSynthesis:
         check(self._L.stormm_phasespace_download_fixed(self.handle, _ptr(c), _ptr(f)))
         return self._split(c), self._split(f)
 
+    def _range_atoms(self, lo: int, hi: int) -> int:
+        counts = self.ag.atom_counts
+        if not (0 <= lo < hi <= len(counts)):
+            raise ValueError(f"system range [{lo}, {hi}) is empty or outside the synthesis's "
+                             f"{len(counts)} systems")
+        return int(sum(counts[lo:hi]))
+
+    def upload_systems(self, lo: int, hi: int, coords) -> None:
+        """Coordinates of systems [lo, hi) only (PhaseSpaceSynthesis::upload(kind, lower, upper),
+        src/Synthesis/phasespace_synthesis.cpp:2327-2333); the other systems keep theirs."""
+        n = self._range_atoms(lo, hi)
+        if isinstance(coords, (list, tuple)):
+            coords = np.concatenate([np.asarray(c, dtype=np.float64).reshape(-1, 3)
+                                     for c in coords])
+        a = np.ascontiguousarray(np.asarray(coords, dtype=np.float64).reshape(-1, 3))
+        if a.shape[0] != n:
+            raise ValueError(f"systems [{lo}, {hi}) hold {n} atoms, got {a.shape[0]}")
+        check(self._L.stormm_phasespace_upload_systems(self.handle, lo, hi, _ptr(a)))
+
+    def download_systems(self, lo: int, hi: int):
+        """Coordinates and forces of systems [lo, hi) (PhaseSpaceSynthesis::download(kind, lower,
+        upper), phasespace_synthesis.cpp:2379-2385), one array per system."""
+        n = self._range_atoms(lo, hi)
+        c = np.zeros((n, 3))
+        f = np.zeros((n, 3))
+        check(self._L.stormm_phasespace_download_systems(self.handle, lo, hi, _ptr(c), _ptr(f)))
+        cuts = np.cumsum(self.ag.atom_counts[lo:hi])[:-1]
+        return np.split(c, cuts), np.split(f, cuts)
+
+    def import_system(self, system_index: int, coords) -> None:
+        """PhaseSpaceSynthesis::importSystem (phasespace_synthesis.cpp:3410-3436): one system's
+        coordinates, written to the device."""
+        self.upload_systems(system_index, system_index + 1, coords)
+
+    def export_system(self, system_index: int):
+        """PhaseSpaceSynthesis::exportSystem (phasespace_synthesis.cpp:2472-): one system's
+        coordinates and forces, read from the device."""
+        c, f = self.download_systems(system_index, system_index + 1)
+        return c[0], f[0]
+
     def initialize_forces(self, stream=None) -> None:
         check(self._L.stormm_phasespace_initialize_forces(self.handle, _stream_ptr(stream)))
 

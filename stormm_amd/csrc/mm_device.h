@@ -55,8 +55,20 @@ template <> struct Prec<float> {
 
 template <> struct Prec<double> {
   using crd_t = double;
-  static __device__ __forceinline__ double rsqrt(double x) { return 1.0 / ::sqrt(x); }
-  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+  // The hardware estimates refined by two Newton-Raphson steps, y <- y + y (1 - x y) for the
+  // reciprocal and y <- y + (y / 2) (1 - x y^2) for the reciprocal square root: each step about
+  // squares the estimate's relative error, so two reach the last few bits of a double.  For
+  // positive, finite, normal arguments (distances and radii here).
+  static __device__ __forceinline__ double rsqrt(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+    return fma(0.5 * y, fma(-x * y, y, 1.0), y);
+  }
+  static __device__ __forceinline__ double rcp(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = fma(y, fma(-x, y, 1.0), y);
+    return fma(y, fma(-x, y, 1.0), y);
+  }
   static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
   static __device__ __forceinline__ double cbrt(double x) { return ::cbrt(x); }
   static __device__ __forceinline__ double log(double x) { return ::log(x); }

@@ -285,7 +285,7 @@ __device__ __forceinline__ float gb_efac(float r2, float rr) {
                                 (-0.25f * 1.44269504088896340736f));
 }
 __device__ __forceinline__ double gb_efac(double r2, double rr) {
-  return ::exp(-0.25 * r2 * (1.0 / rr));
+  return ::exp(-0.25 * r2 * Prec<double>::rcp(rr));
 }
 
 // Coordinates of a pair tile's atoms in Angstroms in the working precision.  Single precision

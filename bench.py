@@ -70,13 +70,14 @@ def parity_errors(S, oracle, topos, coords, prec, gpos, frc, nrg):
             "force_rel_rms": worst_f}
 
 
-def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu):
+def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu, shape=None):
     """FP32 minimization of the 4096-system batch: three launch strategies and the CPU rate."""
     topos, coords = synthetic.config2(nsys=nsys, seed=2)
     sd, dx0 = 50, 0.01
     mc = S.minimize_controls(cycles, sd, dx0, 0.0)
-    # Work-unit shape from the autotuner (timed force evaluations of the whole batch).
-    shape, tuned_ms = S.autotune(topos, "single", coords)
+    if shape is None:
+        # Work-unit shape from the autotuner (timed force evaluations of the whole batch).
+        shape, tuned_ms = S.autotune(topos, "single", coords)
 
     def setup(sub_t, sub_c, sub_shape=None):
         ag = S.AtomGraphSynthesis(sub_t, sub_shape or shape)
@@ -193,6 +194,9 @@ def main():
     ap.add_argument("--min-cycles", type=int, default=500)
     ap.add_argument("--serve-seconds", type=float, default=5.0,
                     help="arrival window of the served-minimization stream")
+    ap.add_argument("--minimize-only", metavar="CAPACITY,PER_CU,THREADS", default=None,
+                    help="run only the minimization workload on the given work-unit shape (no "
+                         "autotuning, no CPU baseline) and print its JSON block; for profiling")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this path runs every system of a batch on a single "
@@ -202,6 +206,13 @@ def main():
 
     from stormm_amd import roofline, synthetic
     from stormm_amd import synthesis as S
+
+    if args.minimize_only:
+        cap, per_cu, threads = (int(v) for v in args.minimize_only.split(","))
+        shape = {"wu_atom_capacity": cap, "target_wu_per_cu": per_cu, "block_threads": threads}
+        res = minimization_workload(S, synthetic, None, 4096, args.min_cycles, False, shape)
+        print(json.dumps({"minimization": res}), flush=True)
+        return
 
     log(f"device: {S.device_info()}")
     t0 = time.time()

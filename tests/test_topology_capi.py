@@ -23,6 +23,20 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert missing == []
 
 
+def test_system_transfers_reject_missing_handles_without_a_device():
+    # The per-system and ranged phase-space transfers check their arguments on the host
+    # before any device call.
+    lib = _lib.load()
+    buf = np.zeros((16, 3))
+    calls = [lambda: lib.stormm_phasespace_upload_systems(None, 0, 1, buf.ctypes.data),
+             lambda: lib.stormm_phasespace_download_systems(None, 0, 1, buf.ctypes.data, None),
+             lambda: lib.stormm_phasespace_import_system(None, 0, buf.ctypes.data),
+             lambda: lib.stormm_phasespace_export_system(None, 0, buf.ctypes.data, None)]
+    for call in calls:
+        assert call() != 0
+        assert b"null" in lib.stormm_last_error()
+
+
 def test_prmtop_round_trip_preserves_energies(tmp_path):
     topo, xyz = synthetic.branched_molecule(37, seed=3, gb=True)
     top = os.path.join(tmp_path, "mol.top")

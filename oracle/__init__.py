@@ -29,10 +29,16 @@ SLOT_NAMES = ["BOND", "ANGLE", "PROPER_DIHEDRAL", "IMPROPER_DIHEDRAL", "UREY_BRA
 SLOT = {name: k for k, name in enumerate(SLOT_NAMES)}
 
 
+# Optimised for the x86-64-v3 level (AVX2 + FMA), which every current x86 server core has, so
+# that the library built here also runs on the host of the GPU machine; floating-point
+# contraction stays off so that the arithmetic is exactly the source's on any machine.
+CFLAGS = ["-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fPIC", "-shared", "-std=c11",
+          "-D_GNU_SOURCE"]
+
+
 def build(verbose: bool = False) -> str:
     """Compile the oracle with gcc into oracle/liboracle.so."""
-    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-D_GNU_SOURCE", "-o", _LIB_PATH] + \
-        _SOURCES + ["-lm", "-lpthread"]
+    cmd = ["gcc"] + CFLAGS + ["-o", _LIB_PATH] + _SOURCES + ["-lm", "-lpthread"]
     if verbose:
         print(" ".join(cmd))
     subprocess.check_call(cmd)
@@ -119,6 +125,10 @@ def lib():
                                    ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_int, vp]
         L.orc_minimize.restype = None
+        L.orc_minimize_record.argtypes = L.orc_minimize.argtypes + [vp]
+        L.orc_minimize_record.restype = None
+        L.orc_plant_mistake.argtypes = [ctypes.c_int]
+        L.orc_plant_mistake.restype = None
         _lib = L
     return _lib
 
@@ -287,23 +297,45 @@ def cubic_move(evec: Sequence[float], mvec: Sequence[float]):
     return mf, nbm.value, coefs
 
 
+# Columns of the per-cycle line-search record of minimize(..., record=True).
+LINE_RECORD = ["nrg_a", "nrg_b", "nrg_c", "nrg_d", "mfac_b", "mfac_c", "move_factor", "l_move",
+               "gamma"]
+
+# Deliberate mistakes a test may plant in the minimizer (plant_mistake), stormm_oracle.h.
+MISTAKES = {"fletcher_reeves": 1, "no_conjugation": 2, "gamma_sign": 3, "first_growth": 4,
+            "first_shrink": 5, "second_growth": 6, "normalization_rounding": 7}
+
+
+def plant_mistake(name: Optional[str]) -> None:
+    """Make the oracle's minimizer commit one deliberate mistake (None: none).  Tests only: a
+    check that still passes with the mistake planted does not pin what it claims to pin."""
+    lib().orc_plant_mistake(0 if name is None else MISTAKES[name])
+
+
 def minimize(topo, xyz_fixed: np.ndarray, gpos_bits: int, frc_bits: int, nrg_bits: int,
              nsteps: int, sd_cycles: int = 50, dx0: float = 0.01, dp_mode: bool = False,
              handle: Optional[SystemHandle] = None, clash_cycles: int = 0,
-             clash_r0: float = 0.2, clash_ratio: float = 0.5, restraints=None):
-    """GPU-semantics minimization on fixed-point coordinates; returns (coords, energy trace).
-    The first clash_cycles cycles run with clash damping shrinking from (clash_r0,
-    clash_ratio) to zero; restraints follow the minimizer's step counters."""
+             clash_r0: float = 0.2, clash_ratio: float = 0.5, restraints=None,
+             record: bool = False):
+    """GPU-semantics minimization on fixed-point coordinates; returns (coords, energy trace)
+    and, with record=True, also the per-cycle line-search record ([nsteps, 9], columns
+    LINE_RECORD).  The first clash_cycles cycles run with clash damping shrinking from
+    (clash_r0, clash_ratio) to zero; restraints follow the minimizer's step counters."""
     h = handle or SystemHandle(topo, restraints)
     c = np.array(xyz_fixed, dtype=np.int64).reshape(-1, 3)
     x = np.ascontiguousarray(c[:, 0])
     y = np.ascontiguousarray(c[:, 1])
     z = np.ascontiguousarray(c[:, 2])
     trace = np.zeros(nsteps + 1)
-    lib().orc_minimize(ctypes.byref(h.struct), _ptr(x), _ptr(y), _ptr(z), gpos_bits, frc_bits,
-                       nrg_bits, nsteps, sd_cycles, dx0, int(clash_cycles), float(clash_r0),
-                       float(clash_ratio), int(dp_mode), _ptr(trace))
-    return np.stack([x, y, z], axis=1), trace
+    rec = np.zeros((max(nsteps, 1), len(LINE_RECORD)))
+    lib().orc_minimize_record(ctypes.byref(h.struct), _ptr(x), _ptr(y), _ptr(z), gpos_bits,
+                              frc_bits, nrg_bits, nsteps, sd_cycles, dx0, int(clash_cycles),
+                              float(clash_r0), float(clash_ratio), int(dp_mode), _ptr(trace),
+                              _ptr(rec))
+    out = np.stack([x, y, z], axis=1)
+    if record:
+        return out, trace, rec[:nsteps]
+    return out, trace
 
 
 class Batch:

@@ -1366,10 +1366,19 @@ static int restraint_step(int i, int move, int clash_cycles) {
   return (3 * j) + move - 1;
 }
 
-void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long long *zcrd,
-                  int gpos_bits, int frc_bits, int nrg_bits, int nsteps, int sd_cycles,
-                  double dx0, int clash_cycles, double clash_r0, double clash_ratio,
-                  int dp_mode, double *energy_trace) {
+/* Deliberate mistakes that tests plant in the minimizer to show that their checks catch them
+ * (0: none).  The product never calls this; it exists only so that a test can prove a check
+ * sensitive to the mistake it is meant to exclude. */
+static int planted_mistake = 0;
+
+void orc_plant_mistake(int code) {
+  planted_mistake = code;
+}
+
+void orc_minimize_record(const orc_system *s, long long *xcrd, long long *ycrd, long long *zcrd,
+                         int gpos_bits, int frc_bits, int nrg_bits, int nsteps, int sd_cycles,
+                         double dx0, int clash_cycles, double clash_r0, double clash_ratio,
+                         int dp_mode, double *energy_trace, double *line_record) {
   const int n = s->natom;
   long long *buf = (long long *)calloc((size_t)9 * n, sizeof(long long));
   long long *fx = buf, *fy = buf + n, *fz = buf + 2 * n;
@@ -1409,7 +1418,21 @@ void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long lo
         dgg += dd * df;
       }
     }
-    const double gam = (step_counter >= sd_cycles) ? (dgg / gg) : 0.0;
+    if (planted_mistake == ORC_MISTAKE_FLETCHER_REEVES) {
+      dgg = 0.0;
+      for (int d = 0; d < 3; d++) {
+        for (int i = 0; i < n; i++) {
+          dgg += (double)frc[d][i] * (double)frc[d][i];
+        }
+      }
+    }
+    double gam = (step_counter >= sd_cycles) ? (dgg / gg) : 0.0;
+    if (planted_mistake == ORC_MISTAKE_NO_CONJUGATION) {
+      gam = 0.0;
+    }
+    else if (planted_mistake == ORC_MISTAKE_GAMMA_SIGN) {
+      gam = -gam;
+    }
     double msum = 0.0;
     for (int d = 0; d < 3; d++) {
       for (int i = 0; i < n; i++) {
@@ -1433,7 +1456,9 @@ void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long lo
     for (int d = 0; d < 3; d++) {
       for (int i = 0; i < n; i++) {
         const double v = (double)frc[d][i] * inv_factor;
-        frc[d][i] = dp_mode ? d2ll_rn(v) : (long long)v;
+        const int round_nearest = (planted_mistake == ORC_MISTAKE_NORMALIZATION_ROUNDING) ?
+                                  !dp_mode : dp_mode;
+        frc[d][i] = round_nearest ? d2ll_rn(v) : (long long)v;
       }
     }
 
@@ -1458,12 +1483,15 @@ void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long lo
       }
       else if (move == 1) {
         nrg_b = cur;
-        move_factor = (cur < nrg_a) ? 1.04 : 0.8;
+        move_factor = (cur < nrg_a) ? ((planted_mistake == ORC_MISTAKE_FIRST_GROWTH) ? 1.05 : 1.04)
+                                    : ((planted_mistake == ORC_MISTAKE_FIRST_SHRINK) ? 0.75 : 0.8);
         mfac_b = move_factor;
       }
       else if (move == 2) {
         nrg_c = cur;
-        move_factor = (cur < nrg_b) ? mfac_b * 1.03 : mfac_b * 0.85;
+        move_factor = (cur < nrg_b)
+                          ? mfac_b * ((planted_mistake == ORC_MISTAKE_SECOND_GROWTH) ? 1.05 : 1.03)
+                          : mfac_b * 0.85;
         mfac_c = move_factor;
       }
       else {
@@ -1473,6 +1501,18 @@ void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long lo
         double nbm;
         move_factor = orc_cubic_move(evec, mvec, &nbm, NULL);
         l_move = s_move * nbm;
+        if (line_record != NULL) {
+          double *rec = line_record + ((size_t)it * ORC_LINE_RECORD_LENGTH);
+          rec[0] = nrg_a;
+          rec[1] = nrg_b;
+          rec[2] = nrg_c;
+          rec[3] = nrg_d;
+          rec[4] = mfac_b;
+          rec[5] = mfac_c;
+          rec[6] = move_factor;
+          rec[7] = l_move;
+          rec[8] = gam;
+        }
       }
       if (fabs(move_factor) > 1.0e-12) {
         move_factor *= inv_frc_scale * gpos_scale;
@@ -1496,4 +1536,12 @@ void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long lo
   (void)nrg_c;
   free(buf);
   free(work);
+}
+
+void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long long *zcrd,
+                  int gpos_bits, int frc_bits, int nrg_bits, int nsteps, int sd_cycles,
+                  double dx0, int clash_cycles, double clash_r0, double clash_ratio,
+                  int dp_mode, double *energy_trace) {
+  orc_minimize_record(s, xcrd, ycrd, zcrd, gpos_bits, frc_bits, nrg_bits, nsteps, sd_cycles, dx0,
+                      clash_cycles, clash_r0, clash_ratio, dp_mode, energy_trace, NULL);
 }

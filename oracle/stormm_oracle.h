@@ -162,6 +162,29 @@ void orc_minimize(const orc_system *s, long long *xcrd, long long *ycrd, long lo
                   double dx0, int clash_cycles, double clash_r0, double clash_ratio,
                   int dp_mode, double *energy_trace);
 
+/* orc_minimize, also writing for each cycle ORC_LINE_RECORD_LENGTH doubles of the line search
+ * (the LineMinimization record, src/MolecularMechanics/line_minimization.h:85-221): the four
+ * energies nrg_a..nrg_d, the move factors of moves 1 and 2 (mfac_b, mfac_c), the move factor of
+ * the cubic-fit move, the baseline move length for the next cycle and the conjugate-gradient
+ * gamma used this cycle.  line_record may be NULL. */
+#define ORC_LINE_RECORD_LENGTH 9
+void orc_minimize_record(const orc_system *s, long long *xcrd, long long *ycrd, long long *zcrd,
+                         int gpos_bits, int frc_bits, int nrg_bits, int nsteps, int sd_cycles,
+                         double dx0, int clash_cycles, double clash_r0, double clash_ratio,
+                         int dp_mode, double *energy_trace, double *line_record);
+
+/* Test hook: make the minimizer commit one deliberate mistake (or none, code 0) so that a test
+ * can show that its checks fail on that mistake.  Not thread-safe; tests only. */
+#define ORC_MISTAKE_NONE 0
+#define ORC_MISTAKE_FLETCHER_REEVES 1      /* gamma = |f|^2 / |f_prev|^2                     */
+#define ORC_MISTAKE_NO_CONJUGATION 2       /* gamma = 0 every cycle                          */
+#define ORC_MISTAKE_GAMMA_SIGN 3           /* gamma with its sign flipped                    */
+#define ORC_MISTAKE_FIRST_GROWTH 4         /* 1.05 in place of 1.04 after an improving move 1 */
+#define ORC_MISTAKE_FIRST_SHRINK 5         /* 0.75 in place of 0.8 after a failing move 1     */
+#define ORC_MISTAKE_SECOND_GROWTH 6        /* 1.05 in place of 1.03 after an improving move 2 */
+#define ORC_MISTAKE_NORMALIZATION_ROUNDING 7 /* round-to-nearest and truncation swapped       */
+void orc_plant_mistake(int code);
+
 #ifdef __cplusplus
 }
 #endif

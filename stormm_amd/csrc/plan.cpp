@@ -612,8 +612,114 @@ int find_or_add_template(const stormm_topology_batch *b, const std::vector<Syste
     return rc;
   }
   *tid = static_cast<int>(ts.templates.size()) - 1;
+  ts.templates.back().hash = h;
   bucket.push_back(*tid);
   return STORMM_OK;
+}
+
+TemplateStoreMark mark_templates(const TemplateStore &ts) {
+  TemplateStoreMark m;
+  m.templates = ts.templates.size();
+  m.tasks = ts.tasks.size();
+  m.masks = ts.masks.size();
+  m.vterms = ts.vterms.size();
+  m.bond_par = ts.bond_par.size();
+  m.angl_par = ts.angl_par.size();
+  m.dihe14 = ts.dihe14.size();
+  m.ljab = ts.ljab.size();
+  m.ljab14 = ts.ljab14.size();
+  m.dihe_par = ts.dihe_par.size();
+  return m;
+}
+
+void rollback_templates(TemplateStore &ts, const TemplateStoreMark &m) {
+  for (size_t t = m.templates; t < ts.templates.size(); t++) {
+    auto it = ts.by_hash.find(ts.templates[t].hash);
+    if (it == ts.by_hash.end()) {
+      continue;
+    }
+    std::vector<int> &bucket = it->second;
+    bucket.erase(std::remove(bucket.begin(), bucket.end(), static_cast<int>(t)), bucket.end());
+    if (bucket.empty()) {
+      ts.by_hash.erase(it);
+    }
+  }
+  ts.templates.resize(m.templates);
+  ts.tasks.resize(m.tasks);
+  ts.masks.resize(m.masks);
+  ts.vterms.resize(m.vterms);
+  ts.bond_par.resize(m.bond_par);
+  ts.angl_par.resize(m.angl_par);
+  ts.dihe14.resize(m.dihe14);
+  ts.ljab.resize(m.ljab);
+  ts.ljab14.resize(m.ljab14);
+  ts.dihe_par.resize(m.dihe_par);
+}
+
+size_t template_store_bytes(const TemplateStore &ts) {
+  return ts.tasks.size() * sizeof(Task) + ts.masks.size() * sizeof(unsigned) +
+         ts.vterms.size() * sizeof(int4) +
+         (ts.bond_par.size() + ts.angl_par.size() + ts.dihe14.size() + ts.ljab.size() +
+          ts.ljab14.size()) * sizeof(double2) +
+         ts.dihe_par.size() * sizeof(DiheParam);
+}
+
+void compact_templates(TemplateStore &ts, const std::vector<char> &keep, std::vector<int> &remap) {
+  TemplateStore out;
+  remap.assign(ts.templates.size(), -1);
+  for (size_t t = 0; t < ts.templates.size(); t++) {
+    if (!keep[t]) {
+      continue;
+    }
+    TopoTemplate tp = ts.templates[t];
+    // Tile tasks and the exclusion masks of those that carry one (kTile words each).
+    const int ntask = tp.offd_count + tp.diag_count;
+    const int task_first = static_cast<int>(out.tasks.size());
+    for (int i = 0; i < ntask; i++) {
+      Task tk = ts.tasks[tp.task_first + i];
+      if (tk.z >= 0) {
+        const int mask_first = static_cast<int>(out.masks.size());
+        out.masks.insert(out.masks.end(), ts.masks.begin() + tk.z,
+                         ts.masks.begin() + tk.z + kTile);
+        tk.z = mask_first;
+      }
+      out.tasks.push_back(tk);
+    }
+    tp.task_first = task_first;
+    // Valence terms with their parameters (a dihedral's 1:4 factors share its index).
+    const int term_first = static_cast<int>(out.vterms.size());
+    for (int i = 0; i < tp.term_count; i++) {
+      int4 v = ts.vterms[tp.term_first + i];
+      if (v.x == kVBond) {
+        out.bond_par.push_back(ts.bond_par[v.w]);
+        v.w = static_cast<int>(out.bond_par.size()) - 1;
+      }
+      else if (v.x == kVAngle) {
+        out.angl_par.push_back(ts.angl_par[v.w]);
+        v.w = static_cast<int>(out.angl_par.size()) - 1;
+      }
+      else {
+        out.dihe_par.push_back(ts.dihe_par[v.w]);
+        out.dihe14.push_back(ts.dihe14[v.w]);
+        v.w = static_cast<int>(out.dihe_par.size()) - 1;
+      }
+      out.vterms.push_back(v);
+    }
+    tp.term_first = term_first;
+    if (!tp.lb) {
+      const size_t nt2 = static_cast<size_t>(tp.lj_nt) * tp.lj_nt;
+      const int lj_off = static_cast<int>(out.ljab.size());
+      out.ljab.insert(out.ljab.end(), ts.ljab.begin() + tp.lj_off,
+                      ts.ljab.begin() + tp.lj_off + nt2);
+      out.ljab14.insert(out.ljab14.end(), ts.ljab14.begin() + tp.lj_off,
+                        ts.ljab14.begin() + tp.lj_off + nt2);
+      tp.lj_off = lj_off;
+    }
+    remap[t] = static_cast<int>(out.templates.size());
+    out.by_hash[tp.hash].push_back(remap[t]);
+    out.templates.push_back(std::move(tp));
+  }
+  ts = std::move(out);
 }
 
 int fill_atom_parameters(const stormm_topology_batch *b, const SystemTerms &t, int natom,

@@ -39,6 +39,7 @@ struct TopoTemplate {
   bool lb = false;        // the LJ interactions follow Lorentz-Berthelot combining
   std::vector<double> sig, eps;
   double cost = 0.0;      // estimated evaluation cost (rotation steps of the pair passes)
+  unsigned long long hash = 0;  // content hash, the key of TemplateStore::by_hash
 };
 
 // Templates and the arrays they index.
@@ -51,6 +52,24 @@ struct TemplateStore {
   std::vector<DiheParam> dihe_par;
   std::unordered_map<unsigned long long, std::vector<int>> by_hash;
 };
+
+// Sizes of a TemplateStore's arrays, to undo the templates added after the mark.
+struct TemplateStoreMark {
+  size_t templates = 0, tasks = 0, masks = 0, vterms = 0, bond_par = 0, angl_par = 0,
+         dihe14 = 0, ljab = 0, ljab14 = 0, dihe_par = 0;
+};
+
+TemplateStoreMark mark_templates(const TemplateStore &ts);
+
+// Removes every template added after `m` (complete or partly built) and its array entries.
+void rollback_templates(TemplateStore &ts, const TemplateStoreMark &m);
+
+// Device bytes of a store's arrays.
+size_t template_store_bytes(const TemplateStore &ts);
+
+// Keeps only the templates with keep[t] != 0, packing their tasks, masks, valence terms and
+// parameters to the front; remap[t] is template t's new index (-1 when dropped).
+void compact_templates(TemplateStore &ts, const std::vector<char> &keep, std::vector<int> &remap);
 
 // GB and electrostatic settings of a synthesis or pool.
 struct SolventModel {

@@ -35,14 +35,6 @@ constexpr size_t kPoolBytesPerSlot = (4 * 4) + 8 + (9 * 8) + (kScoreStride * 8);
 // Share of the budget kept for work templates (tile tasks, masks, valence terms, parameters).
 constexpr double kTemplateShare = 0.2;
 
-size_t template_bytes(const TemplateStore &ts) {
-  return ts.tasks.size() * sizeof(Task) + ts.masks.size() * sizeof(unsigned) +
-         ts.vterms.size() * sizeof(int4) +
-         (ts.bond_par.size() + ts.angl_par.size() + ts.dihe14.size() + ts.ljab.size() +
-          ts.ljab14.size()) * sizeof(double2) +
-         ts.dihe_par.size() * sizeof(DiheParam);
-}
-
 // Upload of a growing host array into a device buffer (reallocated when it grows).
 template <typename T>
 hipError_t sync_array(DevMem &d, const std::vector<T> &h, size_t &uploaded) {
@@ -81,6 +73,7 @@ struct stormm_pool {
   std::map<int, int> free_atoms;        // start -> length (multiples of kGlobalPad)
   std::vector<int> free_slots;
   std::vector<int> slot_start, slot_natom, slot_template, slot_live, slot_table;
+  std::vector<int> template_refs;       // resident systems using each template
   // Each admission takes one contiguous atom range, returned when its last system leaves.
   struct Range {
     int start = 0, len = 0, refs = 0;
@@ -209,7 +202,7 @@ extern "C" int stormm_pool_capacity(const stormm_pool *pool, int64_t *free_atoms
     *free_slots = static_cast<int32_t>(pool->free_slots.size());
   }
   if (template_bytes_used) {
-    *template_bytes_used = static_cast<int64_t>(template_bytes(pool->ts));
+    *template_bytes_used = static_cast<int64_t>(template_store_bytes(pool->ts));
   }
   if (capacity_atoms) {
     *capacity_atoms = pool->cap_atoms;
@@ -249,6 +242,103 @@ static void give_atoms(stormm_pool *p, int start, int len) {
   }
 }
 
+// Uploads the work templates admitted since the last upload (device arrays grow as needed).
+static hipError_t sync_templates(stormm_pool *p) {
+  const hipError_t steps[] = {
+      sync_array(p->tasks, p->ts.tasks, p->up_tasks),
+      sync_array(p->masks, p->ts.masks, p->up_masks),
+      sync_array(p->vterms, p->ts.vterms, p->up_vterms),
+      sync_array(p->bond_par, p->ts.bond_par, p->up_bond),
+      sync_array(p->angl_par, p->ts.angl_par, p->up_angl),
+      sync_array(p->dihe_par, p->ts.dihe_par, p->up_dihe),
+      sync_array(p->dihe14, p->ts.dihe14, p->up_d14),
+      sync_array(p->ljab, p->ts.ljab, p->up_ljab),
+      sync_array(p->ljab14, p->ts.ljab14, p->up_ljab14)};
+  for (hipError_t e : steps) {
+    if (e != hipSuccess) {
+      return e;
+    }
+  }
+  return hipSuccess;
+}
+
+// Whether every template array on the device matches the host store.
+static bool templates_in_sync(const stormm_pool *p) {
+  return p->tasks.ptr != nullptr && p->up_tasks == p->ts.tasks.size() &&
+         p->up_masks == p->ts.masks.size() && p->up_vterms == p->ts.vterms.size() &&
+         p->up_bond == p->ts.bond_par.size() && p->up_angl == p->ts.angl_par.size() &&
+         p->up_dihe == p->ts.dihe_par.size() && p->up_d14 == p->ts.dihe14.size() &&
+         p->up_ljab == p->ts.ljab.size() && p->up_ljab14 == p->ts.ljab14.size();
+}
+
+// The next upload rewrites every template array from the start.
+static void invalidate_template_upload(stormm_pool *p) {
+  p->up_tasks = p->up_masks = p->up_vterms = p->up_bond = p->up_angl = p->up_dihe = 0;
+  p->up_d14 = p->up_ljab = p->up_ljab14 = 0;
+}
+
+// Drops the templates that no resident system uses and renumbers the rest; false when every
+// template is in use.
+static bool drop_unused_templates(stormm_pool *p) {
+  std::vector<char> keep(p->ts.templates.size());
+  bool any_dropped = false;
+  for (size_t t = 0; t < keep.size(); t++) {
+    keep[t] = p->template_refs[t] > 0;
+    any_dropped = any_dropped || !keep[t];
+  }
+  if (!any_dropped) {
+    return false;
+  }
+  std::vector<int> remap;
+  compact_templates(p->ts, keep, remap);
+  std::vector<int> refs(p->ts.templates.size(), 0);
+  for (size_t t = 0; t < remap.size(); t++) {
+    if (remap[t] >= 0) {
+      refs[remap[t]] = p->template_refs[t];
+    }
+  }
+  p->template_refs.swap(refs);
+  for (int s = 0; s < p->cap_slots; s++) {
+    p->slot_template[s] = p->slot_live[s] ? remap[p->slot_template[s]] : -1;
+  }
+  invalidate_template_upload(p);
+  return true;
+}
+
+// Templates of the admitted entries (tids[e]), built or found in the store.  Nothing is kept on
+// failure; when the new templates would exceed the template budget, the templates no resident
+// system uses are dropped and the search runs once more.  `mark` receives the store's state
+// before this admission's templates.
+static int admit_templates(stormm_pool *p, const stormm_topology_batch *batch,
+                           const std::vector<SystemTerms> &st, const std::vector<int> &sys_of,
+                           int batch_id, std::vector<int> &tids, TemplateStoreMark &mark) {
+  for (int attempt = 0; attempt < 2; attempt++) {
+    mark = mark_templates(p->ts);
+    std::vector<int> tid_of_sys(batch->system_count, -1);
+    int rc = STORMM_OK;
+    for (size_t e = 0; e < sys_of.size() && rc == STORMM_OK; e++) {
+      const int s = sys_of[e];
+      if (tid_of_sys[s] < 0) {
+        rc = find_or_add_template(batch, st, s, batch_id, p->ts, &tid_of_sys[s]);
+      }
+      tids[e] = tid_of_sys[s];
+    }
+    if (rc != STORMM_OK) {
+      rollback_templates(p->ts, mark);
+      return rc;
+    }
+    if (template_store_bytes(p->ts) <= p->template_budget) {
+      p->template_refs.resize(p->ts.templates.size(), 0);
+      return STORMM_OK;
+    }
+    rollback_templates(p->ts, mark);
+    if (attempt > 0 || !drop_unused_templates(p)) {
+      break;
+    }
+  }
+  return fail(STORMM_ERR_CAPACITY, "work templates exceed the pool's template budget");
+}
+
 // Admits `count` systems; entry e is system sys_of[e] of the batch with coordinates at
 // xyz + 3 * (first atom of entry e), entries laid out one after another.
 static int admit_entries(stormm_pool *pool, const stormm_topology_batch *batch,
@@ -281,28 +371,20 @@ static int admit_entries(stormm_pool *pool, const stormm_topology_batch *batch,
   }
   // Templates first (they may be shared with resident systems), within the template budget.
   const int batch_id = ++p->batch_counter;
-  std::vector<int> tid_of_sys(batch->system_count, -1);
   std::vector<int> tids(ns);
-  for (int e = 0; e < ns; e++) {
-    const int s = sys_of[e];
-    if (tid_of_sys[s] < 0) {
-      rc = find_or_add_template(batch, st, s, batch_id, p->ts, &tid_of_sys[s]);
-      if (rc != STORMM_OK) {
-        return rc;
-      }
-    }
-    tids[e] = tid_of_sys[s];
+  TemplateStoreMark mark;
+  rc = admit_templates(p, batch, st, sys_of, batch_id, tids, mark);
+  if (rc != STORMM_OK) {
+    return rc;
   }
   for (TopoTemplate &tp : p->ts.templates) {
     if (tp.batch_id == batch_id) {
       tp.batch_id = -1;  // later batches match it by hash and counts
     }
   }
-  if (template_bytes(p->ts) > p->template_budget) {
-    return fail(STORMM_ERR_CAPACITY, "work templates exceed the pool's template budget");
-  }
   const int start = take_atoms(p, total);
   if (start < 0) {
+    rollback_templates(p->ts, mark);
     return fail(STORMM_ERR_CAPACITY, "not enough free atoms in the pool's budget");
   }
   // Parameters, coordinates and fresh minimization state of the new systems, staged for one
@@ -319,6 +401,7 @@ static int admit_entries(stormm_pool *pool, const stormm_topology_batch *batch,
     rc = fill_atom_parameters(batch, st[s], n, p->ts.templates[tids[e]], p->sm, off, ap, s);
     if (rc != STORMM_OK) {
       give_atoms(p, start, total);
+      rollback_templates(p->ts, mark);
       return rc;
     }
     for (int i = 0; i < n; i++) {
@@ -329,6 +412,17 @@ static int admit_entries(stormm_pool *pool, const stormm_topology_batch *batch,
     }
     off += round_up(n, kGlobalPad);
     xoff += static_cast<size_t>(n);
+  }
+  // Work templates reach the device with their first admission, so that minimization calls
+  // never allocate for them.
+  {
+    const hipError_t e = sync_templates(p);
+    if (e != hipSuccess) {
+      give_atoms(p, start, total);
+      rollback_templates(p->ts, mark);
+      invalidate_template_upload(p);
+      return fail(STORMM_ERR_HIP, std::string("template upload: ") + hipGetErrorString(e));
+    }
   }
   auto put = [&](DevMem &d, const void *src, size_t elt) {
     return hipMemcpy(static_cast<char *>(d.ptr) + static_cast<size_t>(start) * elt, src,
@@ -379,6 +473,7 @@ static int admit_entries(stormm_pool *pool, const stormm_topology_batch *batch,
     p->slot_live[slot] = 1;
     p->slot_table[slot] = p->ts.templates[tids[e]].lb ? 0 : 1;
     p->slot_range[slot] = rid;
+    p->template_refs[tids[e]]++;
     off += round_up(batch->atom_counts[s], kGlobalPad);
     e_slot[e] = slot;
     e_start[e] = p->slot_start[slot];
@@ -452,6 +547,8 @@ extern "C" int stormm_pool_release(stormm_pool *pool, const int32_t *slots, int3
   for (int i = 0; i < count; i++) {
     const int s = slots[i];
     pool->slot_live[s] = 0;
+    pool->template_refs[pool->slot_template[s]]--;
+    pool->slot_template[s] = -1;
     const int rid = pool->slot_range[s];
     pool->slot_range[s] = -1;
     if (--pool->ranges[rid].refs == 0) {
@@ -463,8 +560,9 @@ extern "C" int stormm_pool_release(stormm_pool *pool, const int32_t *slots, int3
   return STORMM_OK;
 }
 
-// Kits over the pool's arrays for the systems listed in `active` (device array of slots).
-static MMKit pool_mm_kit(stormm_pool *p, int nwu, int maxa, int block) {
+// Kits over the pool's arrays for the work units of one call (wu, wu_sys: device arrays).
+static MMKit pool_mm_kit(stormm_pool *p, int nwu, int maxa, int block, const WuDesc *wu,
+                         const WuSys *wu_sys) {
   MMKit k;
   std::memset(&k, 0, sizeof(k));
   k.nsys = p->cap_slots;
@@ -479,8 +577,8 @@ static MMKit pool_mm_kit(stormm_pool *p, int nwu, int maxa, int block) {
   k.kappa = p->sm.kappa;
   k.sys_start = p->starts.as<int>();
   k.sys_natom = p->natoms.as<int>();
-  k.wu = p->wu.as<WuDesc>();
-  k.wu_sys = p->wu_sys.as<WuSys>();
+  k.wu = wu;
+  k.wu_sys = wu_sys;
   k.tasks = p->tasks.as<Task>();
   k.masks = p->masks.as<unsigned>();
   k.vterms = p->vterms.as<int4>();
@@ -512,8 +610,17 @@ static MMKit pool_mm_kit(stormm_pool *p, int nwu, int maxa, int block) {
   return k;
 }
 
-extern "C" int stormm_pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *running,
-                                    void *stream) {
+// Where one call's work units, their system entries and the list of running slots go: the
+// pool's own (growing) buffers, or a caller-provided region of fixed size.
+struct PoolStaging {
+  void *base = nullptr;   // null: the pool's buffers
+  size_t bytes = 0;
+};
+
+static size_t align256(size_t n) { return (n + 255) & ~static_cast<size_t>(255); }
+
+static int pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *running, void *stream,
+                         const PoolStaging &staging) {
   if (pool == nullptr || cycles < 0) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "invalid minimization request");
   }
@@ -540,16 +647,10 @@ extern "C" int stormm_pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *
   if (act.empty() || cycles == 0) {
     return STORMM_OK;
   }
-  // Templates and work units of this call.
-  HIP_TRY(sync_array(p->tasks, p->ts.tasks, p->up_tasks));
-  HIP_TRY(sync_array(p->masks, p->ts.masks, p->up_masks));
-  HIP_TRY(sync_array(p->vterms, p->ts.vterms, p->up_vterms));
-  HIP_TRY(sync_array(p->bond_par, p->ts.bond_par, p->up_bond));
-  HIP_TRY(sync_array(p->angl_par, p->ts.angl_par, p->up_angl));
-  HIP_TRY(sync_array(p->dihe_par, p->ts.dihe_par, p->up_dihe));
-  HIP_TRY(sync_array(p->dihe14, p->ts.dihe14, p->up_d14));
-  HIP_TRY(sync_array(p->ljab, p->ts.ljab, p->up_ljab));
-  HIP_TRY(sync_array(p->ljab14, p->ts.ljab14, p->up_ljab14));
+  // The admissions uploaded every template this call uses.
+  if (!templates_in_sync(p)) {
+    return fail(STORMM_ERR_STATE, "the pool's work templates are not on the device");
+  }
   const int na = static_cast<int>(act.size());
   std::vector<double> cost(na);
   std::vector<int> natom(na), start(na), tmpl(na), index(na);
@@ -579,22 +680,46 @@ extern "C" int stormm_pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *
     maxa = std::max(maxa, append_work_unit(bins[u], modes[u], index, start, natom, tmpl, p->ts,
                                            wu, wu_sys));
   }
-  if (p->wu.bytes < wu.size() * sizeof(WuDesc)) {
-    HIP_TRY(p->wu.alloc(2 * wu.size() * sizeof(WuDesc)));
+  const size_t wu_bytes = wu.size() * sizeof(WuDesc);
+  const size_t wu_sys_bytes = wu_sys.size() * sizeof(WuSys);
+  const size_t act_bytes = act.size() * sizeof(int);
+  WuDesc *d_wu;
+  WuSys *d_wu_sys;
+  int *d_active;
+  if (staging.base == nullptr) {
+    if (p->wu.bytes < wu_bytes) {
+      HIP_TRY(p->wu.alloc(2 * wu_bytes));
+    }
+    if (p->wu_sys.bytes < wu_sys_bytes) {
+      HIP_TRY(p->wu_sys.alloc(2 * wu_sys_bytes));
+    }
+    d_wu = p->wu.as<WuDesc>();
+    d_wu_sys = p->wu_sys.as<WuSys>();
+    d_active = p->active.as<int>();
   }
-  if (p->wu_sys.bytes < wu_sys.size() * sizeof(WuSys)) {
-    HIP_TRY(p->wu_sys.alloc(2 * wu_sys.size() * sizeof(WuSys)));
+  else {
+    const size_t need = align256(wu_bytes) + align256(wu_sys_bytes) + align256(act_bytes);
+    if (need > staging.bytes) {
+      return fail(STORMM_ERR_CAPACITY,
+                  "the call's work units need " + std::to_string(need) + " bytes of staging (" +
+                      std::to_string(wu.size()) + " units, " + std::to_string(na) +
+                      " systems); the allocation-free entry point holds " +
+                      std::to_string(staging.bytes));
+    }
+    char *b = static_cast<char *>(staging.base);
+    d_wu = reinterpret_cast<WuDesc *>(b);
+    d_wu_sys = reinterpret_cast<WuSys *>(b + align256(wu_bytes));
+    d_active = reinterpret_cast<int *>(b + align256(wu_bytes) + align256(wu_sys_bytes));
   }
-  HIP_TRY(hipMemcpy(p->wu.ptr, wu.data(), wu.size() * sizeof(WuDesc), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(p->wu_sys.ptr, wu_sys.data(), wu_sys.size() * sizeof(WuSys),
-                    hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(p->active.ptr, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_wu, wu.data(), wu_bytes, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_wu_sys, wu_sys.data(), wu_sys_bytes, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_active, act.data(), act_bytes, hipMemcpyHostToDevice));
   const int prec = (p->cfg.precision == STORMM_PREC_DOUBLE) ? 0 : 1;
-  const MMKit mm = pool_mm_kit(p, static_cast<int>(wu.size()), maxa, block);
+  const MMKit mm = pool_mm_kit(p, static_cast<int>(wu.size()), maxa, block, d_wu, d_wu_sys);
   CgKit cg;
   std::memset(&cg, 0, sizeof(cg));
   cg.nsys = na;
-  cg.slots = p->active.as<int>();
+  cg.slots = d_active;
   cg.sys_start = p->starts.as<int>();
   cg.sys_natom = p->natoms.as<int>();
   cg.steps = p->steps.as<int>();
@@ -616,7 +741,7 @@ extern "C" int stormm_pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *
   LineKit lk;
   std::memset(&lk, 0, sizeof(lk));
   lk.nsys = na;
-  lk.slots = p->active.as<int>();
+  lk.slots = d_active;
   lk.done = p->done.as<int>();
   lk.sys_start = cg.sys_start;
   lk.sys_natom = cg.sys_natom;
@@ -674,7 +799,17 @@ extern "C" int stormm_pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *
   }
   // Energies at the coordinates the call leaves.
   HIP_TRY(launch_mm_eval(prec, mm, terms, 0, 1, st));
+  if (staging.base != nullptr) {
+    // The staging region is shared by every call on the device: it must not be rewritten
+    // before this call's kernels have read it.
+    HIP_TRY(hipStreamSynchronize(st));
+  }
   return STORMM_OK;
+}
+
+extern "C" int stormm_pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *running,
+                                    void *stream) {
+  return pool_minimize(pool, cycles, running, stream, PoolStaging());
 }
 
 extern "C" int stormm_pool_status(const stormm_pool *pool, const int32_t *slots, int32_t count,

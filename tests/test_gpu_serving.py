@@ -146,3 +146,43 @@ def test_served_results_equal_plain_batched_launches():
     for a, b in zip(sorted(served, key=lambda r: r.rid), sorted(plain, key=lambda r: r.rid)):
         assert np.array_equal(a.steps, b.steps)
         assert np.array_equal(a.energies, b.energies)
+
+
+def test_template_budget_rejection_rolls_back_and_unused_templates_are_dropped():
+    """A molecule whose work template does not fit next to a resident one is refused with nothing
+    kept (the resident topology is still admitted afterwards); once the resident systems leave,
+    their unused template is dropped and the new molecule fits.  Its trajectory is the same as in
+    a pool of its own (the dropped template's arrays were packed away underneath it)."""
+    kw = dict(precision="double", total_cycles=30, sd_cycles=5, conv_rms=0.0, max_systems=4)
+    a_topo, a_confs = conformers(150, 21, 3)
+    b_topo, b_confs = conformers(170, 22, 2)
+    probe_a = serving.MinimizationPool(**kw)
+    probe_a.admit_conformers(a_topo, a_confs[:1])
+    t_a = probe_a.capacity()["template_bytes"]
+    probe_b = serving.MinimizationPool(**kw)
+    b_alone = probe_b.admit_conformers(b_topo, b_confs)
+    t_b = probe_b.capacity()["template_bytes"]
+    assert t_a > 0 and t_b > 0
+    # Template share of the budget (one fifth): room for either template, not for both.
+    budget = int(1.5 * max(t_a, t_b) / 0.2)
+    assert 1.5 * max(t_a, t_b) < t_a + t_b
+    pool = serving.MinimizationPool(budget_bytes=budget, **kw)
+    a_slots = pool.admit_conformers(a_topo, a_confs[:2])
+    assert a_slots is not None
+    before = pool.capacity()
+    assert before["template_bytes"] == t_a
+    assert pool.admit_conformers(b_topo, b_confs) is None
+    assert pool.capacity() == before
+    more = pool.admit_conformers(a_topo, a_confs[2:])
+    assert more is not None and pool.capacity()["template_bytes"] == t_a
+    pool.minimize(3)
+    pool.release(a_slots + more)
+    b_slots = pool.admit_conformers(b_topo, b_confs)
+    assert b_slots is not None
+    assert pool.capacity()["template_bytes"] == t_b
+    run_until_done(pool, b_slots, 7)
+    run_until_done(probe_b, b_alone, 7)
+    assert np.array_equal(pool.status(b_slots)[3], probe_b.status(b_alone)[3])
+    for s, r in zip(b_slots, b_alone):
+        assert np.array_equal(pool.coordinates(s, b_topo.natom, fixed=True),
+                              probe_b.coordinates(r, b_topo.natom, fixed=True))

@@ -11,7 +11,7 @@ coordinates and parameters resident in device memory.
 Prints one JSON line.  The double-precision mode is timed alongside, and both modes' errors
 against the FP64 CPU oracle are measured on a 16-system subsample.  The CPU baseline times the
 oracle (the restatement of the reference's CPU path) on a bounded sample of the same systems with
-one thread per host core (up to 16).
+one thread per core of the job's CPU share.
 
 Two further workloads ride on the same line (skip them with --no-extra):
   "minimization": 4096 systems, N ~ U[16, 128], starts perturbed by N(0, 0.3 A), 500 cycles
@@ -45,6 +45,36 @@ METRIC = ("system energy+force evals/s (valence+NB+GB); served minimizations/s, 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: one per core of this job's CPU share.  On the GPU
+    machine a one-GPU job is given 16 cores (OMP_NUM_THREADS carries the share); the
+    affinity mask may list more cores that other jobs hold."""
+    visible = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        return min(visible, int(share)), visible
+    return visible, visible
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def size_stratified(natoms, count):
+    """`count` system indices spread evenly over the batch sorted by N^2 (pair work), so the
+    sample's pair-work distribution follows the batch's."""
+    order = np.argsort(np.asarray(natoms) ** 2, kind="stable")
+    picks = np.linspace(0, len(order) - 1, count).round().astype(int)
+    return [int(order[i]) for i in picks]
 
 
 def parity_errors(S, oracle, topos, coords, prec, gpos, frc, nrg):
@@ -122,6 +152,15 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu, shape=No
             sub_t, sub_c = [topos[i] for i in idx], [coords[i] for i in idx]
             parts.append(setup(sub_t, sub_c, persistent_shape))
     t_class = run(parts, True)
+    # The per-class launches give every system the same result as the whole-batch launch
+    # (results do not depend on the batch's composition).
+    e_class = np.empty(nsys)
+    for (lo, hi), (ag_c, ps_c, sc_c, lm_c) in zip(
+            [(lo, hi) for lo, hi in zip(edges[:-1], edges[1:])
+             if any(lo <= t.natom < hi for t in topos)], parts):
+        idx = [i for i, t in enumerate(topos) if lo <= t.natom < hi]
+        e_class[idx] = sc_c.potential_energies()
+    class_equal = bool(np.array_equal(e_class, e_pers))
     out = {"systems": nsys, "cycles": cycles, "sd_cycles": sd, "precision": "fp32",
            "autotuned_shape": shape, "persistent_shape": persistent_shape,
            "persistent_s": t_pers, "per_size_class_s": t_class, "per_step_launches_s": t_step,
@@ -132,12 +171,14 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu, shape=No
            "graph_vs_per_step": t_step / t_graph,
            "persistent_equals_per_step": bool(np.array_equal(e_pers, e_step)),
            "graph_equals_per_step": bool(np.array_equal(e_graph, e_step)),
+           "per_size_class_equals_whole_batch": class_equal,
            "size_classes": len(parts)}
     if with_cpu:
         from concurrent.futures import ThreadPoolExecutor
         nsample = 128
-        threads = max(1, min(16, os.cpu_count() or 1))
-        fixed = [np.rint(np.asarray(c) * 2.0 ** 32).astype(np.int64) for c in coords[:nsample]]
+        threads, visible = cpu_threads()
+        pick = size_stratified([t.natom for t in topos], nsample)
+        fixed = {i: np.rint(np.asarray(coords[i]) * 2.0 ** 32).astype(np.int64) for i in pick}
 
         def one(i):
             oracle.minimize(topos[i], fixed[i], 32, 23, 26, cycles, sd, dx0, dp_mode=False,
@@ -145,18 +186,27 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu, shape=No
 
         t0 = time.perf_counter()
         with ThreadPoolExecutor(threads) as ex:
-            list(ex.map(one, range(nsample)))
+            list(ex.map(one, pick))
         secs = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": nsample / secs, "unit": "minimizations/s",
-                               "cores": threads, "kind": "port",
-                               "sample": f"first {nsample} of the {nsys} systems, {cycles} "
-                                         f"cycles each, {threads} threads, {secs:.1f} s"}
+        # The sample's share of the batch's pair work scales its rate to the whole batch.
+        n2 = np.array([t.natom for t in topos], dtype=np.float64) ** 2
+        work_share = n2[pick].sum() / n2.sum()
+        out["cpu_baseline"] = {"value": nsys / (secs / work_share),
+                               "unit": "minimizations/s", "cores": threads, "kind": "port",
+                               "sample": f"{nsample} of the {nsys} systems spread evenly over "
+                                         f"the batch sorted by N^2 ({100 * work_share:.2f} % of "
+                                         f"its pair work), {cycles} cycles each, {threads} "
+                                         f"threads, {secs:.1f} s; rate = batch systems / "
+                                         f"(time scaled by pair work)",
+                               "host_cores_visible": visible, "cpu_model": cpu_model(),
+                               "compiler": "gcc " + " ".join(oracle.CFLAGS)}
     return out
 
 
 def large_batch_workload(S, synthetic, oracle, nsys, reps):
     """16384-system evaluation throughput in both precision modes and their oracle errors."""
     topos, coords = synthetic.config4(nsys=nsys, seed=4)
+    from stormm_amd import roofline
     out = {"systems": nsys}
     for prec, bits in (("single", (32, 23, 26)), ("double", (32, 40, 32))):
         # Smaller molecules than the headline batch: each precision on its own tuned shape.
@@ -167,10 +217,20 @@ def large_batch_workload(S, synthetic, oracle, nsys, reps):
         ps.upload(coords)
         S.time_mm_evaluation(prec, ag, ps, sc, True, 3)
         ms = S.time_mm_evaluation(prec, ag, ps, sc, True, reps)
+        info = ag.info
+        flops = roofline.eval_flops(info.pair_count, info.valence_term_count, ag.total_atoms)
+        nbytes = roofline.eval_bytes(ag.total_atoms, info.valence_term_count, info.task_count)
+        peak = roofline.PEAK_FP32_TFLOPS if prec == "single" else roofline.PEAK_FP64_TFLOPS
+        tflops = flops / (ms * 1.0e-3) / 1.0e12
         out["fp32" if prec == "single" else "fp64"] = {
             "kernel_ms": ms, "system_evals_per_s": nsys / (ms * 1.0e-3),
             "autotuned_shape": shape,
-            "force_bits": bits[1], "energy_bits": bits[2]}
+            "force_bits": bits[1], "energy_bits": bits[2],
+            "pairs": info.pair_count, "flop_per_launch": flops,
+            "achieved_tflops": tflops, "valu_peak_tflops": peak, "valu_frac": tflops / peak,
+            "algorithmic_bytes_per_launch": nbytes,
+            "achieved_hbm_gbps": nbytes / (ms * 1.0e-3) / 1.0e9,
+            "hbm_frac": nbytes / (ms * 1.0e-3) / 1.0e9 / roofline.PEAK_HBM_GBPS}
         del ag, ps, sc
     sub = slice(0, 64)
     out["errors"] = {
@@ -286,18 +346,33 @@ def main():
     cpu = None
     if not args.no_cpu:
         import oracle
-        nsample = 64
-        threads = max(1, min(16, os.cpu_count() or 1))
-        batch = oracle.Batch(topos[:nsample], coords[:nsample])
+        threads, visible = cpu_threads()
+        # The whole batch, one thread per core of the job's share, systems split into
+        # contiguous ranges of equal pair work.
+        batch = oracle.Batch(topos, coords)
         secs, _ = batch.run(repeats=1, nthreads=threads)
-        reps_cpu = int(max(1, min(5000, args.cpu_seconds / max(secs, 1e-6))))
+        reps_cpu = int(max(1, min(5000, round(args.cpu_seconds / max(secs, 1e-6)))))
         secs, _ = batch.run(repeats=reps_cpu, nthreads=threads)
-        cpu_value = nsample * reps_cpu / secs
+        cpu_value = len(topos) * reps_cpu / secs
+        # Single-core rate on a pair-work-stratified quarter of the batch.
+        pick = size_stratified([t.natom for t in topos], len(topos) // 4)
+        one = oracle.Batch([topos[i] for i in pick], [coords[i] for i in pick])
+        n2 = np.array([t.natom for t in topos], dtype=np.float64) ** 2
+        share = n2[pick].sum() / n2.sum()
+        secs1, _ = one.run(repeats=1, nthreads=1)
+        single = len(topos) / (secs1 / share)
         cpu = {"value": cpu_value, "unit": "system-evals/s", "cores": threads, "kind": "port",
-               "sample": f"first {nsample} of the 1024 systems x {reps_cpu} repeats, FP64 "
-                         f"energy+force per system, {threads} threads over disjoint systems, "
-                         f"{secs:.1f} s"}
-        log(f"CPU oracle: {cpu_value:.4e} system-evals/s on {threads} threads")
+               "sample": f"all {len(topos)} systems x {reps_cpu} passes ({secs:.1f} s), FP64 "
+                         f"energy+force per system, {threads} threads over contiguous system "
+                         f"ranges of equal pair work",
+               "single_core_value": single,
+               "single_core_sample": f"{len(pick)} systems spread evenly over the batch "
+                                     f"sorted by N^2 ({100 * share:.1f} % of its pair work), "
+                                     f"one pass, {secs1:.1f} s, scaled by pair work",
+               "host_cores_visible": visible, "cpu_model": cpu_model(),
+               "compiler": "gcc " + " ".join(oracle.CFLAGS)}
+        log(f"CPU oracle: {cpu_value:.4e} system-evals/s on {threads} threads "
+            f"({single:.4e} on one)")
 
     extra = {}
     if not args.no_extra:
@@ -339,9 +414,9 @@ def main():
             "parallelism": "single GPU, one work unit per block",
         },
         "roofline": {
-            "bound": "mfma",
-            "engine": "FP32 vector ALU (no matrix instructions; the FP32 peak is the same "
-                      "figure for vector and matrix units)",
+            "bound": "valu",
+            "engine": "FP32 vector ALU: pairwise terms are no dense contraction, so no matrix "
+                      "instructions; peak = the FP32 vector datasheet figure",
             "achieved": achieved,
             "peak": roofline.PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",

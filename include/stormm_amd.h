@@ -161,7 +161,14 @@ typedef struct {
 } stormm_synthesis_info;
 
 /* Device pointers of a PhaseSpaceSynthesis, the PsSynthesisWriter layout
- * (src/Synthesis/phasespace_synthesis.h:87-195). */
+ * (src/Synthesis/phasespace_synthesis.h:87-195): per-atom fixed-precision arrays, each system
+ * starting at atom_starts[s].  Values above the reference's plain 64-bit limits are split into a
+ * 64-bit word and a 32-bit overflow word, value = ovrf * 2^63 + word
+ * (src/Numerics/accumulation.cui:121-403): coordinates when gpos_bits > 38, forces, previous
+ * forces and conjugate-gradient directions when frc_bits > 40.  Overflow arrays are read and
+ * written only then (they may be NULL otherwise).  The reference keeps the previous forces in the
+ * alternate-coordinate arrays (xalt) and the conjugate-gradient directions in the velocity
+ * arrays (xvel) of the writer (src/Math/reduction_abstracts.cpp:25-39). */
 typedef struct {
   int32_t system_count;
   int32_t gpos_bits, frc_bits;
@@ -171,7 +178,36 @@ typedef struct {
   int64_t *xfrc, *yfrc, *zfrc;
   int64_t *xprv, *yprv, *zprv;          /* prior-step forces for conjugate gradient            */
   int64_t *x_cg_temp, *y_cg_temp, *z_cg_temp;
+  int32_t *xcrd_ovrf, *ycrd_ovrf, *zcrd_ovrf;
+  int32_t *xfrc_ovrf, *yfrc_ovrf, *zfrc_ovrf;
+  int32_t *xprv_ovrf, *yprv_ovrf, *zprv_ovrf;
+  int32_t *x_cg_temp_ovrf, *y_cg_temp_ovrf, *z_cg_temp_ovrf;
 } stormm_ps_writer;
+
+/* ScoreCardWriter layout (src/Potential/scorecard.h:64-95): instantaneous accumulators, one row
+ * of data_stride (32) int64 slots per system in StateVariable order, at nrg_bits. */
+typedef struct {
+  int32_t system_count;
+  int32_t data_stride;
+  int32_t nrg_bits;
+  int64_t *instantaneous_accumulators;
+} stormm_scorecard_writer;
+
+/* LinMinWriter layout (src/MolecularMechanics/line_minimization.h:15-50): per-system line-search
+ * state in double precision. */
+typedef struct {
+  int32_t system_count;
+  double *l_move, *s_move, *mfac_a, *mfac_b, *mfac_c, *nrg_a, *nrg_b, *nrg_c, *nrg_d;
+} stormm_linmin_writer;
+
+/* The fields of MMControlKit (src/MolecularMechanics/mm_controls.h:52-93) this path reads: the
+ * step counter (restraint mixing, steepest descent vs conjugate gradient) and the number of
+ * steepest-descent cycles.  The work-unit progress counters are not needed (each block owns one
+ * work unit). */
+typedef struct {
+  int32_t step;
+  int32_t sd_cycles;
+} stormm_mm_kit;
 
 /* Minimization controls, the subset of MinimizeControls this path reads
  * (src/Namelists/nml_minimize.h:21-30).  The reference's defaults are maxcyc 200, ncyc 50,
@@ -216,7 +252,9 @@ int stormm_autotune_launch_shape(const stormm_topology_batch *batch, int32_t pre
                                  double *best_ms);
 
 /* ---- PhaseSpaceSynthesis ------------------------------------------------------------------ */
-/* src/Synthesis/phasespace_synthesis.h:325-383 */
+/* src/Synthesis/phasespace_synthesis.h:325-383.  gpos_bits in [24, 72], frc_bits in [18, 72]
+ * (src/Constants/fixed_precision.h:30-85); the overflow words are allocated with the arrays and
+ * used above 38 coordinate bits and 40 force bits. */
 int stormm_phasespace_create(const stormm_synthesis *ag, int32_t gpos_bits, int32_t frc_bits,
                              stormm_phasespace **out);
 int stormm_phasespace_destroy(stormm_phasespace *ps);
@@ -242,6 +280,14 @@ int stormm_phasespace_import_system(stormm_phasespace *ps, int32_t system_index,
                                     const double *xyz);
 int stormm_phasespace_export_system(const stormm_phasespace *ps, int32_t system_index,
                                     double *xyz, double *fxyz);
+/* Raw fixed-precision values split into words and overflow words ([sum atoms][3] each; the
+ * overflow arrays may be NULL, then every value must fit its 64-bit word). */
+int stormm_phasespace_upload_fixed_split(stormm_phasespace *ps, const int64_t *xyz,
+                                         const int32_t *xyz_ovrf);
+int stormm_phasespace_download_fixed_split(const stormm_phasespace *ps, int64_t *xyz,
+                                           int32_t *xyz_ovrf, int64_t *fxyz, int32_t *fxyz_ovrf);
+/* The handle's device arrays in the PsSynthesisWriter layout (PhaseSpaceSynthesis::data(),
+ * phasespace_synthesis.h:564). */
 int stormm_phasespace_writer(stormm_phasespace *ps, stormm_ps_writer *w);
 int stormm_phasespace_initialize_forces(stormm_phasespace *ps, void *stream);
 int stormm_phasespace_prime_conjugate_gradient(stormm_phasespace *ps, void *stream);
@@ -258,6 +304,9 @@ int stormm_scorecard_download(const stormm_scorecard *sc, int64_t *accumulators)
 int stormm_scorecard_download_history(const stormm_scorecard *sc, double *running,
                                       double *squared, int32_t *sample_count);
 int64_t *stormm_scorecard_device_accumulators(stormm_scorecard *sc);
+/* The handle's accumulators in the ScoreCardWriter layout (ScoreCard::data(), scorecard.h:
+ * 290-296). */
+int stormm_scorecard_data(stormm_scorecard *sc, stormm_scorecard_writer *w);
 
 /* ---- MolecularMechanicsControls ----------------------------------------------------------- */
 /* src/MolecularMechanics/mm_controls.h:100-363 */
@@ -266,6 +315,8 @@ int stormm_mm_controls_create(const stormm_minimize_controls *mincon,
 int stormm_mm_controls_destroy(stormm_mm_controls *ctrl);
 int stormm_mm_controls_get_step(const stormm_mm_controls *ctrl, int32_t *step);
 int stormm_mm_controls_set_step(stormm_mm_controls *ctrl, int32_t step);
+/* MolecularMechanicsControls::dpData / spData (mm_controls.cpp:300-320). */
+int stormm_mm_controls_data(const stormm_mm_controls *ctrl, stormm_mm_kit *kit);
 
 /* ---- LineMinimization --------------------------------------------------------------------- */
 /* src/MolecularMechanics/line_minimization.h:85-221 */
@@ -273,6 +324,8 @@ int stormm_line_min_create(int32_t system_count, double dx0, stormm_line_min **o
 int stormm_line_min_destroy(stormm_line_min *lm);
 /* [nsys] each: l_move, s_move, mfac_a, mfac_b, mfac_c, nrg_a, nrg_b, nrg_c, nrg_d */
 int stormm_line_min_download(const stormm_line_min *lm, double *values9xN);
+/* LineMinimization::data (line_minimization.h:160-166). */
+int stormm_line_min_data(stormm_line_min *lm, stormm_linmin_writer *w);
 
 /* ---- Launch entry points ------------------------------------------------------------------ */
 /* Coulomb + Lennard-Jones + (if the synthesis has an implicit-solvent model) Born radii, GB pair
@@ -318,6 +371,55 @@ int stormm_launch_minimization(int32_t precision, const stormm_synthesis *ag,
 /* Per-system status after a persistent minimization: cycles run and final RMS force. */
 int stormm_minimization_status(const stormm_synthesis *ag, int32_t *cycles_run,
                                double *rms_force);
+
+/* ---- Launch entry points on caller-owned arrays (kit form) --------------------------------- */
+/* The same launches as above with the state in the caller's device arrays, passed in the
+ * reference's abstract layouts, as its launch functions take them (launchNonbonded
+ * hpc_nonbonded_potential.h:124-140, launchValence hpc_valence_potential.h:167-210,
+ * launchConjugateGradient hpc_reduction.h:55-63, launchLineAdvance hpc_minimization.h:63-70,
+ * launchMinimization hpc_minimization.h:122-151).  The parameters still come from the synthesis
+ * handle; the arrays must follow its atom layout (stormm_synthesis_atom_layout, checked by
+ * stormm_check_ps_writer).  A NULL controls kit means step 0.  Forces above 40 bits are
+ * accumulated at 40 bits within a work unit and stored at frc_bits; single-precision launches
+ * take at most 40 force bits.  Mode 1 (persistent) of the minimization takes no overflow words
+ * (gpos_bits <= 38, frc_bits <= 40). */
+int stormm_check_ps_writer(const stormm_synthesis *ag, const stormm_ps_writer *psw);
+int stormm_launch_nonbonded_kit(int32_t precision, const stormm_synthesis *ag,
+                                const stormm_mm_kit *ctrl, const stormm_ps_writer *psw,
+                                const stormm_scorecard_writer *scw, int32_t eval_force,
+                                int32_t eval_energy, double clash_minimum_distance,
+                                double clash_ratio, void *stream);
+int stormm_launch_valence_kit(int32_t precision, const stormm_synthesis *ag,
+                              const stormm_mm_kit *ctrl, const stormm_ps_writer *psw,
+                              const stormm_scorecard_writer *scw, int32_t eval_force,
+                              int32_t eval_energy, double clash_minimum_distance,
+                              double clash_ratio, void *stream);
+int stormm_launch_mm_evaluation_kit(int32_t precision, const stormm_synthesis *ag,
+                                    const stormm_mm_kit *ctrl, const stormm_ps_writer *psw,
+                                    const stormm_scorecard_writer *scw, int32_t eval_force,
+                                    int32_t eval_energy, double clash_minimum_distance,
+                                    double clash_ratio, void *stream);
+int stormm_launch_conjugate_gradient_kit(int32_t precision, const stormm_synthesis *ag,
+                                         const stormm_ps_writer *psw, const stormm_mm_kit *ctrl,
+                                         void *stream);
+int stormm_launch_line_advance_kit(int32_t precision, const stormm_synthesis *ag,
+                                   const stormm_ps_writer *psw,
+                                   const stormm_scorecard_writer *scw,
+                                   const stormm_linmin_writer *lmw, int32_t move_number,
+                                   void *stream);
+int stormm_launch_minimization_kit(int32_t precision, const stormm_synthesis *ag,
+                                   const stormm_ps_writer *psw,
+                                   const stormm_minimize_controls *mincon,
+                                   const stormm_scorecard_writer *scw,
+                                   const stormm_linmin_writer *lmw, int32_t persistent,
+                                   void *stream);
+/* PhaseSpaceSynthesis::initializeForces / primeConjugateGradientCalculation and
+ * ScoreCard::initialize on caller-owned arrays. */
+int stormm_ps_writer_initialize_forces(const stormm_ps_writer *psw, int32_t padded_atoms,
+                                       void *stream);
+int stormm_ps_writer_prime_conjugate_gradient(const stormm_ps_writer *psw, int32_t padded_atoms,
+                                              void *stream);
+int stormm_scorecard_writer_initialize(const stormm_scorecard_writer *scw, void *stream);
 
 /* ---- Timing helpers (HIP events on the launch stream) -------------------------------------- */
 /* Run `repeats` fused evaluations back to back on `stream` and return the mean kernel time in

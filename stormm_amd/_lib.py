@@ -69,7 +69,26 @@ class PsWriter(ctypes.Structure):
                 ("atom_starts", vp), ("atom_counts", vp),
                 ("xcrd", vp), ("ycrd", vp), ("zcrd", vp), ("xfrc", vp), ("yfrc", vp),
                 ("zfrc", vp), ("xprv", vp), ("yprv", vp), ("zprv", vp),
-                ("x_cg_temp", vp), ("y_cg_temp", vp), ("z_cg_temp", vp)]
+                ("x_cg_temp", vp), ("y_cg_temp", vp), ("z_cg_temp", vp),
+                ("xcrd_ovrf", vp), ("ycrd_ovrf", vp), ("zcrd_ovrf", vp),
+                ("xfrc_ovrf", vp), ("yfrc_ovrf", vp), ("zfrc_ovrf", vp),
+                ("xprv_ovrf", vp), ("yprv_ovrf", vp), ("zprv_ovrf", vp),
+                ("x_cg_temp_ovrf", vp), ("y_cg_temp_ovrf", vp), ("z_cg_temp_ovrf", vp)]
+
+
+class ScoreCardWriter(ctypes.Structure):
+    _fields_ = [("system_count", c_int32), ("data_stride", c_int32), ("nrg_bits", c_int32),
+                ("instantaneous_accumulators", vp)]
+
+
+class LinMinWriter(ctypes.Structure):
+    _fields_ = [("system_count", c_int32)] + [
+        (n, vp) for n in ("l_move", "s_move", "mfac_a", "mfac_b", "mfac_c", "nrg_a", "nrg_b",
+                          "nrg_c", "nrg_d")]
+
+
+class MMKit(ctypes.Structure):
+    _fields_ = [("step", c_int32), ("sd_cycles", c_int32)]
 
 
 class MinimizeControls(ctypes.Structure):
@@ -121,6 +140,30 @@ _SIGS = {
     "stormm_phasespace_import_system": (c_int32, [vp, c_int32, vp]),
     "stormm_phasespace_export_system": (c_int32, [vp, c_int32, vp, vp]),
     "stormm_phasespace_writer": (c_int32, [vp, P(PsWriter)]),
+    "stormm_phasespace_upload_fixed_split": (c_int32, [vp, vp, vp]),
+    "stormm_phasespace_download_fixed_split": (c_int32, [vp, vp, vp, vp, vp]),
+    "stormm_scorecard_data": (c_int32, [vp, P(ScoreCardWriter)]),
+    "stormm_mm_controls_data": (c_int32, [vp, P(MMKit)]),
+    "stormm_line_min_data": (c_int32, [vp, P(LinMinWriter)]),
+    "stormm_check_ps_writer": (c_int32, [vp, P(PsWriter)]),
+    "stormm_launch_nonbonded_kit": (c_int32, [c_int32, vp, P(MMKit), P(PsWriter),
+                                              P(ScoreCardWriter), c_int32, c_int32, c_double,
+                                              c_double, vp]),
+    "stormm_launch_valence_kit": (c_int32, [c_int32, vp, P(MMKit), P(PsWriter),
+                                            P(ScoreCardWriter), c_int32, c_int32, c_double,
+                                            c_double, vp]),
+    "stormm_launch_mm_evaluation_kit": (c_int32, [c_int32, vp, P(MMKit), P(PsWriter),
+                                                  P(ScoreCardWriter), c_int32, c_int32,
+                                                  c_double, c_double, vp]),
+    "stormm_launch_conjugate_gradient_kit": (c_int32, [c_int32, vp, P(PsWriter), P(MMKit), vp]),
+    "stormm_launch_line_advance_kit": (c_int32, [c_int32, vp, P(PsWriter), P(ScoreCardWriter),
+                                                 P(LinMinWriter), c_int32, vp]),
+    "stormm_launch_minimization_kit": (c_int32, [c_int32, vp, P(PsWriter),
+                                                 P(MinimizeControls), P(ScoreCardWriter),
+                                                 P(LinMinWriter), c_int32, vp]),
+    "stormm_ps_writer_initialize_forces": (c_int32, [P(PsWriter), c_int32, vp]),
+    "stormm_ps_writer_prime_conjugate_gradient": (c_int32, [P(PsWriter), c_int32, vp]),
+    "stormm_scorecard_writer_initialize": (c_int32, [P(ScoreCardWriter), vp]),
     "stormm_phasespace_initialize_forces": (c_int32, [vp, vp]),
     "stormm_phasespace_prime_conjugate_gradient": (c_int32, [vp, vp]),
     "stormm_scorecard_create": (c_int32, [c_int32, c_int32, P(vp)]),

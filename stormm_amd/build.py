@@ -28,6 +28,26 @@ def _compile(src: str) -> str:
     return obj
 
 
+ROOT = os.path.dirname(HERE)
+CAPI_DIR = os.path.join(ROOT, "tests", "capi")
+CAPI_BIN = os.path.join(CAPI_DIR, "bin")
+# Test programs written against the C-ABI (tests/capi/*.cpp): the host check of the 95-bit
+# helpers, and a C++ caller that owns its device arrays.
+CAPI_PROGRAMS = {"int95_host": [], "kit_boundary": ["-L", HERE, "-lstormm_amd",
+                                                    "-Wl,-rpath,$ORIGIN/../../../stormm_amd"]}
+
+
+def build_capi_programs(verbose: bool = False) -> None:
+    os.makedirs(CAPI_BIN, exist_ok=True)
+    for name, extra in CAPI_PROGRAMS.items():
+        out = os.path.join(CAPI_BIN, name)
+        cmd = [HIPCC, "-O2", "-std=c++17", "-x", "hip", f"--offload-arch={ARCH}",
+               os.path.join(CAPI_DIR, name + ".cpp"), "-o", out] + extra
+        subprocess.run(cmd, check=True)
+        if verbose:
+            print(f"built {out}")
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
@@ -38,6 +58,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_capi_programs(verbose)
     return LIB
 
 

@@ -29,6 +29,12 @@ constexpr int kPsiBits = 40;
 constexpr int kDedrBits = 32;
 // LDS coordinate resolution (bits after the binary point) in single-precision mode.
 constexpr int kLdsCrdBits = 24;
+// Fixed-precision bits of the per-system origins kept in LDS for 95-bit coordinates (values at
+// 2^-38 A fit a 64-bit word to 3.3e7 A), and the largest coordinate and force bits that fit a
+// plain 64-bit word (src/Constants/fixed_precision.h:37,85).
+constexpr int kOriginBits = 38;
+constexpr int kMaxPlainGposBits = 38;
+constexpr int kMaxPlainFrcBits = 40;
 // Threads of an evaluation block: the default, and the largest a single-precision launch may
 // use; double-precision launches use at most kMaxEvalBlockDouble threads (their kernels need
 // more than the 128 registers per lane that 16-wavefront blocks allow).
@@ -131,6 +137,12 @@ struct MMKit {
   const double *gb_alpha, *gb_beta, *gb_gamma;
   long long *xcrd, *ycrd, *zcrd;
   long long *xfrc, *yfrc, *zfrc;
+  // Overflow words of 95-bit coordinates (gpos_bits > 38) and forces (stored frc_bits > 40),
+  // value = ovrf 2^63 + word; null when the values are plain 64-bit integers.  Forces are
+  // accumulated at frc_bits and stored shifted left by frc_store_shift bits.
+  const int *xcrd_ovrf, *ycrd_ovrf, *zcrd_ovrf;
+  int *xfrc_ovrf, *yfrc_ovrf, *zfrc_ovrf;
+  int frc_store_shift;
   long long *scorecard;
   int *status;
   int store_overwrite;       // 1: forces and energy rows are written, not added to

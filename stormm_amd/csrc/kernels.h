@@ -8,6 +8,8 @@ namespace stormm_amd {
 hipError_t launch_mm_eval(int precision, const MMKit &k, int terms, int eval_force,
                           int eval_energy, hipStream_t stream);
 hipError_t launch_fill(long long *a, long long *b, long long *c, size_t n, hipStream_t stream);
+// Zeroes three int arrays of n entries (overflow words).
+hipError_t launch_fill_int(int *a, int *b, int *c, size_t n, hipStream_t stream);
 hipError_t launch_scorecard_commit(const long long *inst, double *running, double *squared,
                                    int nsys, double inv_scale, hipStream_t stream);
 
@@ -24,6 +26,11 @@ struct CgKit {
   long long *xfrc, *yfrc, *zfrc;
   long long *xprv, *yprv, *zprv;
   long long *xcg, *ycg, *zcg;
+  // Overflow words of 95-bit forces, previous forces and directions (frc_bits > 40, double
+  // precision only); null otherwise.
+  int *xfrc_ovrf, *yfrc_ovrf, *zfrc_ovrf;
+  int *xprv_ovrf, *yprv_ovrf, *zprv_ovrf;
+  int *xcg_ovrf, *ycg_ovrf, *zcg_ovrf;
   double inv_frc_scale;
   int dp_mode;
 };
@@ -34,6 +41,8 @@ struct LineKit {
   const int *sys_start, *sys_natom;
   long long *xcrd, *ycrd, *zcrd;
   const long long *xfrc, *yfrc, *zfrc;
+  int *xcrd_ovrf, *ycrd_ovrf, *zcrd_ovrf;          // 95-bit coordinates (null: 64-bit)
+  const int *xfrc_ovrf, *yfrc_ovrf, *zfrc_ovrf;    // 95-bit directions (null: 64-bit)
   const long long *scorecard;
   double inv_nrg_scale;
   double inv_frc_scale, gpos_scale;

@@ -3,6 +3,7 @@
 // work-unit LDS layout and the valence term evaluators.
 #pragma once
 
+#include "int95.h"
 #include "internal.h"
 
 namespace stormm_amd {
@@ -312,12 +313,28 @@ template <> struct CrdConv<float> {
     const long long d = (v - origin) >> shift;
     return d > -2147483000LL && d < 2147483000LL;
   }
+  // Displacements from a 95-bit value (exact 128-bit difference from the origin).
+  static __device__ __forceinline__ int to_lds_wide(__int128 d, int shift) {
+    const __int128 half = (shift > 0) ? (static_cast<__int128>(1) << (shift - 1)) : 0;
+    return static_cast<int>((d + half) >> shift);
+  }
+  static __device__ __forceinline__ bool in_range_wide(__int128 d, int shift) {
+    const __int128 q = d >> shift;
+    return q > -2147483000 && q < 2147483000;
+  }
 };
 template <> struct CrdConv<double> {
   static __device__ __forceinline__ double to_lds(long long v, long long origin, int shift) {
     return static_cast<double>(v - origin) * ldexp(1.0, -shift);
   }
   static __device__ __forceinline__ bool in_range(long long, long long, int) { return true; }
+  static __device__ __forceinline__ double to_lds_wide(__int128 d, int shift) {
+    return i128_to_double(d) * ldexp(1.0, -shift);
+  }
+  static __device__ __forceinline__ bool in_range_wide(__int128 d, int shift) {
+    const __int128 q = d >> shift;
+    return q > -1099511627776LL && q < 1099511627776LL;
+  }
 };
 
 }  // namespace stormm_amd

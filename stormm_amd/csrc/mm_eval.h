@@ -22,6 +22,7 @@ template <typename T> struct StageCtx {
   WuDesc wu;
   int tid, nthr;
   int crd_shift;     // right shift from global fixed precision to LDS coordinates
+  int org_shift;     // right shift from global fixed precision to the stored origins
   T crd_scale;       // LDS coordinate unit in Angstroms (single precision)
   int cycle;         // minimization cycle of this evaluation (clash-damping schedule)
   int rst_step;      // restraint step number when the kit gives no per-system cycle counts
@@ -109,10 +110,22 @@ __device__ void stage_load_coords(StageCtx<T> &c, const MMKit &k) {
       L.clash_on[c.tid] = on ? 1 : 0;
     }
     const int gs = L.sys[c.tid].global_start;
-    const long long mask = ~((1LL << c.crd_shift) - 1LL);
-    L.org[3 * c.tid + 0] = k.xcrd[gs] & mask;
-    L.org[3 * c.tid + 1] = k.ycrd[gs] & mask;
-    L.org[3 * c.tid + 2] = k.zcrd[gs] & mask;
+    if (k.xcrd_ovrf != nullptr) {
+      // 95-bit coordinates: the first atom's value with the bits below the LDS resolution
+      // cleared, kept in units of 2^-(gpos_bits - org_shift) A.
+      const long long *crd[3] = {k.xcrd, k.ycrd, k.zcrd};
+      const int *ovr[3] = {k.xcrd_ovrf, k.ycrd_ovrf, k.zcrd_ovrf};
+      for (int d = 0; d < 3; d++) {
+        const __int128 v = (i95_value(crd[d][gs], ovr[d][gs]) >> c.crd_shift) << c.crd_shift;
+        L.org[3 * c.tid + d] = static_cast<long long>(v >> c.org_shift);
+      }
+    }
+    else {
+      const long long mask = ~((1LL << c.crd_shift) - 1LL);
+      L.org[3 * c.tid + 0] = k.xcrd[gs] & mask;
+      L.org[3 * c.tid + 1] = k.ycrd[gs] & mask;
+      L.org[3 * c.tid + 2] = k.zcrd[gs] & mask;
+    }
   }
   for (int e = c.tid; e < wu.sys_count * 16; e += c.nthr) {
     L.nrg[e] = 0;
@@ -131,7 +144,23 @@ __device__ void stage_load_coords(StageCtx<T> &c, const MMKit &k) {
     L.fz[a] = 0;
     L.rad[a] = T(1);
     L.sd[a] = T(0);
-    if (g >= 0) {
+    if (g >= 0 && k.xcrd_ovrf != nullptr) {
+      const __int128 dx = i95_value(k.xcrd[g], k.xcrd_ovrf[g]) -
+                          (static_cast<__int128>(L.org[3 * s]) << c.org_shift);
+      const __int128 dy = i95_value(k.ycrd[g], k.ycrd_ovrf[g]) -
+                          (static_cast<__int128>(L.org[3 * s + 1]) << c.org_shift);
+      const __int128 dz = i95_value(k.zcrd[g], k.zcrd_ovrf[g]) -
+                          (static_cast<__int128>(L.org[3 * s + 2]) << c.org_shift);
+      if (!(CrdConv<T>::in_range_wide(dx, c.crd_shift) &&
+            CrdConv<T>::in_range_wide(dy, c.crd_shift) &&
+            CrdConv<T>::in_range_wide(dz, c.crd_shift))) {
+        atomicOr(k.status, static_cast<int>(kStatusCoordinateRange));
+      }
+      L.cx[a] = CrdConv<T>::to_lds_wide(dx, c.crd_shift);
+      L.cy[a] = CrdConv<T>::to_lds_wide(dy, c.crd_shift);
+      L.cz[a] = CrdConv<T>::to_lds_wide(dz, c.crd_shift);
+    }
+    else if (g >= 0) {
       const long long gx = k.xcrd[g], gy = k.ycrd[g], gz = k.zcrd[g];
       const long long ox = L.org[3 * s], oy = L.org[3 * s + 1], oz = L.org[3 * s + 2];
       if (!(CrdConv<T>::in_range(gx, ox, c.crd_shift) &&
@@ -165,6 +194,7 @@ __device__ void init_ctx(StageCtx<T> &c, const MMKit &k, int wu_index, int terms
     c.crd_shift = k.gpos_bits;
     c.crd_scale = T(1);
   }
+  c.org_shift = (k.xcrd_ovrf != nullptr) ? (k.gpos_bits - kOriginBits) : 0;
   c.gb = (k.igb != 0) && ((terms & kTermNonbonded) != 0);
   c.cycle = k.clash_cycle;
   c.rst_step = k.rst_step;
@@ -1553,7 +1583,7 @@ template <typename T, bool FORCES>
 __device__ void stage_restraints(StageCtx<T> &c, const MMKit &k) {
   const double fscale = ldexp(1.0, k.frc_bits);
   const double nrg_scale = ldexp(1.0, k.nrg_bits);
-  const double inv_gpos = ldexp(1.0, -k.gpos_bits);
+  const double inv_gpos = ldexp(1.0, -(k.gpos_bits - c.org_shift));
   ValenceCtxT<typename Prec<T>::crd_t> v;
   v.cx = c.L.cx;
   v.cy = c.L.cy;
@@ -1632,7 +1662,26 @@ __device__ __forceinline__ void evaluate_wu(StageCtx<T> &c, const MMKit &k, int 
 template <typename T, bool FORCES>
 __device__ __forceinline__ void stage_store(StageCtx<T> &c, const MMKit &k, bool eval_energy) {
   WuLds<T> &L = c.L;
-  if (FORCES) {
+  if (FORCES && k.xfrc_ovrf != nullptr) {
+    // 95-bit forces: the LDS sums scaled to the stored precision, added to (or replacing) the
+    // stored values and split into word and overflow word.
+    long long *frc[3] = {k.xfrc, k.yfrc, k.zfrc};
+    int *ovr[3] = {k.xfrc_ovrf, k.yfrc_ovrf, k.zfrc_ovrf};
+    const long long *lds[3] = {L.fx, L.fy, L.fz};
+    for (int a = c.tid; a < c.wu.lds_atoms; a += c.nthr) {
+      const int g = L.gidx[a];
+      if (g >= 0) {
+        for (int d = 0; d < 3; d++) {
+          __int128 v = static_cast<__int128>(lds[d][a]) << k.frc_store_shift;
+          if (!k.store_overwrite) {
+            v += i95_value(frc[d][g], ovr[d][g]);
+          }
+          i95_split(v, &frc[d][g], &ovr[d][g]);
+        }
+      }
+    }
+  }
+  else if (FORCES) {
     for (int a = c.tid; a < c.wu.lds_atoms; a += c.nthr) {
       const int g = L.gidx[a];
       if (g >= 0) {

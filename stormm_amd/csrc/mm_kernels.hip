@@ -85,6 +85,26 @@ hipError_t launch_fill(long long *a, long long *b, long long *c, size_t n, hipSt
   return hipGetLastError();
 }
 
+__global__ void k_fill_int(int *a, int *b, int *c, size_t n) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    a[i] = 0;
+    b[i] = 0;
+    c[i] = 0;
+  }
+}
+
+hipError_t launch_fill_int(int *a, int *b, int *c, size_t n, hipStream_t stream) {
+  if (n == 0) {
+    return hipSuccess;
+  }
+  const int block = 256;
+  const size_t want = (n + block - 1) / block;
+  const int grid = static_cast<int>(want < 4096 ? want : 4096);
+  hipLaunchKernelGGL(k_fill_int, dim3(grid), dim3(block), 0, stream, a, b, c, n);
+  return hipGetLastError();
+}
+
 // ScoreCard::commit (kScoreCardCommit, src/Potential/hpc_scorecard.cu:72): fold the
 // instantaneous accumulators into running sums and sums of squares, and store the potential
 // energy total in its slot.

@@ -30,6 +30,106 @@ __device__ __forceinline__ double wave_sum64(double v) {
 // them again in each pass.  Both forms accumulate in the same order.
 constexpr int kCgCached = 4;
 
+// Per-system stop rule of the serving and graph minimizers: records the RMS force, and either
+// stops the system (returns false) or advances its cycle count into *step.
+__device__ __forceinline__ bool cg_continue(const CgKit &k, int s, int lane, int n, double ff,
+                                            int *step) {
+  if (k.steps == nullptr) {
+    return true;
+  }
+  const int prior = k.steps[s];
+  const double rms = sqrt(ff / (3.0 * n)) * k.inv_frc_scale;
+  if (lane == 0 && k.rms != nullptr) {
+    k.rms[s] = rms;
+  }
+  const bool stop = (k.conv_rms > 0.0 && rms <= k.conv_rms) ||
+                    (k.max_steps > 0 && prior >= k.max_steps);
+  if (stop) {
+    if (lane == 0) {
+      k.done[s] = 1;
+    }
+    return false;
+  }
+  *step = prior + 1;
+  if (lane == 0) {
+    k.steps[s] = *step;
+  }
+  return true;
+}
+
+// The same update on 95-bit forces, previous forces and directions (double precision with more
+// than 40 force bits): values y 2^63 + x are read as doubles, new directions rounded to nearest
+// and split as doubleToInt95 does (src/Math/hpc_reduction.cu:95-321, the overflow forms).
+__device__ void cg_wide(const CgKit &k, int s, int lane, int start, int n, int step,
+                        int sd_cycles) {
+  long long *frc[3] = {k.xfrc, k.yfrc, k.zfrc};
+  long long *prv[3] = {k.xprv, k.yprv, k.zprv};
+  long long *cg[3] = {k.xcg, k.ycg, k.zcg};
+  int *frc_o[3] = {k.xfrc_ovrf, k.yfrc_ovrf, k.zfrc_ovrf};
+  int *prv_o[3] = {k.xprv_ovrf, k.yprv_ovrf, k.zprv_ovrf};
+  int *cg_o[3] = {k.xcg_ovrf, k.ycg_ovrf, k.zcg_ovrf};
+  double ff = 0.0, gg = 0.0, dgg = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const int g = start + i;
+    for (int d = 0; d < 3; d++) {
+      const __int128 f = i95_value(frc[d][g], frc_o[d][g]);
+      const __int128 p = i95_value(prv[d][g], prv_o[d][g]);
+      const double df = i128_to_double(f);
+      const double dp = i128_to_double(p);
+      ff += df * df;
+      gg += dp * dp;
+      dgg += i128_to_double(f - p) * df;
+    }
+  }
+  ff = wave_sum64(ff);
+  gg = wave_sum64(gg);
+  dgg = wave_sum64(dgg);
+  if (!cg_continue(k, s, lane, n, ff, &step)) {
+    return;
+  }
+  const double gam = (step >= sd_cycles) ? (dgg / gg) : 0.0;
+  const bool use_gam = fabs(gam) > 1.0e-12;
+  double msum = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const int g = start + i;
+    for (int d = 0; d < 3; d++) {
+      const long long fx = frc[d][g];
+      const int fy = frc_o[d][g];
+      const double df = i95_to_double(fx, fy);
+      prv[d][g] = fx;
+      prv_o[d][g] = fy;
+      if (use_gam) {
+        const double dfcg = df + (gam * i95_to_double(cg[d][g], cg_o[d][g]));
+        msum += dfcg * dfcg;
+        long long x;
+        int y;
+        i95_from_double(dfcg, &x, &y);
+        cg[d][g] = x;
+        cg_o[d][g] = y;
+        frc[d][g] = x;
+        frc_o[d][g] = y;
+      }
+      else {
+        msum += df * df;
+        cg[d][g] = fx;
+        cg_o[d][g] = fy;
+      }
+    }
+  }
+  msum = wave_sum64(msum);
+  const double inv_factor = 1.0 / (sqrt(msum) * k.inv_frc_scale);
+  for (int i = lane; i < n; i += 64) {
+    const int g = start + i;
+    for (int d = 0; d < 3; d++) {
+      long long x;
+      int y;
+      i95_from_double(i95_to_double(frc[d][g], frc_o[d][g]) * inv_factor, &x, &y);
+      frc[d][g] = x;
+      frc_o[d][g] = y;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(64) k_conjugate_gradient(CgKit k, int step, int sd_cycles) {
   const int s = (k.slots != nullptr) ? k.slots[blockIdx.x] : static_cast<int>(blockIdx.x);
   const int lane = threadIdx.x;
@@ -38,7 +138,10 @@ __global__ void __launch_bounds__(64) k_conjugate_gradient(CgKit k, int step, in
   }
   const int start = k.sys_start[s];
   const int n = k.sys_natom[s];
-  const int prior = (k.steps != nullptr) ? k.steps[s] : 0;
+  if (k.xfrc_ovrf != nullptr) {
+    cg_wide(k, s, lane, start, n, step, sd_cycles);
+    return;
+  }
   long long *frc[3] = {k.xfrc, k.yfrc, k.zfrc};
   long long *prv[3] = {k.xprv, k.yprv, k.zprv};
   long long *cg[3] = {k.xcg, k.ycg, k.zcg};
@@ -85,23 +188,8 @@ __global__ void __launch_bounds__(64) k_conjugate_gradient(CgKit k, int step, in
   ff = wave_sum64(ff);
   gg = wave_sum64(gg);
   dgg = wave_sum64(dgg);
-  if (k.steps != nullptr) {
-    const double rms = sqrt(ff / (3.0 * n)) * k.inv_frc_scale;
-    if (lane == 0 && k.rms != nullptr) {
-      k.rms[s] = rms;
-    }
-    const bool stop = (k.conv_rms > 0.0 && rms <= k.conv_rms) ||
-                      (k.max_steps > 0 && prior >= k.max_steps);
-    if (stop) {
-      if (lane == 0) {
-        k.done[s] = 1;
-      }
-      return;
-    }
-    step = prior + 1;
-    if (lane == 0) {
-      k.steps[s] = step;
-    }
+  if (!cg_continue(k, s, lane, n, ff, &step)) {
+    return;
   }
   const double gam = (step >= sd_cycles) ? (dgg / gg) : 0.0;
   const bool use_gam = fabs(gam) > 1.0e-12;
@@ -241,11 +329,40 @@ __global__ void __launch_bounds__(64 * kLineWaves) k_line_advance(LineKit k, int
   }
   const int start = k.sys_start[s];
   const int n = k.sys_natom[s];
+  if (k.xcrd_ovrf == nullptr && k.xfrc_ovrf == nullptr) {
+    for (int i = lane; i < n; i += 64) {
+      const int g = start + i;
+      k.xcrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.xfrc[g])));
+      k.ycrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.yfrc[g])));
+      k.zcrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.zfrc[g])));
+    }
+    return;
+  }
+  // 95-bit coordinates and / or directions (line_movement.cui:343-406, the overflow forms): the
+  // increment is split as doubleToInt95 does and added with the carry of splitFPSum.
+  long long *crd[3] = {k.xcrd, k.ycrd, k.zcrd};
+  int *crd_o[3] = {k.xcrd_ovrf, k.ycrd_ovrf, k.zcrd_ovrf};
+  const long long *frc[3] = {k.xfrc, k.yfrc, k.zfrc};
+  const int *frc_o[3] = {k.xfrc_ovrf, k.yfrc_ovrf, k.zfrc_ovrf};
   for (int i = lane; i < n; i += 64) {
     const int g = start + i;
-    k.xcrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.xfrc[g])));
-    k.ycrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.yfrc[g])));
-    k.zcrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.zfrc[g])));
+    for (int d = 0; d < 3; d++) {
+      const double f = (frc_o[d] != nullptr) ? i95_to_double(frc[d][g], frc_o[d][g])
+                                             : static_cast<double>(frc[d][g]);
+      long long ix;
+      int iy;
+      i95_from_double(mf * f, &ix, &iy);
+      if (crd_o[d] != nullptr) {
+        long long rx;
+        int ry;
+        i95_add(ix, iy, crd[d][g], crd_o[d][g], &rx, &ry);
+        crd[d][g] = rx;
+        crd_o[d][g] = ry;
+      }
+      else {
+        crd[d][g] += ix;
+      }
+    }
   }
 }
 

@@ -7,7 +7,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <climits>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -16,6 +18,7 @@
 
 #include "../../include/stormm_amd.h"
 #include "host_common.h"
+#include "int95.h"
 #include "internal.h"
 #include "kernels.h"
 #include "plan.h"
@@ -51,6 +54,9 @@ struct stormm_phasespace {
   const stormm_synthesis *ag = nullptr;
   int gpos_bits = 32, frc_bits = 23;
   DevMem xcrd, ycrd, zcrd, xfrc, yfrc, zfrc, xprv, yprv, zprv, xcg, ycg, zcg;
+  // Overflow words of the same arrays (value = ovrf 2^63 + word above the plain limits).
+  DevMem xcrd_o, ycrd_o, zcrd_o, xfrc_o, yfrc_o, zfrc_o, xprv_o, yprv_o, zprv_o, xcg_o, ycg_o,
+      zcg_o;
 };
 
 struct stormm_scorecard {
@@ -232,28 +238,52 @@ extern "C" int stormm_synthesis_atom_layout(const stormm_synthesis *ag, int32_t 
 }
 
 // ---- PhaseSpaceSynthesis -----------------------------------------------------------------
+// Fixed-precision values cross between host and device as 128-bit integers: the device holds a
+// 64-bit word per value and, above 38 coordinate bits or 40 force bits, a 32-bit overflow word
+// (value = ovrf 2^63 + word).
+static bool wide_coordinates(const stormm_phasespace *ps) {
+  return ps->gpos_bits > kMaxPlainGposBits;
+}
+static bool wide_forces(const stormm_phasespace *ps) { return ps->frc_bits > kMaxPlainFrcBits; }
+
+// hostDoubleToInt95 (src/Numerics/split_fixed_precision.cpp): a value already scaled to the
+// fixed-precision grid, rounded half away from zero.
+static __int128 host_fixed(double scaled) {
+  if (std::fabs(scaled) >= kTwo63) {
+    const int spill = static_cast<int>(scaled / kTwo63);
+    return i95_value(std::llround(scaled - static_cast<double>(spill) * kTwo63), spill);
+  }
+  return static_cast<__int128>(std::llround(scaled));
+}
+
 extern "C" int stormm_phasespace_create(const stormm_synthesis *ag, int32_t gpos_bits,
                                         int32_t frc_bits, stormm_phasespace **out) {
   if (ag == nullptr || out == nullptr) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
   }
-  if (gpos_bits < 24 || gpos_bits > 38) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "gpos_bits must lie in [24, 38] (64-bit "
-                                             "coordinates without overflow words)");
+  if (gpos_bits < 24 || gpos_bits > 72) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "gpos_bits must lie in [24, 72]");
   }
-  if (frc_bits < 18 || frc_bits > 40) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "frc_bits must lie in [18, 40]");
+  if (frc_bits < 18 || frc_bits > 72) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "frc_bits must lie in [18, 72]");
   }
   std::unique_ptr<stormm_phasespace> ps(new stormm_phasespace());
   ps->ag = ag;
   ps->gpos_bits = gpos_bits;
   ps->frc_bits = frc_bits;
-  const size_t n = static_cast<size_t>(ag->plan.padded_atoms) * sizeof(long long);
-  DevMem *arrs[12] = {&ps->xcrd, &ps->ycrd, &ps->zcrd, &ps->xfrc, &ps->yfrc, &ps->zfrc,
-                      &ps->xprv, &ps->yprv, &ps->zprv, &ps->xcg, &ps->ycg, &ps->zcg};
-  for (DevMem *d : arrs) {
-    HIP_TRY(d->alloc(n));
-    HIP_TRY(hipMemset(d->ptr, 0, n));
+  const size_t na = static_cast<size_t>(ag->plan.padded_atoms);
+  DevMem *words[12] = {&ps->xcrd, &ps->ycrd, &ps->zcrd, &ps->xfrc, &ps->yfrc, &ps->zfrc,
+                       &ps->xprv, &ps->yprv, &ps->zprv, &ps->xcg, &ps->ycg, &ps->zcg};
+  for (DevMem *d : words) {
+    HIP_TRY(d->alloc(na * sizeof(long long)));
+    HIP_TRY(hipMemset(d->ptr, 0, na * sizeof(long long)));
+  }
+  DevMem *ovrf[12] = {&ps->xcrd_o, &ps->ycrd_o, &ps->zcrd_o, &ps->xfrc_o, &ps->yfrc_o,
+                      &ps->zfrc_o, &ps->xprv_o, &ps->yprv_o, &ps->zprv_o, &ps->xcg_o,
+                      &ps->ycg_o, &ps->zcg_o};
+  for (DevMem *d : ovrf) {
+    HIP_TRY(d->alloc(na * sizeof(int)));
+    HIP_TRY(hipMemset(d->ptr, 0, na * sizeof(int)));
   }
   *out = ps.release();
   return STORMM_OK;
@@ -264,119 +294,8 @@ extern "C" int stormm_phasespace_destroy(stormm_phasespace *ps) {
   return STORMM_OK;
 }
 
-static int total_atoms(const SynthesisPlan &P) {
-  int t = 0;
-  for (int s = 0; s < P.nsys; s++) {
-    t += P.atom_counts[s];
-  }
-  return t;
-}
-
-extern "C" int stormm_phasespace_upload_fixed(stormm_phasespace *ps, const int64_t *xyz) {
-  if (ps == nullptr || xyz == nullptr) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
-  }
-  const SynthesisPlan &P = ps->ag->plan;
-  std::vector<long long> hx(P.padded_atoms, 0), hy(P.padded_atoms, 0), hz(P.padded_atoms, 0);
-  int c = 0;
-  for (int s = 0; s < P.nsys; s++) {
-    for (int i = 0; i < P.atom_counts[s]; i++) {
-      const int g = P.atom_starts[s] + i;
-      hx[g] = xyz[3 * c];
-      hy[g] = xyz[3 * c + 1];
-      hz[g] = xyz[3 * c + 2];
-      c++;
-    }
-  }
-  const size_t n = hx.size() * sizeof(long long);
-  HIP_TRY(hipMemcpy(ps->xcrd.ptr, hx.data(), n, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(ps->ycrd.ptr, hy.data(), n, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(ps->zcrd.ptr, hz.data(), n, hipMemcpyHostToDevice));
-  return STORMM_OK;
-}
-
-extern "C" int stormm_phasespace_upload_coordinates(stormm_phasespace *ps, const double *xyz) {
-  if (ps == nullptr || xyz == nullptr) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
-  }
-  const int tot = total_atoms(ps->ag->plan);
-  std::vector<int64_t> fx(static_cast<size_t>(3) * tot);
-  const double scale = std::ldexp(1.0, ps->gpos_bits);
-  for (size_t k = 0; k < fx.size(); k++) {
-    fx[k] = std::llround(xyz[k] * scale);
-  }
-  return stormm_phasespace_upload_fixed(ps, fx.data());
-}
-
-extern "C" int stormm_phasespace_download_fixed(const stormm_phasespace *ps, int64_t *xyz,
-                                                int64_t *fxyz) {
-  if (ps == nullptr) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
-  }
-  const SynthesisPlan &P = ps->ag->plan;
-  const size_t n = static_cast<size_t>(P.padded_atoms) * sizeof(long long);
-  std::vector<long long> a(P.padded_atoms), b(P.padded_atoms), c(P.padded_atoms);
-  for (int pass = 0; pass < 2; pass++) {
-    int64_t *dst = (pass == 0) ? xyz : fxyz;
-    if (dst == nullptr) {
-      continue;
-    }
-    const DevMem &dx = (pass == 0) ? ps->xcrd : ps->xfrc;
-    const DevMem &dy = (pass == 0) ? ps->ycrd : ps->yfrc;
-    const DevMem &dz = (pass == 0) ? ps->zcrd : ps->zfrc;
-    HIP_TRY(hipMemcpy(a.data(), dx.ptr, n, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(b.data(), dy.ptr, n, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(c.data(), dz.ptr, n, hipMemcpyDeviceToHost));
-    int k = 0;
-    for (int s = 0; s < P.nsys; s++) {
-      for (int i = 0; i < P.atom_counts[s]; i++) {
-        const int g = P.atom_starts[s] + i;
-        dst[3 * k] = a[g];
-        dst[3 * k + 1] = b[g];
-        dst[3 * k + 2] = c[g];
-        k++;
-      }
-    }
-  }
-  return STORMM_OK;
-}
-
-extern "C" int stormm_phasespace_download_coordinates(const stormm_phasespace *ps, double *xyz) {
-  if (ps == nullptr || xyz == nullptr) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
-  }
-  const int tot = total_atoms(ps->ag->plan);
-  std::vector<int64_t> fx(static_cast<size_t>(3) * tot);
-  int rc = stormm_phasespace_download_fixed(ps, fx.data(), nullptr);
-  if (rc != STORMM_OK) {
-    return rc;
-  }
-  const double inv = std::ldexp(1.0, -ps->gpos_bits);
-  for (size_t k = 0; k < fx.size(); k++) {
-    xyz[k] = static_cast<double>(fx[k]) * inv;
-  }
-  return STORMM_OK;
-}
-
-extern "C" int stormm_phasespace_download_forces(const stormm_phasespace *ps, double *fxyz) {
-  if (ps == nullptr || fxyz == nullptr) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
-  }
-  const int tot = total_atoms(ps->ag->plan);
-  std::vector<int64_t> ff(static_cast<size_t>(3) * tot);
-  int rc = stormm_phasespace_download_fixed(ps, nullptr, ff.data());
-  if (rc != STORMM_OK) {
-    return rc;
-  }
-  const double inv = std::ldexp(1.0, -ps->frc_bits);
-  for (size_t k = 0; k < ff.size(); k++) {
-    fxyz[k] = static_cast<double>(ff[k]) * inv;
-  }
-  return STORMM_OK;
-}
-
 // Systems [lo, hi) occupy one contiguous span of device atoms, system after system with each
-// system padded to the wave width; the helpers below move that span in one copy per axis.
+// system padded to the wave width; the helpers below move that span in one copy per array.
 static int system_span(const stormm_phasespace *ps, int lo, int hi, int *begin, int *end) {
   if (ps == nullptr) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
@@ -392,67 +311,239 @@ static int system_span(const stormm_phasespace *ps, int lo, int hi, int *begin, 
   return STORMM_OK;
 }
 
-extern "C" int stormm_phasespace_upload_systems(stormm_phasespace *ps, int32_t lo, int32_t hi,
-                                                const double *xyz) {
+// Coordinates (forces == false) or forces of device atoms [b, e), per axis.
+static int read_span(const stormm_phasespace *ps, bool forces, int b, int e,
+                     std::vector<__int128> out[3]) {
+  const size_t n = static_cast<size_t>(e - b);
+  const DevMem *w[3] = {forces ? &ps->xfrc : &ps->xcrd, forces ? &ps->yfrc : &ps->ycrd,
+                        forces ? &ps->zfrc : &ps->zcrd};
+  const DevMem *o[3] = {forces ? &ps->xfrc_o : &ps->xcrd_o, forces ? &ps->yfrc_o : &ps->ycrd_o,
+                        forces ? &ps->zfrc_o : &ps->zcrd_o};
+  const bool wide = forces ? wide_forces(ps) : wide_coordinates(ps);
+  std::vector<long long> lo(n);
+  std::vector<int> hi(n, 0);
+  for (int d = 0; d < 3; d++) {
+    HIP_TRY(hipMemcpy(lo.data(), w[d]->as<long long>() + b, n * sizeof(long long),
+                      hipMemcpyDeviceToHost));
+    if (wide) {
+      HIP_TRY(hipMemcpy(hi.data(), o[d]->as<int>() + b, n * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    out[d].resize(n);
+    for (size_t i = 0; i < n; i++) {
+      out[d][i] = i95_value(lo[i], hi[i]);
+    }
+  }
+  return STORMM_OK;
+}
+
+// Coordinates of device atoms [b, e), per axis (padding atoms keep the values given).
+static int write_span_coordinates(stormm_phasespace *ps, int b, int e,
+                                  const std::vector<__int128> v[3]) {
+  const size_t n = static_cast<size_t>(e - b);
+  DevMem *w[3] = {&ps->xcrd, &ps->ycrd, &ps->zcrd};
+  DevMem *o[3] = {&ps->xcrd_o, &ps->ycrd_o, &ps->zcrd_o};
+  const bool wide = wide_coordinates(ps);
+  std::vector<long long> lo(n);
+  std::vector<int> hi(n);
+  for (int d = 0; d < 3; d++) {
+    for (size_t i = 0; i < n; i++) {
+      if (wide) {
+        i95_split(v[d][i], &lo[i], &hi[i]);
+      }
+      else {
+        const __int128 x = v[d][i];
+        if (x > static_cast<__int128>(LLONG_MAX) || x < static_cast<__int128>(LLONG_MIN)) {
+          return fail(STORMM_ERR_INVALID_ARGUMENT, "coordinate beyond the 64-bit range of " +
+                                                       std::to_string(ps->gpos_bits) + " bits");
+        }
+        lo[i] = static_cast<long long>(x);
+      }
+    }
+    HIP_TRY(hipMemcpy(w[d]->as<long long>() + b, lo.data(), n * sizeof(long long),
+                      hipMemcpyHostToDevice));
+    if (wide) {
+      HIP_TRY(hipMemcpy(o[d]->as<int>() + b, hi.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    }
+  }
+  return STORMM_OK;
+}
+
+// Coordinates of systems [lo, hi) from [atoms][3] fixed-precision values.
+static int put_systems(stormm_phasespace *ps, int lo, int hi,
+                       const std::function<__int128(size_t)> &value) {
   int b = 0, e = 0;
   int rc = system_span(ps, lo, hi, &b, &e);
   if (rc != STORMM_OK) {
     return rc;
   }
-  if (xyz == nullptr) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "null coordinates");
-  }
   const SynthesisPlan &P = ps->ag->plan;
-  const double scale = std::ldexp(1.0, ps->gpos_bits);
-  std::vector<long long> hx(e - b, 0), hy(e - b, 0), hz(e - b, 0);
+  std::vector<__int128> v[3];
+  for (int d = 0; d < 3; d++) {
+    v[d].assign(static_cast<size_t>(e - b), 0);
+  }
   size_t c = 0;
   for (int s = lo; s < hi; s++) {
     for (int i = 0; i < P.atom_counts[s]; i++) {
       const int g = P.atom_starts[s] + i - b;
-      hx[g] = std::llround(xyz[3 * c] * scale);
-      hy[g] = std::llround(xyz[3 * c + 1] * scale);
-      hz[g] = std::llround(xyz[3 * c + 2] * scale);
+      for (int d = 0; d < 3; d++) {
+        v[d][g] = value(3 * c + d);
+      }
       c++;
     }
   }
-  const size_t n = hx.size() * sizeof(long long);
-  HIP_TRY(hipMemcpy(ps->xcrd.as<long long>() + b, hx.data(), n, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(ps->ycrd.as<long long>() + b, hy.data(), n, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(ps->zcrd.as<long long>() + b, hz.data(), n, hipMemcpyHostToDevice));
-  return STORMM_OK;
+  return write_span_coordinates(ps, b, e, v);
 }
 
-extern "C" int stormm_phasespace_download_systems(const stormm_phasespace *ps, int32_t lo,
-                                                  int32_t hi, double *xyz, double *fxyz) {
+// Coordinates or forces of systems [lo, hi) into [atoms][3] outputs.
+static int get_systems(const stormm_phasespace *ps, bool forces, int lo, int hi,
+                       const std::function<void(size_t, __int128)> &take) {
   int b = 0, e = 0;
   int rc = system_span(ps, lo, hi, &b, &e);
   if (rc != STORMM_OK) {
     return rc;
   }
   const SynthesisPlan &P = ps->ag->plan;
-  const size_t n = static_cast<size_t>(e - b) * sizeof(long long);
-  std::vector<long long> a(e - b), bb(e - b), cc(e - b);
+  std::vector<__int128> v[3];
+  if ((rc = read_span(ps, forces, b, e, v)) != STORMM_OK) {
+    return rc;
+  }
+  size_t c = 0;
+  for (int s = lo; s < hi; s++) {
+    for (int i = 0; i < P.atom_counts[s]; i++) {
+      const int g = P.atom_starts[s] + i - b;
+      for (int d = 0; d < 3; d++) {
+        take(3 * c + d, v[d][g]);
+      }
+      c++;
+    }
+  }
+  return STORMM_OK;
+}
+
+// Real value of a fixed-precision integer in units of 2^-bits.
+static double fixed_to_real(__int128 v, int bits) {
+  const double inv = std::ldexp(1.0, -bits);
+  if (v <= static_cast<__int128>(LLONG_MAX) && v >= static_cast<__int128>(LLONG_MIN)) {
+    return static_cast<double>(static_cast<long long>(v)) * inv;
+  }
+  return i128_to_double(v) * inv;
+}
+
+extern "C" int stormm_phasespace_upload_fixed_split(stormm_phasespace *ps, const int64_t *xyz,
+                                                    const int32_t *xyz_ovrf) {
+  if (ps == nullptr || xyz == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  return put_systems(ps, 0, ps->ag->plan.nsys, [&](size_t k) {
+    return i95_value(xyz[k], (xyz_ovrf != nullptr) ? xyz_ovrf[k] : 0);
+  });
+}
+
+extern "C" int stormm_phasespace_upload_fixed(stormm_phasespace *ps, const int64_t *xyz) {
+  return stormm_phasespace_upload_fixed_split(ps, xyz, nullptr);
+}
+
+extern "C" int stormm_phasespace_upload_coordinates(stormm_phasespace *ps, const double *xyz) {
+  if (ps == nullptr || xyz == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  const double scale = std::ldexp(1.0, ps->gpos_bits);
+  return put_systems(ps, 0, ps->ag->plan.nsys,
+                     [&](size_t k) { return host_fixed(xyz[k] * scale); });
+}
+
+extern "C" int stormm_phasespace_download_fixed_split(const stormm_phasespace *ps, int64_t *xyz,
+                                                      int32_t *xyz_ovrf, int64_t *fxyz,
+                                                      int32_t *fxyz_ovrf) {
+  if (ps == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  for (int pass = 0; pass < 2; pass++) {
+    int64_t *w = (pass == 0) ? xyz : fxyz;
+    int32_t *o = (pass == 0) ? xyz_ovrf : fxyz_ovrf;
+    if (w == nullptr) {
+      continue;
+    }
+    bool overflow = false;
+    const int rc = get_systems(ps, pass == 1, 0, ps->ag->plan.nsys, [&](size_t k, __int128 v) {
+      if (o != nullptr) {
+        long long x;
+        int y;
+        i95_split(v, &x, &y);
+        if (y == 0 || v > static_cast<__int128>(LLONG_MAX) ||
+            v < static_cast<__int128>(LLONG_MIN)) {
+          w[k] = x;
+          o[k] = y;
+        }
+        else {
+          w[k] = static_cast<long long>(v);  // values of one word keep an overflow word of 0
+          o[k] = 0;
+        }
+      }
+      else {
+        overflow = overflow || v > static_cast<__int128>(LLONG_MAX) ||
+                   v < static_cast<__int128>(LLONG_MIN);
+        w[k] = static_cast<long long>(v);
+      }
+    });
+    if (rc != STORMM_OK) {
+      return rc;
+    }
+    if (overflow) {
+      return fail(STORMM_ERR_STATE, "values exceed 64 bits: download them with their overflow "
+                                    "words");
+    }
+  }
+  return STORMM_OK;
+}
+
+extern "C" int stormm_phasespace_download_fixed(const stormm_phasespace *ps, int64_t *xyz,
+                                                int64_t *fxyz) {
+  return stormm_phasespace_download_fixed_split(ps, xyz, nullptr, fxyz, nullptr);
+}
+
+extern "C" int stormm_phasespace_download_coordinates(const stormm_phasespace *ps, double *xyz) {
+  if (ps == nullptr || xyz == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  return get_systems(ps, false, 0, ps->ag->plan.nsys, [&](size_t k, __int128 v) {
+    xyz[k] = fixed_to_real(v, ps->gpos_bits);
+  });
+}
+
+extern "C" int stormm_phasespace_download_forces(const stormm_phasespace *ps, double *fxyz) {
+  if (ps == nullptr || fxyz == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  return get_systems(ps, true, 0, ps->ag->plan.nsys, [&](size_t k, __int128 v) {
+    fxyz[k] = fixed_to_real(v, ps->frc_bits);
+  });
+}
+
+extern "C" int stormm_phasespace_upload_systems(stormm_phasespace *ps, int32_t lo, int32_t hi,
+                                                const double *xyz) {
+  if (ps == nullptr || xyz == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null phase space or coordinates");
+  }
+  const double scale = std::ldexp(1.0, ps->gpos_bits);
+  return put_systems(ps, lo, hi, [&](size_t k) { return host_fixed(xyz[k] * scale); });
+}
+
+extern "C" int stormm_phasespace_download_systems(const stormm_phasespace *ps, int32_t lo,
+                                                  int32_t hi, double *xyz, double *fxyz) {
+  if (ps == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
+  }
   for (int pass = 0; pass < 2; pass++) {
     double *dst = (pass == 0) ? xyz : fxyz;
     if (dst == nullptr) {
       continue;
     }
-    const DevMem &dx = (pass == 0) ? ps->xcrd : ps->xfrc;
-    const DevMem &dy = (pass == 0) ? ps->ycrd : ps->yfrc;
-    const DevMem &dz = (pass == 0) ? ps->zcrd : ps->zfrc;
-    HIP_TRY(hipMemcpy(a.data(), dx.as<long long>() + b, n, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(bb.data(), dy.as<long long>() + b, n, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(cc.data(), dz.as<long long>() + b, n, hipMemcpyDeviceToHost));
-    const double inv = std::ldexp(1.0, (pass == 0) ? -ps->gpos_bits : -ps->frc_bits);
-    size_t k = 0;
-    for (int s = lo; s < hi; s++) {
-      for (int i = 0; i < P.atom_counts[s]; i++) {
-        const int g = P.atom_starts[s] + i - b;
-        dst[3 * k] = static_cast<double>(a[g]) * inv;
-        dst[3 * k + 1] = static_cast<double>(bb[g]) * inv;
-        dst[3 * k + 2] = static_cast<double>(cc[g]) * inv;
-        k++;
-      }
+    const int bits = (pass == 0) ? ps->gpos_bits : ps->frc_bits;
+    const int rc = get_systems(ps, pass == 1, lo, hi,
+                               [&](size_t k, __int128 v) { dst[k] = fixed_to_real(v, bits); });
+    if (rc != STORMM_OK) {
+      return rc;
     }
   }
   return STORMM_OK;
@@ -472,6 +563,7 @@ extern "C" int stormm_phasespace_writer(stormm_phasespace *ps, stormm_ps_writer 
   if (ps == nullptr || w == nullptr) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
   }
+  std::memset(w, 0, sizeof(*w));
   w->system_count = ps->ag->plan.nsys;
   w->gpos_bits = ps->gpos_bits;
   w->frc_bits = ps->frc_bits;
@@ -493,29 +585,80 @@ extern "C" int stormm_phasespace_writer(stormm_phasespace *ps, stormm_ps_writer 
   w->x_cg_temp = ps->xcg.as<int64_t>();
   w->y_cg_temp = ps->ycg.as<int64_t>();
   w->z_cg_temp = ps->zcg.as<int64_t>();
+  w->xcrd_ovrf = ps->xcrd_o.as<int32_t>();
+  w->ycrd_ovrf = ps->ycrd_o.as<int32_t>();
+  w->zcrd_ovrf = ps->zcrd_o.as<int32_t>();
+  w->xfrc_ovrf = ps->xfrc_o.as<int32_t>();
+  w->yfrc_ovrf = ps->yfrc_o.as<int32_t>();
+  w->zfrc_ovrf = ps->zfrc_o.as<int32_t>();
+  w->xprv_ovrf = ps->xprv_o.as<int32_t>();
+  w->yprv_ovrf = ps->yprv_o.as<int32_t>();
+  w->zprv_ovrf = ps->zprv_o.as<int32_t>();
+  w->x_cg_temp_ovrf = ps->xcg_o.as<int32_t>();
+  w->y_cg_temp_ovrf = ps->ycg_o.as<int32_t>();
+  w->z_cg_temp_ovrf = ps->zcg_o.as<int32_t>();
   return STORMM_OK;
+}
+
+// Zeroes the words (and, above the plain limits, the overflow words) of up to three groups of
+// per-atom arrays of a writer.
+static int zero_writer_arrays(int64_t *a, int64_t *b, int64_t *c, int32_t *ao, int32_t *bo,
+                              int32_t *co, bool wide, size_t n, hipStream_t st) {
+  HIP_TRY(launch_fill(reinterpret_cast<long long *>(a), reinterpret_cast<long long *>(b),
+                      reinterpret_cast<long long *>(c), n, st));
+  if (wide) {
+    if (ao == nullptr || bo == nullptr || co == nullptr) {
+      return fail(STORMM_ERR_INVALID_ARGUMENT, "overflow arrays are required above 40 force "
+                                               "bits");
+    }
+    HIP_TRY(launch_fill_int(ao, bo, co, n, st));
+  }
+  return STORMM_OK;
+}
+
+extern "C" int stormm_ps_writer_initialize_forces(const stormm_ps_writer *w, int32_t padded_atoms,
+                                                  void *stream) {
+  if (w == nullptr || padded_atoms < 0 || w->xfrc == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "invalid writer");
+  }
+  return zero_writer_arrays(w->xfrc, w->yfrc, w->zfrc, w->xfrc_ovrf, w->yfrc_ovrf, w->zfrc_ovrf,
+                            w->frc_bits > kMaxPlainFrcBits, static_cast<size_t>(padded_atoms),
+                            static_cast<hipStream_t>(stream));
+}
+
+extern "C" int stormm_ps_writer_prime_conjugate_gradient(const stormm_ps_writer *w,
+                                                         int32_t padded_atoms, void *stream) {
+  if (w == nullptr || padded_atoms < 0 || w->xprv == nullptr || w->x_cg_temp == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "invalid writer");
+  }
+  const bool wide = w->frc_bits > kMaxPlainFrcBits;
+  const size_t n = static_cast<size_t>(padded_atoms);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int rc = zero_writer_arrays(w->xprv, w->yprv, w->zprv, w->xprv_ovrf, w->yprv_ovrf,
+                              w->zprv_ovrf, wide, n, st);
+  if (rc == STORMM_OK) {
+    rc = zero_writer_arrays(w->x_cg_temp, w->y_cg_temp, w->z_cg_temp, w->x_cg_temp_ovrf,
+                            w->y_cg_temp_ovrf, w->z_cg_temp_ovrf, wide, n, st);
+  }
+  return rc;
 }
 
 extern "C" int stormm_phasespace_initialize_forces(stormm_phasespace *ps, void *stream) {
   if (ps == nullptr) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
   }
-  HIP_TRY(launch_fill(ps->xfrc.as<long long>(), ps->yfrc.as<long long>(),
-                      ps->zfrc.as<long long>(), ps->ag->plan.padded_atoms,
-                      static_cast<hipStream_t>(stream)));
-  return STORMM_OK;
+  stormm_ps_writer w;
+  stormm_phasespace_writer(ps, &w);
+  return stormm_ps_writer_initialize_forces(&w, ps->ag->plan.padded_atoms, stream);
 }
 
 extern "C" int stormm_phasespace_prime_conjugate_gradient(stormm_phasespace *ps, void *stream) {
   if (ps == nullptr) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
   }
-  const size_t n = ps->ag->plan.padded_atoms;
-  HIP_TRY(launch_fill(ps->xprv.as<long long>(), ps->yprv.as<long long>(),
-                      ps->zprv.as<long long>(), n, static_cast<hipStream_t>(stream)));
-  HIP_TRY(launch_fill(ps->xcg.as<long long>(), ps->ycg.as<long long>(), ps->zcg.as<long long>(),
-                      n, static_cast<hipStream_t>(stream)));
-  return STORMM_OK;
+  stormm_ps_writer w;
+  stormm_phasespace_writer(ps, &w);
+  return stormm_ps_writer_prime_conjugate_gradient(&w, ps->ag->plan.padded_atoms, stream);
 }
 
 // ---- ScoreCard -----------------------------------------------------------------------------
@@ -599,6 +742,29 @@ extern "C" int64_t *stormm_scorecard_device_accumulators(stormm_scorecard *sc) {
   return (sc == nullptr) ? nullptr : sc->inst.as<int64_t>();
 }
 
+extern "C" int stormm_scorecard_data(stormm_scorecard *sc, stormm_scorecard_writer *w) {
+  if (sc == nullptr || w == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  w->system_count = sc->nsys;
+  w->data_stride = kScoreStride;
+  w->nrg_bits = sc->nrg_bits;
+  w->instantaneous_accumulators = sc->inst.as<int64_t>();
+  return STORMM_OK;
+}
+
+extern "C" int stormm_scorecard_writer_initialize(const stormm_scorecard_writer *w,
+                                                  void *stream) {
+  if (w == nullptr || w->instantaneous_accumulators == nullptr || w->system_count < 0 ||
+      w->data_stride <= 0) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "invalid ScoreCard writer");
+  }
+  HIP_TRY(launch_fill(reinterpret_cast<long long *>(w->instantaneous_accumulators), nullptr,
+                      nullptr, static_cast<size_t>(w->system_count) * w->data_stride,
+                      static_cast<hipStream_t>(stream)));
+  return STORMM_OK;
+}
+
 // ---- Controls and line-minimization records ------------------------------------------------
 extern "C" int stormm_mm_controls_create(const stormm_minimize_controls *mincon,
                                          stormm_mm_controls **out) {
@@ -643,6 +809,15 @@ extern "C" int stormm_mm_controls_set_step(stormm_mm_controls *ctrl, int32_t ste
   return STORMM_OK;
 }
 
+extern "C" int stormm_mm_controls_data(const stormm_mm_controls *ctrl, stormm_mm_kit *kit) {
+  if (ctrl == nullptr || kit == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  kit->step = ctrl->step;
+  kit->sd_cycles = ctrl->mc.steepest_descent_cycles;
+  return STORMM_OK;
+}
+
 extern "C" int stormm_line_min_create(int32_t system_count, double dx0, stormm_line_min **out) {
   if (out == nullptr || system_count <= 0) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "invalid line-minimization request");
@@ -663,6 +838,25 @@ extern "C" int stormm_line_min_destroy(stormm_line_min *lm) {
   return STORMM_OK;
 }
 
+extern "C" int stormm_line_min_data(stormm_line_min *lm, stormm_linmin_writer *w) {
+  if (lm == nullptr || w == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  double *v = lm->vals.as<double>();
+  const size_t n = static_cast<size_t>(lm->nsys);
+  w->system_count = lm->nsys;
+  w->l_move = v;
+  w->s_move = v + n;
+  w->mfac_a = v + 2 * n;
+  w->mfac_b = v + 3 * n;
+  w->mfac_c = v + 4 * n;
+  w->nrg_a = v + 5 * n;
+  w->nrg_b = v + 6 * n;
+  w->nrg_c = v + 7 * n;
+  w->nrg_d = v + 8 * n;
+  return STORMM_OK;
+}
+
 extern "C" int stormm_line_min_download(const stormm_line_min *lm, double *values9xN) {
   if (lm == nullptr || values9xN == nullptr) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
@@ -673,6 +867,64 @@ extern "C" int stormm_line_min_download(const stormm_line_min *lm, double *value
 }
 
 // ---- Launches ------------------------------------------------------------------------------
+// Every launch runs on writer views: the handle entry points take the views of their objects,
+// the kit entry points the caller's.
+struct ScView {
+  int nsys = 0, nrg_bits = 26;
+  long long *acc = nullptr;
+};
+struct LmView {
+  int nsys = 0;
+  double *v[9] = {};
+};
+
+static ScView sc_view(stormm_scorecard *sc) {
+  ScView v;
+  if (sc != nullptr) {
+    v.nsys = sc->nsys;
+    v.nrg_bits = sc->nrg_bits;
+    v.acc = sc->inst.as<long long>();
+  }
+  return v;
+}
+
+static int sc_view(const stormm_scorecard_writer *w, ScView *out) {
+  if (w == nullptr) {
+    return STORMM_OK;
+  }
+  if (w->instantaneous_accumulators == nullptr || w->data_stride != kScoreStride ||
+      w->nrg_bits < 22 || w->nrg_bits > 40) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "ScoreCard writer needs accumulators, a data "
+                                             "stride of 32 and energy bits in [22, 40]");
+  }
+  out->nsys = w->system_count;
+  out->nrg_bits = w->nrg_bits;
+  out->acc = reinterpret_cast<long long *>(w->instantaneous_accumulators);
+  return STORMM_OK;
+}
+
+static LmView lm_view(stormm_line_min *lm) {
+  LmView v;
+  v.nsys = lm->nsys;
+  for (int j = 0; j < 9; j++) {
+    v.v[j] = lm->vals.as<double>() + static_cast<size_t>(j) * lm->nsys;
+  }
+  return v;
+}
+
+static int lm_view(const stormm_linmin_writer *w, LmView *out) {
+  double *f[9] = {w->l_move, w->s_move, w->mfac_a, w->mfac_b, w->mfac_c,
+                  w->nrg_a,  w->nrg_b,  w->nrg_c,  w->nrg_d};
+  out->nsys = w->system_count;
+  for (int j = 0; j < 9; j++) {
+    if (f[j] == nullptr) {
+      return fail(STORMM_ERR_INVALID_ARGUMENT, "line-minimization writer with a null array");
+    }
+    out->v[j] = f[j];
+  }
+  return STORMM_OK;
+}
+
 static int check_objects(const stormm_synthesis *ag, const stormm_phasespace *ps,
                          const stormm_scorecard *sc) {
   if (ag == nullptr || ps == nullptr) {
@@ -691,18 +943,90 @@ static int check_objects(const stormm_synthesis *ag, const stormm_phasespace *ps
   return STORMM_OK;
 }
 
-static MMKit make_kit(const stormm_synthesis *ag, stormm_phasespace *ps, stormm_scorecard *sc) {
+// A writer fit for the synthesis: system count, bit ranges, the arrays a launch touches, and
+// the overflow arrays above the plain limits.  `cg`: the conjugate-gradient arrays are needed.
+static int check_writer(const stormm_synthesis *ag, const stormm_ps_writer *w, int precision,
+                        const ScView *sc, bool cg) {
+  if (ag == nullptr || w == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null synthesis or phase-space writer");
+  }
+  if (precision != STORMM_PREC_SINGLE && precision != STORMM_PREC_DOUBLE) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "precision must be STORMM_PREC_SINGLE or "
+                                             "STORMM_PREC_DOUBLE");
+  }
+  if (w->system_count != ag->plan.nsys) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "the phase-space writer holds " +
+                                                 std::to_string(w->system_count) +
+                                                 " systems, the synthesis " +
+                                                 std::to_string(ag->plan.nsys));
+  }
+  if (sc != nullptr && sc->acc != nullptr && sc->nsys != ag->plan.nsys) {
+    // src/MolecularMechanics/hpc_minimization.cu:151-155
+    return fail(STORMM_ERR_INVALID_ARGUMENT,
+                "The energy tracking object is not prepared for the number of systems contained "
+                "in the topology synthesis (" + std::to_string(sc->nsys) + " vs. " +
+                    std::to_string(ag->plan.nsys) + ").");
+  }
+  if (w->gpos_bits < 24 || w->gpos_bits > 72 || w->frc_bits < 18 || w->frc_bits > 72) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "fixed-precision bits out of range (coordinates "
+                                             "24-72, forces 18-72)");
+  }
+  if (precision == STORMM_PREC_SINGLE && w->frc_bits > kMaxPlainFrcBits) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "single-precision launches take at most 40 force "
+                                             "bits");
+  }
+  if (w->xcrd == nullptr || w->ycrd == nullptr || w->zcrd == nullptr || w->xfrc == nullptr ||
+      w->yfrc == nullptr || w->zfrc == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "phase-space writer with null coordinate or force "
+                                             "arrays");
+  }
+  if (w->gpos_bits > kMaxPlainGposBits &&
+      (w->xcrd_ovrf == nullptr || w->ycrd_ovrf == nullptr || w->zcrd_ovrf == nullptr)) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "coordinates above 38 bits need overflow arrays");
+  }
+  if (w->frc_bits > kMaxPlainFrcBits &&
+      (w->xfrc_ovrf == nullptr || w->yfrc_ovrf == nullptr || w->zfrc_ovrf == nullptr)) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "forces above 40 bits need overflow arrays");
+  }
+  if (cg) {
+    if (w->xprv == nullptr || w->yprv == nullptr || w->zprv == nullptr ||
+        w->x_cg_temp == nullptr || w->y_cg_temp == nullptr || w->z_cg_temp == nullptr) {
+      return fail(STORMM_ERR_INVALID_ARGUMENT, "conjugate gradient needs the previous-force and "
+                                               "direction arrays");
+    }
+    if (w->frc_bits > kMaxPlainFrcBits &&
+        (w->xprv_ovrf == nullptr || w->yprv_ovrf == nullptr || w->zprv_ovrf == nullptr ||
+         w->x_cg_temp_ovrf == nullptr || w->y_cg_temp_ovrf == nullptr ||
+         w->z_cg_temp_ovrf == nullptr)) {
+      return fail(STORMM_ERR_INVALID_ARGUMENT, "conjugate gradient above 40 force bits needs "
+                                               "the overflow arrays");
+    }
+  }
+  return STORMM_OK;
+}
+
+static MMKit make_kit(const stormm_synthesis *ag, const stormm_ps_writer &w, const ScView *sc) {
   MMKit k = ag->kit;
-  k.gpos_bits = ps->gpos_bits;
-  k.frc_bits = ps->frc_bits;
-  k.nrg_bits = (sc != nullptr) ? sc->nrg_bits : 26;
-  k.xcrd = ps->xcrd.as<long long>();
-  k.ycrd = ps->ycrd.as<long long>();
-  k.zcrd = ps->zcrd.as<long long>();
-  k.xfrc = ps->xfrc.as<long long>();
-  k.yfrc = ps->yfrc.as<long long>();
-  k.zfrc = ps->zfrc.as<long long>();
-  k.scorecard = (sc != nullptr) ? sc->inst.as<long long>() : nullptr;
+  k.gpos_bits = w.gpos_bits;
+  // Forces above 40 bits are accumulated at 40 bits inside a work unit and shifted on store.
+  k.frc_bits = std::min(w.frc_bits, kMaxPlainFrcBits);
+  k.frc_store_shift = w.frc_bits - k.frc_bits;
+  k.nrg_bits = (sc != nullptr && sc->acc != nullptr) ? sc->nrg_bits : 26;
+  k.xcrd = reinterpret_cast<long long *>(w.xcrd);
+  k.ycrd = reinterpret_cast<long long *>(w.ycrd);
+  k.zcrd = reinterpret_cast<long long *>(w.zcrd);
+  k.xfrc = reinterpret_cast<long long *>(w.xfrc);
+  k.yfrc = reinterpret_cast<long long *>(w.yfrc);
+  k.zfrc = reinterpret_cast<long long *>(w.zfrc);
+  const bool wide_c = w.gpos_bits > kMaxPlainGposBits;
+  const bool wide_f = w.frc_bits > kMaxPlainFrcBits;
+  k.xcrd_ovrf = wide_c ? w.xcrd_ovrf : nullptr;
+  k.ycrd_ovrf = wide_c ? w.ycrd_ovrf : nullptr;
+  k.zcrd_ovrf = wide_c ? w.zcrd_ovrf : nullptr;
+  k.xfrc_ovrf = wide_f ? w.xfrc_ovrf : nullptr;
+  k.yfrc_ovrf = wide_f ? w.yfrc_ovrf : nullptr;
+  k.zfrc_ovrf = wide_f ? w.zfrc_ovrf : nullptr;
+  k.scorecard = (sc != nullptr) ? sc->acc : nullptr;
   k.clash_variant = 0;
   k.clash_cycles = 0;
   k.clash_cycle = 0;
@@ -757,24 +1081,21 @@ static int check_mincon(const stormm_minimize_controls *mc) {
   return STORMM_OK;
 }
 
-static int launch_terms(int32_t precision, const stormm_synthesis *ag, stormm_phasespace *ps,
-                        stormm_scorecard *sc, int terms, int32_t eval_force, int32_t eval_energy,
-                        double clash_d, double clash_r, int step, void *stream) {
-  int rc = check_objects(ag, ps, sc);
+static int launch_terms(int32_t precision, const stormm_synthesis *ag,
+                        const stormm_ps_writer &w, const ScView &sc, int terms,
+                        int32_t eval_force, int32_t eval_energy, double clash_d, double clash_r,
+                        int step, void *stream) {
+  int rc = check_writer(ag, &w, precision, &sc, false);
   if (rc != STORMM_OK) {
     return rc;
   }
-  if (precision != STORMM_PREC_SINGLE && precision != STORMM_PREC_DOUBLE) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "precision must be STORMM_PREC_SINGLE or "
-                                             "STORMM_PREC_DOUBLE");
-  }
-  if (sc == nullptr && eval_energy) {
+  if (sc.acc == nullptr && eval_energy) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "energy evaluation needs a ScoreCard");
   }
   if (!(clash_d >= 0.0) || !(clash_r >= 0.0)) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "clash parameters must be non-negative");
   }
-  MMKit k = make_kit(ag, ps, sc);
+  MMKit k = make_kit(ag, w, &sc);
   set_clash(k, precision, clash_d, clash_r);
   k.rst_step = step;
   HIP_TRY(launch_mm_eval(precision == STORMM_PREC_DOUBLE ? 0 : 1, k, terms, eval_force ? 1 : 0,
@@ -782,14 +1103,44 @@ static int launch_terms(int32_t precision, const stormm_synthesis *ag, stormm_ph
   return STORMM_OK;
 }
 
+static int launch_terms_handles(int32_t precision, const stormm_synthesis *ag,
+                                stormm_mm_controls *ctrl, stormm_phasespace *ps,
+                                stormm_scorecard *sc, int terms, int32_t eval_force,
+                                int32_t eval_energy, double clash_d, double clash_r,
+                                void *stream) {
+  int rc = check_objects(ag, ps, sc);
+  if (rc != STORMM_OK) {
+    return rc;
+  }
+  stormm_ps_writer w;
+  stormm_phasespace_writer(ps, &w);
+  return launch_terms(precision, ag, w, sc_view(sc), terms, eval_force, eval_energy, clash_d,
+                      clash_r, (ctrl != nullptr) ? ctrl->step : 0, stream);
+}
+
+static int launch_terms_kit(int32_t precision, const stormm_synthesis *ag,
+                            const stormm_mm_kit *ctrl, const stormm_ps_writer *psw,
+                            const stormm_scorecard_writer *scw, int terms, int32_t eval_force,
+                            int32_t eval_energy, double clash_d, double clash_r, void *stream) {
+  ScView sc;
+  int rc = sc_view(scw, &sc);
+  if (rc != STORMM_OK) {
+    return rc;
+  }
+  if (psw == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null phase-space writer");
+  }
+  return launch_terms(precision, ag, *psw, sc, terms, eval_force, eval_energy, clash_d, clash_r,
+                      (ctrl != nullptr) ? ctrl->step : 0, stream);
+}
+
 extern "C" int stormm_launch_nonbonded(int32_t precision, const stormm_synthesis *ag,
                                        stormm_mm_controls *ctrl, stormm_phasespace *ps,
                                        stormm_scorecard *sc, int32_t eval_force,
                                        int32_t eval_energy, double clash_minimum_distance,
                                        double clash_ratio, void *stream) {
-  return launch_terms(precision, ag, ps, sc, kTermNonbonded, eval_force, eval_energy,
-                      clash_minimum_distance, clash_ratio,
-                      (ctrl != nullptr) ? ctrl->step : 0, stream);
+  return launch_terms_handles(precision, ag, ctrl, ps, sc, kTermNonbonded, eval_force,
+                              eval_energy, clash_minimum_distance, clash_ratio, stream);
 }
 
 extern "C" int stormm_launch_valence(int32_t precision, const stormm_synthesis *ag,
@@ -797,9 +1148,8 @@ extern "C" int stormm_launch_valence(int32_t precision, const stormm_synthesis *
                                      stormm_scorecard *sc, int32_t eval_force,
                                      int32_t eval_energy, double clash_minimum_distance,
                                      double clash_ratio, void *stream) {
-  return launch_terms(precision, ag, ps, sc, kTermValence, eval_force, eval_energy,
-                      clash_minimum_distance, clash_ratio,
-                      (ctrl != nullptr) ? ctrl->step : 0, stream);
+  return launch_terms_handles(precision, ag, ctrl, ps, sc, kTermValence, eval_force,
+                              eval_energy, clash_minimum_distance, clash_ratio, stream);
 }
 
 extern "C" int stormm_launch_mm_evaluation(int32_t precision, const stormm_synthesis *ag,
@@ -807,29 +1157,83 @@ extern "C" int stormm_launch_mm_evaluation(int32_t precision, const stormm_synth
                                            stormm_scorecard *sc, int32_t eval_force,
                                            int32_t eval_energy, double clash_minimum_distance,
                                            double clash_ratio, void *stream) {
-  return launch_terms(precision, ag, ps, sc, kTermNonbonded | kTermValence, eval_force,
-                      eval_energy, clash_minimum_distance, clash_ratio,
-                      (ctrl != nullptr) ? ctrl->step : 0, stream);
+  return launch_terms_handles(precision, ag, ctrl, ps, sc, kTermNonbonded | kTermValence,
+                              eval_force, eval_energy, clash_minimum_distance, clash_ratio,
+                              stream);
 }
 
-static CgKit make_cg_kit(const stormm_synthesis *ag, stormm_phasespace *ps, int precision) {
+extern "C" int stormm_launch_nonbonded_kit(int32_t precision, const stormm_synthesis *ag,
+                                           const stormm_mm_kit *ctrl,
+                                           const stormm_ps_writer *psw,
+                                           const stormm_scorecard_writer *scw,
+                                           int32_t eval_force, int32_t eval_energy,
+                                           double clash_minimum_distance, double clash_ratio,
+                                           void *stream) {
+  return launch_terms_kit(precision, ag, ctrl, psw, scw, kTermNonbonded, eval_force,
+                          eval_energy, clash_minimum_distance, clash_ratio, stream);
+}
+
+extern "C" int stormm_launch_valence_kit(int32_t precision, const stormm_synthesis *ag,
+                                         const stormm_mm_kit *ctrl, const stormm_ps_writer *psw,
+                                         const stormm_scorecard_writer *scw, int32_t eval_force,
+                                         int32_t eval_energy, double clash_minimum_distance,
+                                         double clash_ratio, void *stream) {
+  return launch_terms_kit(precision, ag, ctrl, psw, scw, kTermValence, eval_force, eval_energy,
+                          clash_minimum_distance, clash_ratio, stream);
+}
+
+extern "C" int stormm_launch_mm_evaluation_kit(int32_t precision, const stormm_synthesis *ag,
+                                               const stormm_mm_kit *ctrl,
+                                               const stormm_ps_writer *psw,
+                                               const stormm_scorecard_writer *scw,
+                                               int32_t eval_force, int32_t eval_energy,
+                                               double clash_minimum_distance, double clash_ratio,
+                                               void *stream) {
+  return launch_terms_kit(precision, ag, ctrl, psw, scw, kTermNonbonded | kTermValence,
+                          eval_force, eval_energy, clash_minimum_distance, clash_ratio, stream);
+}
+
+static CgKit make_cg_kit(const stormm_synthesis *ag, const stormm_ps_writer &w, int precision) {
   CgKit k;
   std::memset(&k, 0, sizeof(k));
   k.nsys = ag->plan.nsys;
   k.sys_start = ag->atom_starts.as<int>();
   k.sys_natom = ag->atom_counts.as<int>();
-  k.xfrc = ps->xfrc.as<long long>();
-  k.yfrc = ps->yfrc.as<long long>();
-  k.zfrc = ps->zfrc.as<long long>();
-  k.xprv = ps->xprv.as<long long>();
-  k.yprv = ps->yprv.as<long long>();
-  k.zprv = ps->zprv.as<long long>();
-  k.xcg = ps->xcg.as<long long>();
-  k.ycg = ps->ycg.as<long long>();
-  k.zcg = ps->zcg.as<long long>();
-  k.inv_frc_scale = std::ldexp(1.0, -ps->frc_bits);
+  k.xfrc = reinterpret_cast<long long *>(w.xfrc);
+  k.yfrc = reinterpret_cast<long long *>(w.yfrc);
+  k.zfrc = reinterpret_cast<long long *>(w.zfrc);
+  k.xprv = reinterpret_cast<long long *>(w.xprv);
+  k.yprv = reinterpret_cast<long long *>(w.yprv);
+  k.zprv = reinterpret_cast<long long *>(w.zprv);
+  k.xcg = reinterpret_cast<long long *>(w.x_cg_temp);
+  k.ycg = reinterpret_cast<long long *>(w.y_cg_temp);
+  k.zcg = reinterpret_cast<long long *>(w.z_cg_temp);
+  if (w.frc_bits > kMaxPlainFrcBits) {
+    k.xfrc_ovrf = w.xfrc_ovrf;
+    k.yfrc_ovrf = w.yfrc_ovrf;
+    k.zfrc_ovrf = w.zfrc_ovrf;
+    k.xprv_ovrf = w.xprv_ovrf;
+    k.yprv_ovrf = w.yprv_ovrf;
+    k.zprv_ovrf = w.zprv_ovrf;
+    k.xcg_ovrf = w.x_cg_temp_ovrf;
+    k.ycg_ovrf = w.y_cg_temp_ovrf;
+    k.zcg_ovrf = w.z_cg_temp_ovrf;
+  }
+  k.inv_frc_scale = std::ldexp(1.0, -w.frc_bits);
   k.dp_mode = (precision == STORMM_PREC_DOUBLE) ? 1 : 0;
   return k;
+}
+
+static int conjugate_gradient(int32_t precision, const stormm_synthesis *ag,
+                              const stormm_ps_writer &w, int step, int sd_cycles,
+                              void *stream) {
+  int rc = check_writer(ag, &w, precision, nullptr, true);
+  if (rc != STORMM_OK) {
+    return rc;
+  }
+  HIP_TRY(launch_conjugate_gradient(make_cg_kit(ag, w, precision), step, sd_cycles,
+                                    static_cast<hipStream_t>(stream)));
+  return STORMM_OK;
 }
 
 extern "C" int stormm_launch_conjugate_gradient(int32_t precision, const stormm_synthesis *ag,
@@ -842,42 +1246,77 @@ extern "C" int stormm_launch_conjugate_gradient(int32_t precision, const stormm_
   if (ctrl == nullptr) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "null controls");
   }
-  HIP_TRY(launch_conjugate_gradient(make_cg_kit(ag, ps, precision), ctrl->step,
-                                    ctrl->mc.steepest_descent_cycles,
-                                    static_cast<hipStream_t>(stream)));
-  return STORMM_OK;
+  stormm_ps_writer w;
+  stormm_phasespace_writer(ps, &w);
+  return conjugate_gradient(precision, ag, w, ctrl->step, ctrl->mc.steepest_descent_cycles,
+                            stream);
 }
 
-static LineKit make_line_kit(const stormm_synthesis *ag, stormm_phasespace *ps,
-                             stormm_scorecard *sc, stormm_line_min *lm, int precision) {
+extern "C" int stormm_launch_conjugate_gradient_kit(int32_t precision, const stormm_synthesis *ag,
+                                                    const stormm_ps_writer *psw,
+                                                    const stormm_mm_kit *ctrl, void *stream) {
+  if (psw == nullptr || ctrl == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null writer or controls kit");
+  }
+  return conjugate_gradient(precision, ag, *psw, ctrl->step, ctrl->sd_cycles, stream);
+}
+
+static LineKit make_line_kit(const stormm_synthesis *ag, const stormm_ps_writer &w,
+                             const ScView &sc, const LmView &lm, int precision) {
   LineKit k;
   std::memset(&k, 0, sizeof(k));
   k.nsys = ag->plan.nsys;
   k.sys_start = ag->atom_starts.as<int>();
   k.sys_natom = ag->atom_counts.as<int>();
-  k.xcrd = ps->xcrd.as<long long>();
-  k.ycrd = ps->ycrd.as<long long>();
-  k.zcrd = ps->zcrd.as<long long>();
-  k.xfrc = ps->xfrc.as<long long>();
-  k.yfrc = ps->yfrc.as<long long>();
-  k.zfrc = ps->zfrc.as<long long>();
-  k.scorecard = sc->inst.as<long long>();
-  k.inv_nrg_scale = std::ldexp(1.0, -sc->nrg_bits);
-  k.inv_frc_scale = std::ldexp(1.0, -ps->frc_bits);
-  k.gpos_scale = std::ldexp(1.0, ps->gpos_bits);
-  double *v = lm->vals.as<double>();
-  const int n = lm->nsys;
-  k.l_move = v;
-  k.s_move = v + n;
-  k.mfac_a = v + 2 * n;
-  k.mfac_b = v + 3 * n;
-  k.mfac_c = v + 4 * n;
-  k.nrg_a = v + 5 * n;
-  k.nrg_b = v + 6 * n;
-  k.nrg_c = v + 7 * n;
-  k.nrg_d = v + 8 * n;
+  k.xcrd = reinterpret_cast<long long *>(w.xcrd);
+  k.ycrd = reinterpret_cast<long long *>(w.ycrd);
+  k.zcrd = reinterpret_cast<long long *>(w.zcrd);
+  k.xfrc = reinterpret_cast<const long long *>(w.xfrc);
+  k.yfrc = reinterpret_cast<const long long *>(w.yfrc);
+  k.zfrc = reinterpret_cast<const long long *>(w.zfrc);
+  if (w.gpos_bits > kMaxPlainGposBits) {
+    k.xcrd_ovrf = w.xcrd_ovrf;
+    k.ycrd_ovrf = w.ycrd_ovrf;
+    k.zcrd_ovrf = w.zcrd_ovrf;
+  }
+  if (w.frc_bits > kMaxPlainFrcBits) {
+    k.xfrc_ovrf = w.xfrc_ovrf;
+    k.yfrc_ovrf = w.yfrc_ovrf;
+    k.zfrc_ovrf = w.zfrc_ovrf;
+  }
+  k.scorecard = sc.acc;
+  k.inv_nrg_scale = std::ldexp(1.0, -sc.nrg_bits);
+  k.inv_frc_scale = std::ldexp(1.0, -w.frc_bits);
+  k.gpos_scale = std::ldexp(1.0, w.gpos_bits);
+  k.l_move = lm.v[0];
+  k.s_move = lm.v[1];
+  k.mfac_a = lm.v[2];
+  k.mfac_b = lm.v[3];
+  k.mfac_c = lm.v[4];
+  k.nrg_a = lm.v[5];
+  k.nrg_b = lm.v[6];
+  k.nrg_c = lm.v[7];
+  k.nrg_d = lm.v[8];
   k.dp_mode = (precision == STORMM_PREC_DOUBLE) ? 1 : 0;
   return k;
+}
+
+static int line_advance(int32_t precision, const stormm_synthesis *ag, const stormm_ps_writer &w,
+                        const ScView &sc, const LmView &lm, int move_number, void *stream) {
+  int rc = check_writer(ag, &w, precision, &sc, false);
+  if (rc != STORMM_OK) {
+    return rc;
+  }
+  if (sc.acc == nullptr || lm.nsys != ag->plan.nsys) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "line advance needs a ScoreCard and a matching "
+                                             "LineMinimization record");
+  }
+  if (move_number < 0 || move_number > 3) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "move_number must be 0..3");
+  }
+  HIP_TRY(launch_line_advance(make_line_kit(ag, w, sc, lm, precision), move_number,
+                              static_cast<hipStream_t>(stream)));
+  return STORMM_OK;
 }
 
 extern "C" int stormm_launch_line_advance(int32_t precision, const stormm_synthesis *ag,
@@ -888,35 +1327,50 @@ extern "C" int stormm_launch_line_advance(int32_t precision, const stormm_synthe
   if (rc != STORMM_OK) {
     return rc;
   }
-  if (sc == nullptr || lm == nullptr || lm->nsys != ag->plan.nsys) {
+  if (sc == nullptr || lm == nullptr) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "line advance needs a ScoreCard and a matching "
                                              "LineMinimization record");
   }
-  if (move_number < 0 || move_number > 3) {
-    return fail(STORMM_ERR_INVALID_ARGUMENT, "move_number must be 0..3");
+  stormm_ps_writer w;
+  stormm_phasespace_writer(ps, &w);
+  return line_advance(precision, ag, w, sc_view(sc), lm_view(lm), move_number, stream);
+}
+
+extern "C" int stormm_launch_line_advance_kit(int32_t precision, const stormm_synthesis *ag,
+                                              const stormm_ps_writer *psw,
+                                              const stormm_scorecard_writer *scw,
+                                              const stormm_linmin_writer *lmw,
+                                              int32_t move_number, void *stream) {
+  if (psw == nullptr || scw == nullptr || lmw == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null writer");
   }
-  HIP_TRY(launch_line_advance(make_line_kit(ag, ps, sc, lm, precision), move_number,
-                              static_cast<hipStream_t>(stream)));
-  return STORMM_OK;
+  ScView sc;
+  LmView lm;
+  int rc = sc_view(scw, &sc);
+  if (rc == STORMM_OK) {
+    rc = lm_view(lmw, &lm);
+  }
+  if (rc != STORMM_OK) {
+    return rc;
+  }
+  return line_advance(precision, ag, *psw, sc, lm, move_number, stream);
 }
 
 // The reference's host loop, src/MolecularMechanics/hpc_minimization.cu:262-405, one fused
 // evaluation per energy point.
-static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag, stormm_phasespace *ps,
-                             const stormm_minimize_controls *mc, stormm_scorecard *sc,
-                             stormm_line_min *lm, hipStream_t stream) {
+static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag,
+                             const stormm_ps_writer &w, const stormm_minimize_controls *mc,
+                             const ScView &sc, const LmView &lm, hipStream_t stream) {
   // The reference's per-cycle launch sequence (src/MolecularMechanics/hpc_minimization.cu:
   // 262-405).  The evaluations overwrite the force arrays and ScoreCard rows, which stands in
   // for the force and ScoreCard zeroing launches that precede them there.
-  stormm_mm_controls ctrl;
-  ctrl.mc = *mc;
-  ctrl.step = 0;
-  int rc = stormm_phasespace_prime_conjugate_gradient(ps, stream);
+  int step = 0;
+  int rc = stormm_ps_writer_prime_conjugate_gradient(&w, ag->plan.padded_atoms, stream);
   if (rc != STORMM_OK) {
     return rc;
   }
   const int prec = (precision == STORMM_PREC_DOUBLE) ? 0 : 1;
-  MMKit k = make_kit(ag, ps, sc);
+  MMKit k = make_kit(ag, w, &sc);
   k.store_overwrite = 1;
   const int terms = kTermNonbonded | kTermValence;
   const int cdcyc = mc->clash_damping_cycles;
@@ -931,8 +1385,9 @@ static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag, stor
     set_clash(k, precision, cd, cr);
     k.rst_step = minimizer_step(cyc, 0, cdcyc);
     HIP_TRY(launch_mm_eval(prec, k, terms, 1, 1, stream));
-    ctrl.step += 1;
-    if ((rc = stormm_launch_conjugate_gradient(precision, ag, ps, &ctrl, stream)) != STORMM_OK) {
+    step += 1;
+    if ((rc = conjugate_gradient(precision, ag, w, step, mc->steepest_descent_cycles, stream)) !=
+        STORMM_OK) {
       return rc;
     }
     for (int move = 0; move < 4; move++) {
@@ -940,8 +1395,7 @@ static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag, stor
         k.rst_step = minimizer_step(cyc, move, cdcyc);
         HIP_TRY(launch_mm_eval(prec, k, terms, 0, 1, stream));
       }
-      if ((rc = stormm_launch_line_advance(precision, ag, ps, sc, lm, move, stream)) !=
-          STORMM_OK) {
+      if ((rc = line_advance(precision, ag, w, sc, lm, move, stream)) != STORMM_OK) {
         return rc;
       }
     }
@@ -988,11 +1442,11 @@ static hipError_t capture_cycle(int prec, const MMKit &kf, const MMKit *ke, cons
   return err;
 }
 
-static int minimize_graph(int32_t precision, const stormm_synthesis *ag_c, stormm_phasespace *ps,
-                          const stormm_minimize_controls *mc, stormm_scorecard *sc,
-                          stormm_line_min *lm, hipStream_t stream) {
+static int minimize_graph(int32_t precision, const stormm_synthesis *ag_c,
+                          const stormm_ps_writer &w, const stormm_minimize_controls *mc,
+                          const ScView &sc, const LmView &lm, hipStream_t stream) {
   stormm_synthesis *ag = const_cast<stormm_synthesis *>(ag_c);
-  int rc = stormm_phasespace_prime_conjugate_gradient(ps, stream);
+  int rc = stormm_ps_writer_prime_conjugate_gradient(&w, ag->plan.padded_atoms, stream);
   if (rc != STORMM_OK) {
     return rc;
   }
@@ -1001,16 +1455,16 @@ static int minimize_graph(int32_t precision, const stormm_synthesis *ag_c, storm
   HIP_TRY(hipMemsetAsync(ag->stop_flags.ptr, 0, ns * sizeof(int), stream));
   HIP_TRY(hipMemsetAsync(ag->rms_force.ptr, 0, ns * sizeof(double), stream));
   const int prec = (precision == STORMM_PREC_DOUBLE) ? 0 : 1;
-  MMKit k = make_kit(ag, ps, sc);
+  MMKit k = make_kit(ag, w, &sc);
   k.store_overwrite = 1;
   set_clash(k, precision, 0.0, 0.0);
-  CgKit cg = make_cg_kit(ag, ps, precision);
+  CgKit cg = make_cg_kit(ag, w, precision);
   cg.steps = ag->cycles_run.as<int>();
   cg.done = ag->stop_flags.as<int>();
   cg.rms = ag->rms_force.as<double>();
   cg.conv_rms = mc->convergence_rms_force;
   cg.max_steps = mc->total_cycles;
-  LineKit lk = make_line_kit(ag, ps, sc, lm, precision);
+  LineKit lk = make_line_kit(ag, w, sc, lm, precision);
   lk.done = ag->stop_flags.as<int>();
   const int terms = kTermNonbonded | kTermValence;
   const int sd = mc->steepest_descent_cycles;
@@ -1076,6 +1530,66 @@ static int minimize_graph(int32_t precision, const stormm_synthesis *ag_c, storm
   return STORMM_OK;
 }
 
+static int minimize(int32_t precision, const stormm_synthesis *ag, const stormm_ps_writer &w,
+                    const stormm_minimize_controls *mincon, const ScView &sc, const LmView &lm,
+                    int32_t persistent, void *stream) {
+  int rc = check_writer(ag, &w, precision, &sc, true);
+  if (rc != STORMM_OK) {
+    return rc;
+  }
+  if (mincon == nullptr || sc.acc == nullptr || lm.nsys != ag->plan.nsys) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "minimization needs controls, a ScoreCard and a "
+                                             "matching LineMinimization record");
+  }
+  if ((rc = check_mincon(mincon)) != STORMM_OK) {
+    return rc;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (persistent == 0) {
+    return minimize_stepwise(precision, ag, w, mincon, sc, lm, st);
+  }
+  if (persistent == 2) {
+    return minimize_graph(precision, ag, w, mincon, sc, lm, st);
+  }
+  if (!ag->plan.rst.empty()) {
+    return fail(STORMM_ERR_STATE, "the persistent minimizer does not evaluate restraints; "
+                                  "minimize a restrained synthesis with the per-step (0) or "
+                                  "graph (2) mode");
+  }
+  if (w.gpos_bits > kMaxPlainGposBits || w.frc_bits > kMaxPlainFrcBits) {
+    return fail(STORMM_ERR_STATE, "the persistent minimizer takes at most 38 coordinate and 40 "
+                                  "force bits; use the per-step (0) or graph (2) mode");
+  }
+  MinKit k;
+  k.mm = make_kit(ag, w, &sc);
+  // The persistent kernel switches to its damped evaluations for cycles below clash_cycles and
+  // takes the schedule from its own cycle counter.
+  set_clash(k.mm, precision, 0.0, 0.0);
+  if (mincon->clash_damping_cycles > 0) {
+    k.mm.clash_variant = 1;
+    k.mm.clash_cycles = mincon->clash_damping_cycles;
+    k.mm.clash_r0 = mincon->clash_minimum_distance;
+    k.mm.clash_ratio = mincon->clash_vdw_ratio;
+  }
+  k.xprv = reinterpret_cast<long long *>(w.xprv);
+  k.yprv = reinterpret_cast<long long *>(w.yprv);
+  k.zprv = reinterpret_cast<long long *>(w.zprv);
+  k.xcg = reinterpret_cast<long long *>(w.x_cg_temp);
+  k.ycg = reinterpret_cast<long long *>(w.y_cg_temp);
+  k.zcg = reinterpret_cast<long long *>(w.z_cg_temp);
+  k.total_cycles = mincon->total_cycles;
+  k.sd_cycles = mincon->steepest_descent_cycles;
+  k.dx0 = mincon->initial_step;
+  k.conv_rms = mincon->convergence_rms_force;
+  k.l_move = lm.v[0];
+  k.cycles_run = const_cast<stormm_synthesis *>(ag)->cycles_run.as<int>();
+  k.rms_force = const_cast<stormm_synthesis *>(ag)->rms_force.as<double>();
+  HIP_TRY(launch_fill(sc.acc, nullptr, nullptr, static_cast<size_t>(sc.nsys) * kScoreStride,
+                      st));
+  HIP_TRY(launch_minimize_persistent(precision == STORMM_PREC_DOUBLE ? 0 : 1, k, st));
+  return STORMM_OK;
+}
+
 extern "C" int stormm_launch_minimization(int32_t precision, const stormm_synthesis *ag,
                                           stormm_phasespace *ps,
                                           const stormm_minimize_controls *mincon,
@@ -1089,49 +1603,58 @@ extern "C" int stormm_launch_minimization(int32_t precision, const stormm_synthe
     return fail(STORMM_ERR_INVALID_ARGUMENT, "minimization needs controls, a ScoreCard and a "
                                              "matching LineMinimization record");
   }
-  if ((rc = check_mincon(mincon)) != STORMM_OK) {
+  stormm_ps_writer w;
+  stormm_phasespace_writer(ps, &w);
+  return minimize(precision, ag, w, mincon, sc_view(sc), lm_view(lm), persistent, stream);
+}
+
+extern "C" int stormm_launch_minimization_kit(int32_t precision, const stormm_synthesis *ag,
+                                              const stormm_ps_writer *psw,
+                                              const stormm_minimize_controls *mincon,
+                                              const stormm_scorecard_writer *scw,
+                                              const stormm_linmin_writer *lmw,
+                                              int32_t persistent, void *stream) {
+  if (psw == nullptr || scw == nullptr || lmw == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null writer");
+  }
+  ScView sc;
+  LmView lm;
+  int rc = sc_view(scw, &sc);
+  if (rc == STORMM_OK) {
+    rc = lm_view(lmw, &lm);
+  }
+  if (rc != STORMM_OK) {
     return rc;
   }
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  if (persistent == 0) {
-    return minimize_stepwise(precision, ag, ps, mincon, sc, lm, st);
+  return minimize(precision, ag, *psw, mincon, sc, lm, persistent, stream);
+}
+
+extern "C" int stormm_check_ps_writer(const stormm_synthesis *ag, const stormm_ps_writer *psw) {
+  if (ag == nullptr || psw == nullptr || psw->atom_starts == nullptr ||
+      psw->atom_counts == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null synthesis, writer or layout arrays");
   }
-  if (persistent == 2) {
-    return minimize_graph(precision, ag, ps, mincon, sc, lm, st);
+  const SynthesisPlan &P = ag->plan;
+  if (psw->system_count != P.nsys) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "the writer holds " +
+                                                 std::to_string(psw->system_count) +
+                                                 " systems, the synthesis " +
+                                                 std::to_string(P.nsys));
   }
-  if (!ag->plan.rst.empty()) {
-    return fail(STORMM_ERR_STATE, "the persistent minimizer does not evaluate restraints; "
-                                  "minimize a restrained synthesis with the per-step (0) or "
-                                  "graph (2) mode");
+  std::vector<int> starts(P.nsys), counts(P.nsys);
+  HIP_TRY(hipMemcpy(starts.data(), psw->atom_starts, P.nsys * sizeof(int),
+                    hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(counts.data(), psw->atom_counts, P.nsys * sizeof(int),
+                    hipMemcpyDeviceToHost));
+  for (int s = 0; s < P.nsys; s++) {
+    if (starts[s] != P.atom_starts[s] || counts[s] != P.atom_counts[s]) {
+      return fail(STORMM_ERR_INVALID_ARGUMENT,
+                  "system " + std::to_string(s) + " lies at atoms " + std::to_string(starts[s]) +
+                      " + " + std::to_string(counts[s]) + " in the writer, at " +
+                      std::to_string(P.atom_starts[s]) + " + " +
+                      std::to_string(P.atom_counts[s]) + " in the synthesis");
+    }
   }
-  MinKit k;
-  k.mm = make_kit(ag, ps, sc);
-  // The persistent kernel switches to its damped evaluations for cycles below clash_cycles and
-  // takes the schedule from its own cycle counter.
-  set_clash(k.mm, precision, 0.0, 0.0);
-  if (mincon->clash_damping_cycles > 0) {
-    k.mm.clash_variant = 1;
-    k.mm.clash_cycles = mincon->clash_damping_cycles;
-    k.mm.clash_r0 = mincon->clash_minimum_distance;
-    k.mm.clash_ratio = mincon->clash_vdw_ratio;
-  }
-  k.xprv = ps->xprv.as<long long>();
-  k.yprv = ps->yprv.as<long long>();
-  k.zprv = ps->zprv.as<long long>();
-  k.xcg = ps->xcg.as<long long>();
-  k.ycg = ps->ycg.as<long long>();
-  k.zcg = ps->zcg.as<long long>();
-  k.total_cycles = mincon->total_cycles;
-  k.sd_cycles = mincon->steepest_descent_cycles;
-  k.dx0 = mincon->initial_step;
-  k.conv_rms = mincon->convergence_rms_force;
-  k.l_move = lm->vals.as<double>();
-  k.cycles_run = const_cast<stormm_synthesis *>(ag)->cycles_run.as<int>();
-  k.rms_force = const_cast<stormm_synthesis *>(ag)->rms_force.as<double>();
-  if ((rc = stormm_scorecard_initialize(sc, stream)) != STORMM_OK) {
-    return rc;
-  }
-  HIP_TRY(launch_minimize_persistent(precision == STORMM_PREC_DOUBLE ? 0 : 1, k, st));
   return STORMM_OK;
 }
 
@@ -1178,7 +1701,13 @@ extern "C" int stormm_time_mm_evaluation(int32_t precision, const stormm_synthes
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  const MMKit k = make_kit(ag, ps, sc);
+  stormm_ps_writer w;
+  stormm_phasespace_writer(ps, &w);
+  const ScView scv = sc_view(sc);
+  if ((rc = check_writer(ag, &w, precision, &scv, false)) != STORMM_OK) {
+    return rc;
+  }
+  const MMKit k = make_kit(ag, w, &scv);
   const int prec = (precision == STORMM_PREC_DOUBLE) ? 0 : 1;
   HIP_TRY(hipEventRecord(e0, st));
   for (int r = 0; r < repeats; r++) {

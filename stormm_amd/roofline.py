@@ -28,6 +28,7 @@ FLOP_ATOM = 40
 
 # MI355X datasheet peaks.
 PEAK_FP32_TFLOPS = 157.3       # dense FP32 (vector and matrix share this figure)
+PEAK_FP64_TFLOPS = 78.6        # dense FP64 vector
 PEAK_HBM_GBPS = 8000.0
 
 

@@ -295,6 +295,23 @@ class PhaseSpaceSynthesis:
         check(self._L.stormm_phasespace_download_fixed(self.handle, _ptr(c), _ptr(f)))
         return self._split(c), self._split(f)
 
+    def fixed_values(self):
+        """Coordinates and forces as exact Python integers (object arrays): value =
+        overflow word * 2^63 + word, the 95-bit form above 38 coordinate / 40 force bits."""
+        n = self.ag.total_atoms
+        c, co = np.zeros((n, 3), dtype=np.int64), np.zeros((n, 3), dtype=np.int32)
+        f, fo = np.zeros((n, 3), dtype=np.int64), np.zeros((n, 3), dtype=np.int32)
+        check(self._L.stormm_phasespace_download_fixed_split(self.handle, _ptr(c), _ptr(co),
+                                                             _ptr(f), _ptr(fo)))
+        big = lambda w, o: (o.astype(object) * (1 << 63)) + w.astype(object)
+        return self._split(big(c, co)), self._split(big(f, fo))
+
+    def writer(self) -> "_lib.PsWriter":
+        """The device arrays in the PsSynthesisWriter layout."""
+        w = _lib.PsWriter()
+        check(self._L.stormm_phasespace_writer(self.handle, ctypes.byref(w)))
+        return w
+
     def _range_atoms(self, lo: int, hi: int) -> int:
         counts = self.ag.atom_counts
         if not (0 <= lo < hi <= len(counts)):

@@ -24,6 +24,15 @@ INPUT_FILES = [
     ("test/Trajectory/trpcage.inpcrd", "trpcage.inpcrd"),
     ("test/Topology/ala_dipeptide.top", "ala_dipeptide.top"),
     ("test/Trajectory/ala_dipeptide.inpcrd", "ala_dipeptide.inpcrd"),
+    # Three of the four small molecules of the reference's GPU minimization test
+    # (test/MolecularMechanics/test_hpc_minimization.cu:1042-1051); the fourth, alanine dipeptide,
+    # carries CMAP terms, which this path does not evaluate.
+    ("test/Topology/bromobenzene_iso.top", "bromobenzene_iso.top"),
+    ("test/Trajectory/bromobenzene_iso.inpcrd", "bromobenzene_iso.inpcrd"),
+    ("test/Topology/stereo_L1.top", "stereo_L1.top"),
+    ("test/Trajectory/stereo_L1.inpcrd", "stereo_L1.inpcrd"),
+    ("test/Topology/symmetry_L1.top", "symmetry_L1.top"),
+    ("test/Trajectory/symmetry_L1.inpcrd", "symmetry_L1.inpcrd"),
 ]
 
 # (snapshot file, variable) pairs taken from the reference's test data.
@@ -63,6 +72,21 @@ KNOWN_ANSWERS = {
     "nonbonded": {"trpcage": [1458.0998129, 62.6481811, -1816.1478854, -119.3500215],
                   "ala": [46.8072425, 3.1589264, -78.8463310, -1.2452480],
                   "margin_trpcage": 2.0e-5, "margin_ala": 1.0e-5},
+    # Final energies of the reference's GPU minimizations, test/MolecularMechanics/min_energy.m
+    # (written by test_hpc_minimization.cu:840-846, snapshot margin 1e-5): "d" is FP64 with 40
+    # position / 40 force / 32 energy bits, 100 cycles of which 10 steepest descent; "f" is FP32
+    # with 28 / 24 / 32 bits, 500 cycles of which 50 steepest descent (:1024-1029, :400-404), no
+    # clash damping.  The ScoreCard holds the energy of the last energy-only evaluation (the
+    # third trial point of the last cycle).  small_mol_* lines 2-592; trp_cage_* 1200-1228;
+    # folded_pro_d 1184-1190 and vste_d 1334-1347 hold Trp-cage minimized in other batches.
+    "minimization": {
+        "bromobenzene_iso": {"d": 2.67090431, "f": 2.66019999},
+        "stereo_L1": {"d": 148.14839914, "f": 144.07275783},
+        "symmetry_L1": {"d": 1.13784720, "f": 0.35714461},
+        "trpcage": {"d": -534.84286089, "f": -713.76852424,
+                    "d_other_batches": [-534.83985375, -534.8423845]},
+        "margin": 1.0e-5,
+    },
 }
 
 

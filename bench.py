@@ -15,16 +15,18 @@ one thread per core of the job's CPU share.
 
 Two further workloads ride on the same line (skip them with --no-extra):
   "minimization": 4096 systems, N ~ U[16, 128], starts perturbed by N(0, 0.3 A), 500 cycles
-    (50 steepest descent, then conjugate gradient) in FP32: the persistent single-launch
-    minimizer, one persistent launch per size class, the reference's per-step launch sequence
-    and that sequence captured in a HIP graph and replayed per cycle, with the oracle's
-    minimizer on host threads as the CPU rate.
+    (50 steepest descent, then conjugate gradient, the first 25 clash-damped) in FP32: the fused
+    cycle kernels captured whole in one HIP graph (whole batch, and once per size class), the
+    reference's per-step launch sequence, and that sequence captured per cycle and replayed,
+    with the oracle's minimizer on host threads as the CPU rate.
   "large_batch": 16384 systems, N ~ U[32, 96]: FP32 and FP64 evaluation throughput and the
     errors of both modes against the FP64 oracle on a 64-system subsample.
   "serving": the served-minimization stream (Poisson arrivals at 200 requests/s over
-    --serve-seconds, 1-64 conformers of one molecule with N ~ logU[16, 512] per request, 300
-    cycles or RMS force <= 0.1): continuous batching and plain batching in real time (p50 / p99
-    latency), and the capacity of continuous serving against one batched launch per request.
+    --serve-seconds, default 60, 1-64 conformers of one molecule with N ~ logU[16, 512] per
+    request, 300 cycles or RMS force <= 0.1): continuous batching and static batching in real
+    time (p50 / p99 latency), and the capacity of continuous serving with every request waiting
+    against static serving, against the batch minimizer given all of the same conformers in one
+    launch, and against one batched launch per request.
 
   python bench.py [--gpus 1] [--steps 500] [--warmup 50] [--cpu-seconds 12] [--no-cpu]
                   [--no-extra]
@@ -127,52 +129,58 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu, shape=No
 
     # Warm-up on a few systems (kernel loading).
     wag, wps, wsc, wlm = setup(topos[:8], coords[:8])
-    for persistent in (1, 0, 2):
+    for mode in (1, 3, 4, 0, 2):
         S.launch_minimization("single", wag, wps, S.minimize_controls(3, 1, dx0, 0.0), wsc, wlm,
-                              persistent=persistent)
-    # The persistent kernel keeps each unit's block for the whole run, so it wants the units to
-    # fit the device in as few passes as possible: the default shape (512-atom units, two per
-    # CU), not the one tuned for single evaluations.
-    persistent_shape = {"wu_atom_capacity": 512, "target_wu_per_cu": 2, "block_threads": 512}
-    whole = [setup(topos, coords, persistent_shape)]
-    t_pers = run(whole, True)
-    e_pers = whole[0][2].potential_energies()
-    whole = [setup(topos, coords)]
-    t_step = run(whole, False)
-    e_step = whole[0][2].potential_energies()
-    whole = [setup(topos, coords)]
-    t_graph = run(whole, 2)
-    e_graph = whole[0][2].potential_energies()
-    # One persistent launch per size class (atom-count quartiles of the range).
+                              persistent=mode)
+    # Starting energies of every system (one energy evaluation at the perturbed coordinates).
+    ag0, ps0, sc0, _ = setup(topos, coords)
+    sc0.initialize()
+    S.launch_mm_evaluation("single", ag0, None, ps0, sc0, False, True)
+    e_start = sc0.potential_energies()
+    del ag0, ps0, sc0
+    timings, energies = {}, {}
+    for mode, name in ((1, "fused_graph"), (3, "fused_cycle_graph"), (4, "fused_direct"),
+                       (0, "per_step"), (2, "graph_replay")):
+        whole = [setup(topos, coords)]
+        timings[name] = run(whole, mode)
+        energies[name] = whole[0][2].potential_energies()
+        del whole
+    # The fused whole-run graph once per size class (atom-count quartiles of the range).
     edges = [16, 44, 72, 100, 129]
-    parts = []
-    for lo, hi in zip(edges[:-1], edges[1:]):
-        idx = [i for i, t in enumerate(topos) if lo <= t.natom < hi]
-        if idx:
-            sub_t, sub_c = [topos[i] for i in idx], [coords[i] for i in idx]
-            parts.append(setup(sub_t, sub_c, persistent_shape))
-    t_class = run(parts, True)
-    # The per-class launches give every system the same result as the whole-batch launch
-    # (results do not depend on the batch's composition).
+    classes = [[i for i, t in enumerate(topos) if lo <= t.natom < hi]
+               for lo, hi in zip(edges[:-1], edges[1:])]
+    classes = [idx for idx in classes if idx]
+    parts = [setup([topos[i] for i in idx], [coords[i] for i in idx]) for idx in classes]
+    t_class = run(parts, 1)
+    # Every system ends where it does in the whole-batch launch (results do not depend on the
+    # batch's composition).
     e_class = np.empty(nsys)
-    for (lo, hi), (ag_c, ps_c, sc_c, lm_c) in zip(
-            [(lo, hi) for lo, hi in zip(edges[:-1], edges[1:])
-             if any(lo <= t.natom < hi for t in topos)], parts):
-        idx = [i for i, t in enumerate(topos) if lo <= t.natom < hi]
+    for idx, (_, _, sc_c, _) in zip(classes, parts):
         e_class[idx] = sc_c.potential_energies()
-    class_equal = bool(np.array_equal(e_class, e_pers))
+    del parts
+    best = min(timings, key=timings.get)
     out = {"systems": nsys, "cycles": cycles, "sd_cycles": sd, "precision": "fp32",
-           "autotuned_shape": shape, "persistent_shape": persistent_shape,
-           "persistent_s": t_pers, "per_size_class_s": t_class, "per_step_launches_s": t_step,
-           "graph_replay_s": t_graph,
-           "minimizations_per_s": nsys / min(t_pers, t_graph),
-           "best_mode": "persistent" if t_pers <= t_graph else "graph_replay",
-           "persistent_vs_per_step": t_step / t_pers,
-           "graph_vs_per_step": t_step / t_graph,
-           "persistent_equals_per_step": bool(np.array_equal(e_pers, e_step)),
-           "graph_equals_per_step": bool(np.array_equal(e_graph, e_step)),
-           "per_size_class_equals_whole_batch": class_equal,
-           "size_classes": len(parts)}
+           "clash_damping_cycles": 25, "autotuned_shape": shape,
+           "fused_graph_s": timings["fused_graph"],
+           "fused_cycle_graph_s": timings["fused_cycle_graph"],
+           "fused_direct_s": timings["fused_direct"],
+           "per_step_launches_s": timings["per_step"],
+           "graph_replay_s": timings["graph_replay"],
+           "per_size_class_s": t_class, "size_classes": len(classes),
+           "minimizations_per_s": nsys / timings[best],
+           "best_mode": best,
+           "fused_vs_per_step": timings["per_step"] / timings["fused_graph"],
+           "graph_vs_per_step": timings["per_step"] / timings["graph_replay"],
+           "fused_equals_per_step": bool(all(np.array_equal(energies[m], energies["per_step"])
+                                             for m in ("fused_graph", "fused_cycle_graph",
+                                                       "fused_direct"))),
+           "graph_equals_per_step": bool(np.array_equal(energies["graph_replay"],
+                                                        energies["per_step"])),
+           "per_size_class_equals_whole_batch": bool(np.array_equal(e_class,
+                                                                    energies["fused_graph"])),
+           "every_energy_fell": bool(np.all(energies["fused_graph"] < e_start)),
+           "mean_energy_start": float(np.mean(e_start)),
+           "mean_energy_final": float(np.mean(energies["fused_graph"]))}
     if with_cpu:
         from concurrent.futures import ThreadPoolExecutor
         nsample = 128
@@ -252,7 +260,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the minimization and large-batch workloads")
     ap.add_argument("--min-cycles", type=int, default=500)
-    ap.add_argument("--serve-seconds", type=float, default=5.0,
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="arrival window of the served-minimization stream")
     ap.add_argument("--minimize-only", metavar="CAPACITY,PER_CU,THREADS", default=None,
                     help="run only the minimization workload on the given work-unit shape (no "
@@ -386,8 +394,10 @@ def main():
         from stormm_amd import serving
         extra["serving"] = serving.serving_benchmark(seconds=args.serve_seconds, log=log)
         log(f"serving: { {k: v for k, v in extra['serving'].items() if k != 'capacity'} }")
-        log(f"serving capacity: {extra['serving']['capacity']['speedup_vs_plain_batched']:.2f}x "
-            "the plain batched launches")
+        cap = extra['serving']['capacity']
+        log(f"serving capacity: {cap['speedup_vs_batched']:.2f}x the batch minimizer on the same "
+            f"conformers, {cap['speedup_vs_static']:.2f}x static serving, "
+            f"{cap['speedup_vs_per_request']:.2f}x one batched launch per request")
 
     out = {
         "metric": METRIC,

@@ -358,17 +358,24 @@ int stormm_launch_conjugate_gradient(int32_t precision, const stormm_synthesis *
 int stormm_launch_line_advance(int32_t precision, const stormm_synthesis *ag,
                                stormm_phasespace *ps, stormm_scorecard *sc, stormm_line_min *lm,
                                int32_t move_number, void *stream);
-/* Whole minimization.  persistent == 1 runs every cycle of every system inside one launch with
- * the systems' state held in LDS; persistent == 0 issues the reference's per-step launch
- * sequence; persistent == 2 captures that sequence for one cycle in a HIP graph and replays it
- * for every cycle (returns when done).  Modes 1 and 2 stop a system once its RMS force reaches
- * convergence_rms_force (if > 0).  Final energies land in the ScoreCard.  A synthesis with
- * restraints is minimized in modes 0 and 2; mode 1 returns STORMM_ERR_STATE for it. */
+/* Whole minimization.  persistent == 0 issues the reference's per-step launch sequence (per
+ * cycle: force evaluation, conjugate gradient, four line moves with three energy evaluations);
+ * persistent == 2 captures that sequence for one cycle in a HIP graph and replays it for every
+ * cycle.  persistent == 1, 3 and 4 run fused cycle kernels: each evaluation block also updates
+ * the conjugate-gradient direction and makes the line move of the whole systems it holds, so a
+ * cycle is four launches; mode 1 captures the whole run in one HIP graph and launches it once,
+ * mode 3 captures one cycle and replays it, mode 4 issues the kernels one by one.  All modes
+ * give bit-identical results.  Modes 1-4 stop a system once its RMS force reaches
+ * convergence_rms_force (if > 0) and record per-system cycle counts and RMS forces
+ * (stormm_minimization_status).  Final energies land in the ScoreCard.  Modes 1, 3 and 4 take
+ * no restraints and at most 38 coordinate and 40 force bits (STORMM_ERR_STATE otherwise); a
+ * restrained synthesis is minimized in modes 0 and 2.  Mode 0 returns once its launches are
+ * queued on the stream; the other modes return when the minimization is done. */
 int stormm_launch_minimization(int32_t precision, const stormm_synthesis *ag,
                                stormm_phasespace *ps, const stormm_minimize_controls *mincon,
                                stormm_scorecard *sc, stormm_line_min *lm, int32_t persistent,
                                void *stream);
-/* Per-system status after a persistent minimization: cycles run and final RMS force. */
+/* Per-system status after a minimization in modes 1-4: cycles run and final RMS force. */
 int stormm_minimization_status(const stormm_synthesis *ag, int32_t *cycles_run,
                                double *rms_force);
 

@@ -41,11 +41,12 @@ constexpr int kMaxPlainFrcBits = 40;
 constexpr int kEvalBlock = 512;
 constexpr int kMaxEvalBlock = 1024;
 constexpr int kMaxEvalBlockDouble = 512;
-// The persistent minimizer (evaluation plus line search in one kernel) runs at most 512 threads.
-constexpr int kMaxMinimizeBlock = 512;
 // Entries of each wavefront's LDS queue of near-field Born-integral pairs.  A wave drains 64
-// entries whenever the queue holds at least 64, and one rotation step adds at most 64.
-constexpr int kNearQueue = 128;
+// entries at a time whenever the queue holds at least 64 after a group of four rotation steps,
+// each of which adds at most 64.
+constexpr int kNearQueue = 320;
+// Each wavefront's queue is followed by 64 slots that take the stores of lanes without an entry.
+constexpr int kNearQueueStride = kNearQueue + 64;
 constexpr int kStageCounters = 3;  // work-item counters of the psi, derivative and valence stages
 
 // StateVariable slots, src/Potential/energy_enumerators.h:36-75.

@@ -53,19 +53,17 @@ hipError_t launch_conjugate_gradient(const CgKit &k, int step, int sd_cycles,
                                      hipStream_t stream);
 hipError_t launch_line_advance(const LineKit &k, int move_number, hipStream_t stream);
 
-// Persistent minimization: every cycle for every system of a work unit inside one block.
-struct MinKit {
-  MMKit mm;
-  long long *xprv, *yprv, *zprv;
-  long long *xcg, *ycg, *zcg;
-  int total_cycles, sd_cycles;
-  double dx0;
-  double conv_rms;        // stop a system when its RMS force falls to this (0 = never)
-  double *l_move;         // per-system line-minimization state (optional, may be null)
-  int *cycles_run;        // per-system cycles executed
-  double *rms_force;      // per-system final RMS force
+// Fused minimization cycle (mm_minimize.hip): evaluations that make the pending line move of
+// the systems they hold while loading the coordinates.  cg.done is required; line holds the
+// line-search state and scales.
+struct CycleKit {
+  CgKit cg;
+  LineKit line;
 };
-hipError_t launch_minimize_persistent(int precision, const MinKit &k, hipStream_t stream);
+// Force + energy (forces) or energy-only evaluation, making line move pending_move (0..3) first
+// (-1: none).
+hipError_t launch_min_cycle(int precision, const MMKit &k, const CycleKit &ck, bool forces,
+                            int pending_move, hipStream_t stream);
 
 // Fresh per-system state of newly admitted pool slots: layout, zero cycle count and stop flag,
 // line-search state primed with dx0, zeroed ScoreCard row.

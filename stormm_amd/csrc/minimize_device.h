@@ -14,10 +14,13 @@ struct LineState {
 // Cubic through (0, E0), (m1, E1), (m2, E2), (m3, E3) by Gauss-Jordan elimination on the
 // augmented [A | I] matrix, then the choice of the final move,
 // src/MolecularMechanics/line_movement.cui:129-338.  Returns the move factor and writes the
-// multiplier for the baseline move length.
+// multiplier for the baseline move length.  Every loop is unrolled and the pivot row swap is a
+// select on each candidate row, so the matrix is never indexed by a run-time value and stays in
+// registers (a run-time index would put it in private memory).
 __host__ __device__ inline double cubic_line_move(const double evec[4], const double mvec[4],
                                                   double *new_baseline) {
   double col[8][4];
+#pragma unroll
   for (int r = 0; r < 4; r++) {
     const double xv = mvec[r];
     const double x2 = xv * xv;
@@ -25,13 +28,16 @@ __host__ __device__ inline double cubic_line_move(const double evec[4], const do
     col[1][r] = x2;
     col[2][r] = xv;
     col[3][r] = 1.0;
+#pragma unroll
     for (int c = 4; c < 8; c++) {
       col[c][r] = (c - 4 == r) ? 1.0 : 0.0;
     }
   }
+#pragma unroll
   for (int i = 0; i < 4; i++) {
     double maxval = fabs(col[i][i]);
     int maxpos = i;
+#pragma unroll
     for (int j = i + 1; j < 4; j++) {
       const double a = fabs(col[i][j]);
       if (a > maxval) {
@@ -39,31 +45,44 @@ __host__ __device__ inline double cubic_line_move(const double evec[4], const do
         maxpos = j;
       }
     }
-    for (int c = 0; c < 8; c++) {
-      const double tmp = col[c][maxpos];
-      col[c][maxpos] = col[c][i];
-      col[c][i] = tmp;
+#pragma unroll
+    for (int j = i + 1; j < 4; j++) {
+      const bool swap = (maxpos == j);
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const double ri = col[c][i];
+        const double rj = col[c][j];
+        col[c][i] = swap ? rj : ri;
+        col[c][j] = swap ? ri : rj;
+      }
     }
+#pragma unroll
     for (int j = i + 1; j < 4; j++) {
       const double mult = col[i][j] / col[i][i];
+#pragma unroll
       for (int c = i; c < 8; c++) {
         col[c][j] -= mult * col[c][i];
       }
     }
   }
+#pragma unroll
   for (int i = 3; i >= 0; i--) {
     const double div = col[i][i];
+#pragma unroll
     for (int c = 0; c < 8; c++) {
       col[c][i] /= div;
     }
+#pragma unroll
     for (int j = i - 1; j >= 0; j--) {
       const double mult = col[i][j];
+#pragma unroll
       for (int c = i; c < 8; c++) {
         col[c][j] -= mult * col[c][i];
       }
     }
   }
   double abcd[4];
+#pragma unroll
   for (int r = 0; r < 4; r++) {
     abcd[r] = (col[4][r] * evec[0]) + (col[5][r] * evec[1]) + (col[6][r] * evec[2]) +
               (col[7][r] * evec[3]);

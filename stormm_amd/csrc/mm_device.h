@@ -98,6 +98,38 @@ __device__ __forceinline__ double row_rotate(double v) {
                               (static_cast<long long>(lo) & 0xffffffffLL));
 }
 
+// The value that lane (L - K) mod 16 of the row holds, K = 0..15 (row_ror:K): the j-atom data a
+// lane meets at rotation step K while the data stay where they were loaded.  With one use per
+// rotated value the compiler folds the move into the arithmetic instruction as its DPP source
+// operand, so reading rotated data costs no instruction of its own.
+template <int K> __device__ __forceinline__ int row_ror(int v) {
+  if constexpr (K == 0) {
+    return v;
+  }
+  else {
+    return __builtin_amdgcn_mov_dpp(v, 0x120 + K, 0xF, 0xF, true);
+  }
+}
+template <int K> __device__ __forceinline__ float row_ror(float v) {
+  return __int_as_float(row_ror<K>(__float_as_int(v)));
+}
+template <int K> __device__ __forceinline__ double row_ror(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = row_ror<K>(static_cast<int>(bits & 0xffffffffLL));
+  const int hi = row_ror<K>(static_cast<int>(bits >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              (static_cast<long long>(lo) & 0xffffffffLL));
+}
+
+// Calls f(std::integral_constant<int, K>) for each K of the sequence, in order (rotation steps
+// unrolled with compile-time rotation counts).
+// A scheduling barrier after each step keeps the steps' live values from overlapping (the
+// unrolled steps would otherwise be interleaved past the register budget).
+template <typename F, int... K>
+__device__ __forceinline__ void unrolled_steps(F &&f, std::integer_sequence<int, K...>) {
+  ((f(std::integral_constant<int, K>{}), __builtin_amdgcn_sched_barrier(0)), ...);
+}
+
 // Sum of a 64-bit integer over the 16 lanes of a row.
 __device__ __forceinline__ long long row_sum(long long v) {
   v += __shfl_xor(v, 8, 16);
@@ -122,24 +154,33 @@ __device__ __forceinline__ void lds_add(long long *addr, long long v) {
 //   b4  r <= |ri - sj|, ri<sj : -sj / (2 (r^2 - sj^2)) - 1/ri       - ln(|r - sj| / (r + sj)) / (4 r)
 // (zero when none applies).  The far and near forms are separate functions so that the tile
 // loops can evaluate the cheap series for every lane and hand the rarer near pairs to a
-// compacted pass.  The near branches share one reciprocal and one logarithm: b3 needs
-// 1/(r + sj), b2 and b4 need 1/(r + sj) and 1/(r^2 - sj^2), both taken from 1/((r + sj)(r - sj))
-// since |r - sj| >= ri keeps that product away from zero in those branches.
+// compacted pass.  The far-field series, -s^3 / r^4 (a + b x + c x^2 + d x^3 + dd x^4) with
+// x = (s / r)^2, and its derivative follow :595-598 and :1121-1124.  The near branches share
+// one reciprocal and one logarithm: b3 needs 1/(r + sj), b2 and b4 need 1/(r + sj) and
+// 1/(r^2 - sj^2), both taken from 1/((r + sj)(r - sj)) since |r - sj| >= ri keeps that product
+// away from zero in those branches.
 // ------------------------------------------------------------------------------------------
+// Squared distance given to an absent pair in the Born-integral and derivative tile loops: far
+// beyond 4 screened radii of any atom, with 1 / r^4 below the smallest denormal, so both far-field
+// terms round to exactly zero and the pair never enters the near-field queue.
+template <typename T> struct FarAway;
+template <> struct FarAway<float> {
+  static constexpr float kR2 = 1.0e30f;
+};
+template <> struct FarAway<double> {
+  static constexpr double kR2 = 1.0e200;
+};
+
+// Far-field series in x = (s / r)^2: the Born-integral term is -s^3 / r^4 times psi_series(x),
+// the derivative term (divided by r) s^3 / r^6 times dpsi_series(x).
 template <typename T>
-__device__ __forceinline__ T psi_far(T sj, T sj2, T invr2) {
-  const T tmpsd = sj2 * invr2;
-  const T dumbo = T(kGTA) + tmpsd * (T(kGTB) + tmpsd * (T(kGTC) + tmpsd * (T(kGTD) +
-                                                                          tmpsd * T(kGTDD))));
-  return -(sj * tmpsd * invr2 * dumbo);
+__device__ __forceinline__ T psi_series(T x) {
+  return T(kGTA) + x * (T(kGTB) + x * (T(kGTC) + x * (T(kGTD) + x * T(kGTDD))));
 }
 
 template <typename T>
-__device__ __forceinline__ T dpsi_far(T sj, T sj2, T invr2) {
-  const T tmpsd = sj2 * invr2;
-  const T dumbo = T(kGTE) + tmpsd * (T(kGTF) + tmpsd * (T(kGTG) + tmpsd * (T(kGTH) +
-                                                                          tmpsd * T(kGTHH))));
-  return tmpsd * sj * invr2 * invr2 * dumbo;
+__device__ __forceinline__ T dpsi_series(T x) {
+  return T(kGTE) + x * (T(kGTF) + x * (T(kGTG) + x * (T(kGTH) + x * T(kGTHH))));
 }
 
 // Shared set-up of the near branches: branch flags, 1/(r + sj), 1/(r^2 - sj^2) (b2, b4) and
@@ -213,7 +254,8 @@ template <> struct EvalBlockLimit<double> {
 template <typename T> struct WuLds {
   using crd_t = typename Prec<T>::crd_t;
   long long *psi, *dedr, *fx, *fy, *fz, *nrg, *org;
-  T *q, *p1, *p2, *rho, *irho, *scr, *rad, *sd;
+  double *move;             // per system: pending line move (coordinate units per direction unit)
+  T *q, *p1, *p2, *rho, *irho, *scr, *rad, *irad, *sd;
   crd_t *cx, *cy, *cz;
   int *gidx, *slot;
   WuSys *sys;
@@ -221,16 +263,17 @@ template <typename T> struct WuLds {
   int *work;                // per-stage work-item counters (kStageCounters)
   T *clash_d, *clash_r;     // per-system clash distance and sigma ratio of this evaluation
   int *clash_on;            // per-system flag: damped potentials apply
-  unsigned *nearq;          // kNearQueue packed pair entries per wavefront
+  unsigned *nearq;          // kNearQueueStride entries per wavefront: the queue, then 64 spare
 
   static __host__ __device__ size_t bytes(int maxa, int waves) {
     const size_t m = static_cast<size_t>(maxa);
     return (5 * m * sizeof(long long)) + (kMaxWuSystems * 16 * sizeof(long long)) +
-           (3 * kMaxWuSystems * sizeof(long long)) + (8 * m * sizeof(T)) +
+           (3 * kMaxWuSystems * sizeof(long long)) + (kMaxWuSystems * sizeof(double)) +
+           (9 * m * sizeof(T)) +
            (3 * m * sizeof(crd_t)) + (2 * m * sizeof(int)) + (kMaxWuSystems * sizeof(WuSys)) +
            (3 * (kMaxWuSystems + 1) * sizeof(int)) + (kStageCounters * sizeof(int)) +
            (2 * kMaxWuSystems * sizeof(T)) + (kMaxWuSystems * sizeof(int)) +
-           (static_cast<size_t>(waves) * kNearQueue * sizeof(unsigned));
+           (static_cast<size_t>(waves) * kNearQueueStride * sizeof(unsigned));
   }
 
   __device__ WuLds(unsigned char *base, int maxa, int waves) {
@@ -248,6 +291,7 @@ template <typename T> struct WuLds {
     fz = reinterpret_cast<long long *>(take(m * sizeof(long long)));
     nrg = reinterpret_cast<long long *>(take(kMaxWuSystems * 16 * sizeof(long long)));
     org = reinterpret_cast<long long *>(take(3 * kMaxWuSystems * sizeof(long long)));
+    move = reinterpret_cast<double *>(take(kMaxWuSystems * sizeof(double)));
     q = reinterpret_cast<T *>(take(m * sizeof(T)));
     p1 = reinterpret_cast<T *>(take(m * sizeof(T)));
     p2 = reinterpret_cast<T *>(take(m * sizeof(T)));
@@ -255,6 +299,7 @@ template <typename T> struct WuLds {
     irho = reinterpret_cast<T *>(take(m * sizeof(T)));
     scr = reinterpret_cast<T *>(take(m * sizeof(T)));
     rad = reinterpret_cast<T *>(take(m * sizeof(T)));
+    irad = reinterpret_cast<T *>(take(m * sizeof(T)));
     sd = reinterpret_cast<T *>(take(m * sizeof(T)));
     clash_d = reinterpret_cast<T *>(take(kMaxWuSystems * sizeof(T)));
     clash_r = reinterpret_cast<T *>(take(kMaxWuSystems * sizeof(T)));
@@ -269,7 +314,7 @@ template <typename T> struct WuLds {
     vpre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
     work = reinterpret_cast<int *>(take(kStageCounters * sizeof(int)));
     clash_on = reinterpret_cast<int *>(take(kMaxWuSystems * sizeof(int)));
-    nearq = reinterpret_cast<unsigned *>(take(static_cast<size_t>(waves) * kNearQueue *
+    nearq = reinterpret_cast<unsigned *>(take(static_cast<size_t>(waves) * kNearQueueStride *
                                               sizeof(unsigned)));
   }
 };
@@ -280,14 +325,18 @@ __device__ __forceinline__ float crd_delta(int a, int b, float scale) {
 }
 __device__ __forceinline__ double crd_delta(double a, double b, double) { return a - b; }
 
-// exp(-r^2 / (4 Ri Rj)) of the GB pair function, given r^2 and Ri Rj.
-__device__ __forceinline__ float gb_efac(float r2, float rr) {
-  return __builtin_amdgcn_exp2f(r2 * __builtin_amdgcn_rcpf(rr) *
-                                (-0.25f * 1.44269504088896340736f));
-}
-__device__ __forceinline__ double gb_efac(double r2, double rr) {
-  return ::exp(-0.25 * r2 * Prec<double>::rcp(rr));
-}
+// exp(-r^2 / (4 Ri Rj)) of the GB pair function as eval(r^2 * (1 / Rj) * (kScale / Ri)): the
+// hardware base-2 exponential in single precision (kScale carries log2 e), the library
+// exponential in double precision.
+template <typename T> struct GbExp;
+template <> struct GbExp<float> {
+  static constexpr float kScale = -0.25f * 1.44269504088896340736f;
+  static __device__ __forceinline__ float eval(float x) { return __builtin_amdgcn_exp2f(x); }
+};
+template <> struct GbExp<double> {
+  static constexpr double kScale = -0.25;
+  static __device__ __forceinline__ double eval(double x) { return ::exp(x); }
+};
 
 // Coordinates of a pair tile's atoms in Angstroms in the working precision.  Single precision
 // converts the LDS fixed-precision coordinates relative to a point between the tile's two atom

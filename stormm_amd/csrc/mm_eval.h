@@ -27,6 +27,7 @@ template <typename T> struct StageCtx {
   int cycle;         // minimization cycle of this evaluation (clash-damping schedule)
   int rst_step;      // restraint step number when the kit gives no per-system cycle counts
   bool gb;
+  bool move_pending = false;  // the coordinate stage adds L.move[slot] x direction first
   __device__ StageCtx(unsigned char *smem, int maxa, int waves) : L(smem, maxa, waves) {}
 };
 
@@ -122,9 +123,20 @@ __device__ void stage_load_coords(StageCtx<T> &c, const MMKit &k) {
     }
     else {
       const long long mask = ~((1LL << c.crd_shift) - 1LL);
-      L.org[3 * c.tid + 0] = k.xcrd[gs] & mask;
-      L.org[3 * c.tid + 1] = k.ycrd[gs] & mask;
-      L.org[3 * c.tid + 2] = k.zcrd[gs] & mask;
+      long long x0 = k.xcrd[gs], y0 = k.ycrd[gs], z0 = k.zcrd[gs];
+      if (c.move_pending) {
+        // The origin comes from the moved first atom, as it would after a separate move.  This
+        // thread wrote the system's move factor itself before the stage began.
+        const double mf = L.move[c.tid];
+        if (mf != 0.0) {
+          x0 += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.xfrc[gs])));
+          y0 += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.yfrc[gs])));
+          z0 += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.zfrc[gs])));
+        }
+      }
+      L.org[3 * c.tid + 0] = x0 & mask;
+      L.org[3 * c.tid + 1] = y0 & mask;
+      L.org[3 * c.tid + 2] = z0 & mask;
     }
   }
   for (int e = c.tid; e < wu.sys_count * 16; e += c.nthr) {
@@ -143,6 +155,7 @@ __device__ void stage_load_coords(StageCtx<T> &c, const MMKit &k) {
     L.fy[a] = 0;
     L.fz[a] = 0;
     L.rad[a] = T(1);
+    L.irad[a] = T(1);
     L.sd[a] = T(0);
     if (g >= 0 && k.xcrd_ovrf != nullptr) {
       const __int128 dx = i95_value(k.xcrd[g], k.xcrd_ovrf[g]) -
@@ -161,7 +174,20 @@ __device__ void stage_load_coords(StageCtx<T> &c, const MMKit &k) {
       L.cz[a] = CrdConv<T>::to_lds_wide(dz, c.crd_shift);
     }
     else if (g >= 0) {
-      const long long gx = k.xcrd[g], gy = k.ycrd[g], gz = k.zcrd[g];
+      long long gx = k.xcrd[g], gy = k.ycrd[g], gz = k.zcrd[g];
+      if (c.move_pending) {
+        // The pending line move, x += rint(mf * direction) with the normalized direction in the
+        // force arrays (line_movement.cui:343-360), written back as the new coordinates.
+        const double mf = L.move[s];
+        if (mf != 0.0) {
+          gx += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.xfrc[g])));
+          gy += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.yfrc[g])));
+          gz += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.zfrc[g])));
+          k.xcrd[g] = gx;
+          k.ycrd[g] = gy;
+          k.zcrd[g] = gz;
+        }
+      }
       const long long ox = L.org[3 * s], oy = L.org[3 * s + 1], oz = L.org[3 * s + 2];
       if (!(CrdConv<T>::in_range(gx, ox, c.crd_shift) &&
             CrdConv<T>::in_range(gy, oy, c.crd_shift) &&
@@ -382,30 +408,27 @@ __device__ __forceinline__ unsigned near_entry(int i, int j, bool near_i, bool n
          (near_j ? kNearJ : 0u);
 }
 
-// Appends the entries of the lanes with `need` set, in lane order; n is wave-uniform.
-__device__ __forceinline__ void nearq_push(unsigned *q, int &n, bool need, unsigned entry) {
+// Appends the entries of the lanes with `need` set, in lane order; n is wave-uniform.  Every
+// lane stores, the lanes without an entry into a slot of their own past the queue's kNearQueue
+// entries (lane - below counts them), so the rotation step has no branch.
+__device__ __forceinline__ void nearq_push(unsigned *q, int &n, bool need, unsigned entry,
+                                           int lane) {
   const unsigned long long m = __ballot(need);
   const int below = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(m >> 32),
                                               __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(m),
                                                                         0u));
-  if (need) {
-    q[n + below] = entry;
-  }
+  q[(need ? n : (kNearQueue + lane - (2 * below))) + below] = entry;
   n += __popcll(m);
   __builtin_amdgcn_wave_barrier();
 }
 
-// Moves the entries past the first 64 (already drained) to the front of the queue.
-__device__ __forceinline__ void nearq_shift(unsigned *q, int &n, int lane) {
-  const int rem = n - 64;
-  __builtin_amdgcn_wave_barrier();
-  if (lane < rem) {
-    const unsigned e = q[64 + lane];
-    q[lane] = e;
-  }
-  n = rem;
-  __builtin_amdgcn_wave_barrier();
-}
+// The Born-integral and derivative stages run their rotation steps in groups of kStepGroup with
+// compile-time rotation counts inside a group; the j-side data move kStepGroup lanes between
+// groups, and the near-pair queue is drained after each group, taking 64 entries at a time from
+// its end.  Before a group the queue holds at most 63 entries, so kNearQueue = 63 + 4 * 64 + 1
+// suffices.
+constexpr int kStepGroup = 4;
+static_assert(kNearQueue >= 64 + (kStepGroup * 64), "near queue too short for a step group");
 
 // Geometry of a queued pair, recomputed from the LDS coordinates.
 template <typename T> struct NearPair {
@@ -447,9 +470,11 @@ __device__ __forceinline__ void psi_drain(StageCtx<T> &c, const unsigned *q, int
 
 // Born-radius integrals (src/Potential/nonbonded_potential.tpp:560-650).  Each lane evaluates
 // the far-field series of its pair at every rotation step and queues the pair when either
-// direction is within 4 screened radii; a full wavefront of queued pairs is evaluated at once.
+// direction is within 4 screened radii; full wavefronts of queued pairs are evaluated together.
 // The series terms of one task are summed in the working precision (at most 16 small terms per
-// lane) and added to the fixed-precision accumulators once per task.
+// lane) and added to the fixed-precision accumulators once per task.  With x = (s / r)^2 for the
+// sphere of screened radius s on the other atom, the far-field term is -s^3 / r^4 times the
+// series in x, and r > 4 s is x < 1/16.
 // Chunks of valence terms (`val_chunks`, possibly 0) are taken as work items alongside the tiles.
 template <typename T, bool FORCES, bool CLASH>
 __device__ void stage_psi(StageCtx<T> &c, const MMKit &k, int val_chunks) {
@@ -458,7 +483,7 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k, int val_chunks) {
   const int il = c.tid & 15;
   const int lane = c.tid & 63;
   const int wave = c.tid >> 6;
-  unsigned *q = L.nearq + (wave * kNearQueue);
+  unsigned *q = L.nearq + (wave * kNearQueueStride);
   int qn = 0;
   const double psi_scale = ldexp(1.0, kPsiBits);
   const int quads = (c.wu.task_count + 3) >> 2;
@@ -474,46 +499,62 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k, int val_chunks) {
     tile_coords(c, tt, xi, yi, zi, xj, yj, zj);
     const T si = L.scr[tt.i];
     const T si2 = si * si;
-    T sj = L.scr[tt.j0];
-    // Queue entry bits of the pair: i in bits 0..8, the rotating j atom in bits 9..17.
+    const T si3 = si2 * si;
+    const T sj = L.scr[tt.j0];
+    T sj2 = sj * sj;
+    T sj3 = sj2 * sj;
+    // Queue entry bits of the pair: i in bits 0..8, the j atom in bits 9..17.
     const unsigned ei = static_cast<unsigned>(tt.i);
-    unsigned ej = static_cast<unsigned>(tt.j0) << 9;
+    int ej = tt.j0 << 9;
     T acci = T(0), accj = T(0);
-    for (int ks = 0; ks < tt.nsteps; ks++) {
-      const bool valid = step_bit(tt.valid, ks);
-      const T dx = xj - xi;
-      const T dy = yj - yi;
-      const T dz = zj - zi;
-      const T r2 = valid ? ((dx * dx) + (dy * dy) + (dz * dz)) : T(1);
-      const T invr2 = P::rcp(r2);
-      // r > 4 s  <=>  r^2 / 16 > s^2
-      const T r2q = T(0.0625) * r2;
-      const T sj2 = sj * sj;
-      const bool far_i = r2q > sj2;
-      const bool far_j = r2q > si2;
-      const T ti = psi_far(sj, sj2, invr2);
-      const T tj = psi_far(si, si2, invr2);
-      acci += (valid && far_i) ? ti : T(0);
-      accj += (valid && far_j) ? tj : T(0);
-      const bool near_i = valid && !far_i;
-      const bool near_j = valid && !far_j;
-      nearq_push(q, qn, near_i || near_j,
-                 ei | ej | (near_i ? kNearI : 0u) | (near_j ? kNearJ : 0u));
-      if (qn >= 64) {
-        psi_drain(c, q, 64, lane);
-        nearq_shift(q, qn, lane);
+    unsigned vbits = tt.valid;
+    for (int g = 0; g < tt.nsteps; g += kStepGroup) {
+      auto step = [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        if (g + K >= tt.nsteps) {
+          return;
+        }
+        // An absent pair is put far away: both directions are then far-field with terms
+        // that round to exactly zero, so no validity test follows.
+        const bool valid = ((vbits >> K) & 1u) != 0u;
+        const T dx = row_ror<K>(xj) - xi;
+        const T dy = row_ror<K>(yj) - yi;
+        const T dz = row_ror<K>(zj) - zi;
+        const T r2c = (dx * dx) + (dy * dy) + (dz * dz);
+        const T r2 = valid ? r2c : FarAway<T>::kR2;
+        const T invr2 = P::rcp(r2);
+        const T invr4 = invr2 * invr2;
+        const T x_i = row_ror<K>(sj2) * invr2;
+        const T x_j = si2 * invr2;
+        const bool far_i = x_i < T(0.0625);
+        const bool far_j = x_j < T(0.0625);
+        const T ti = psi_series(x_i) * (row_ror<K>(sj3) * invr4);
+        const T tj = psi_series(x_j) * (si3 * invr4);
+        acci -= far_i ? ti : T(0);
+        accj = row_rotate(accj) - (far_j ? tj : T(0));
+        nearq_push(q, qn, !(far_i && far_j),
+                   ei | static_cast<unsigned>(row_ror<K>(ej)) | (far_i ? 0u : kNearI) |
+                       (far_j ? 0u : kNearJ),
+                   lane);
+      };
+      unrolled_steps(step, std::integer_sequence<int, 0, 1, 2, 3>{});
+      vbits >>= kStepGroup;
+      xj = row_ror<kStepGroup>(xj);
+      yj = row_ror<kStepGroup>(yj);
+      zj = row_ror<kStepGroup>(zj);
+      sj2 = row_ror<kStepGroup>(sj2);
+      sj3 = row_ror<kStepGroup>(sj3);
+      ej = row_ror<kStepGroup>(ej);
+      while (qn >= 64) {
+        psi_drain(c, q + (qn - 64), 64, lane);
+        qn -= 64;
       }
-      xj = row_rotate(xj);
-      yj = row_rotate(yj);
-      zj = row_rotate(zj);
-      sj = row_rotate(sj);
-      accj = row_rotate(accj);
-      ej = static_cast<unsigned>(row_rotate(static_cast<int>(ej)));
     }
     if (tt.ivalid) {
       lds_add(&L.psi[tt.i], Prec<double>::fixed(static_cast<double>(acci) * psi_scale));
     }
-    const int jl_home = (il - tt.nsteps) & 15;
+    // After the last step the j sums sit nsteps - 1 lanes to the right of their atoms.
+    const int jl_home = (il - (tt.nsteps - 1)) & 15;
     if (jl_home < tt.nj) {
       lds_add(&L.psi[tt.jb + jl_home], Prec<double>::fixed(static_cast<double>(accj) *
                                                            psi_scale));
@@ -558,6 +599,7 @@ __device__ void stage_radii(StageCtx<T> &c, const MMKit &k, double coulomb_inv_s
       }
     }
     L.rad[a] = static_cast<T>(R);
+    L.irad[a] = static_cast<T>(1.0 / R);
     // q holds charge * sqrt(Coulomb), so q^2 = Coulomb * charge^2.
     const double qg = k.q[g];
     const double expmkf = ((k.kappa != 0.0) ? exp(-kGbKscale * k.kappa * R) : 1.0) * inv_diel;
@@ -633,7 +675,11 @@ __device__ __forceinline__ void clash_pair(T r, T qq, T lja, T ljb, T sigma, T c
 }
 
 // CLASH selects the kernels that damp Coulomb and Lennard-Jones for the systems whose clash
-// flag is set in this evaluation; the other systems take the plain potentials.
+// flag is set in this evaluation; the other systems take the plain potentials.  The rotation
+// steps run in groups of kStepGroup with compile-time rotation counts, as in stage_psi: within a
+// group the j-side data stay put and are read through DPP operands; the j-side accumulators move
+// one lane per step.  The radius-derivative sums hold dE/dR / R and are scaled
+// by the atom's radius at the end.
 template <typename T, bool FORCES, bool LJTABLE, bool GB, bool CLASH>
 __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
   using P = Prec<T>;
@@ -657,6 +703,9 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
     tile_coords(c, tt, xi, yi, zi, xj, yj, zj);
     const T qi = L.q[tt.i], p1i = L.p1[tt.i], p2i = L.p2[tt.i], Ri = L.rad[tt.i];
     T qj = L.q[tt.j0], p1j = L.p1[tt.j0], p2j = L.p2[tt.j0], Rj = L.rad[tt.j0];
+    // exp(-r^2 / (4 Ri Rj)) = GbExp(r^2 * (1 / Rj) * (scale / Ri)).
+    T iRj = GB ? L.irad[tt.j0] : T(0);
+    const T iRi = GB ? L.irad[tt.i] * GbExp<T>::kScale : T(0);
     T fxi = T(0), fyi = T(0), fzi = T(0), dri = T(0);
     T fxj = T(0), fyj = T(0), fzj = T(0), drj = T(0);
     // Each lane sums its energies over the task in the calculation precision and converts the
@@ -677,120 +726,133 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
       ca = T(-0.5) / (cd * cd * (cd + T(1)));
       cb = (T(1) / cd) - (ca * (cd + T(1)) * (cd + T(1)));
     }
-    for (int ks = 0; ks < tt.nsteps; ks++) {
-      const bool valid = step_bit(tt.valid, ks);
-      const bool nbok = step_bit(nbmask, ks);
-      const T dx = xj - xi;
-      const T dy = yj - yi;
-      const T dz = zj - zi;
-      const T r2 = valid ? ((dx * dx) + (dy * dy) + (dz * dz)) : T(1);
-      const T qiqj = qi * qj;
-      T fmag = T(0);
-      if (GB) {
-        const T qgb = valid ? qiqj : T(0);
-        const T rr = Ri * Rj;
-        const T efac = gb_efac(r2, rr);
-        const T fgb2 = r2 + (rr * efac);
-        const T fgbi = P::rsqrt(fgb2);
-        // egb = qq (1 - exp(-kappa fgb) / diel) / fgb, temp6 = -(dE/dfgb) / fgb.
-        T egb, temp6;
-        if (use_kappa) {
-          const T fgbk = -kap * fgb2 * fgbi;
-          const T expmkf = P::exp(fgbk) * inv_diel;
-          const T dielfac = T(1) - expmkf;
-          const T qf = qgb * fgbi;
-          egb = qf * dielfac;
-          temp6 = qf * fgbi * fgbi * (dielfac + (fgbk * expmkf));
+    // The clash-damped kernels evaluate both potentials in one step (damping is per system, so a
+    // wavefront's rows can differ): they rotate every step to stay within the register budget.
+    constexpr int kGroup = CLASH ? 1 : kStepGroup;
+    unsigned vbits = tt.valid, nbbits = nbmask;
+    for (int g = 0; g < tt.nsteps; g += kGroup) {
+      auto step = [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        if (g + K >= tt.nsteps) {
+          return;
         }
-        else {
-          egb = qgb * fgbi * dielfac0;
-          temp6 = egb * fgbi * fgbi;
-        }
-        e_gb -= egb;
-        if (FORCES) {
-          fmag += temp6 * (T(1) - (T(0.25) * efac));
-          const T temp5 = T(0.5) * efac * temp6 * (rr + (T(0.25) * r2));
-          dri += Ri * temp5;
-          drj += Rj * temp5;
-        }
-      }
-      const T qnb = nbok ? qiqj : T(0);
-      if (CLASH && clash_on) {
-        T lja, ljb, sigma;
-        if (LJTABLE) {
-          const double2 ab = k.ljab[lj_off + (static_cast<int>(p1i) * lj_nt) +
-                                    static_cast<int>(p1j)];
-          lja = nbok ? static_cast<T>(ab.x) : T(0);
-          ljb = nbok ? static_cast<T>(ab.y) : T(0);
-          sigma = (ljb > T(1.0e-6)) ? P::sqrt(P::cbrt(lja / ljb)) : T(0);
-        }
-        else {
-          const T sig = p1i + p1j;
-          const T sig2 = sig * sig;
-          const T sig6 = sig2 * sig2 * sig2;
-          const T e4 = nbok ? p2i * p2j : T(0);
-          ljb = e4 * sig6;
-          lja = ljb * sig6;
-          sigma = (ljb > T(1.0e-6)) ? sig : T(0);
-        }
-        clash_pair<T, FORCES>(P::sqrt(r2), qnb, lja, ljb, sigma, cd, ca, cb, cratio, e_qq, e_lj,
-                              fmag);
-      }
-      else {
-        const T invr = P::rsqrt(r2);
-        const T invr2 = invr * invr;
-        const T eqq = qnb * invr;
-        e_qq += eqq;
-        if (LJTABLE) {
-          const double2 ab = k.ljab[lj_off + (static_cast<int>(p1i) * lj_nt) +
-                                    static_cast<int>(p1j)];
-          const T lja = nbok ? static_cast<T>(ab.x) : T(0);
-          const T ljb = nbok ? static_cast<T>(ab.y) : T(0);
-          const T invr4 = invr2 * invr2;
-          e_lj += ((lja * invr4 * invr4) - (ljb * invr2)) * invr4;
-          if (FORCES) {
-            fmag += ((T(6) * ljb) - (T(12) * lja * invr4 * invr2)) * invr4 * invr4;
-          }
-        }
-        else {
-          const T sig = p1i + p1j;
-          const T s2 = sig * sig * invr2;
-          const T s6 = s2 * s2 * s2;
-          const T e4 = nbok ? p2i * p2j : T(0);
-          const T e4s6 = e4 * s6;
-          e_lj += e4s6 * (s6 - T(1));
-          if (FORCES) {
-            fmag += e4s6 * (T(6) - (T(12) * s6)) * invr2;
-          }
-        }
-        if (FORCES) {
-          fmag -= eqq * invr2;
-        }
-      }
-      if (FORCES) {
-        fxi += fmag * dx;
-        fyi += fmag * dy;
-        fzi += fmag * dz;
-        fxj -= fmag * dx;
-        fyj -= fmag * dy;
-        fzj -= fmag * dz;
-      }
-      xj = row_rotate(xj);
-      yj = row_rotate(yj);
-      zj = row_rotate(zj);
-      qj = row_rotate(qj);
-      p1j = row_rotate(p1j);
-      p2j = row_rotate(p2j);
-      if (GB) {
-        Rj = row_rotate(Rj);
-      }
-      if (FORCES) {
-        fxj = row_rotate(fxj);
-        fyj = row_rotate(fyj);
-        fzj = row_rotate(fzj);
+        const bool valid = ((vbits >> K) & 1u) != 0u;
+        const bool nbok = ((nbbits >> K) & 1u) != 0u;
+        const T dx = row_ror<K>(xj) - xi;
+        const T dy = row_ror<K>(yj) - yi;
+        const T dz = row_ror<K>(zj) - zi;
+        const T r2c = (dx * dx) + (dy * dy) + (dz * dz);
+        const T r2 = valid ? r2c : T(1);
+        const T qiqj = row_ror<K>(qj) * qi;
+        T fmag = T(0);
         if (GB) {
-          drj = row_rotate(drj);
+          const T qgb = valid ? qiqj : T(0);
+          const T rr = row_ror<K>(Rj) * Ri;
+          const T efac = GbExp<T>::eval(r2 * (row_ror<K>(iRj) * iRi));
+          const T fgb2 = r2 + (rr * efac);
+          const T fgbi = P::rsqrt(fgb2);
+          // egb = qq (1 - exp(-kappa fgb) / diel) / fgb, temp6 = -(dE/dfgb) / fgb.
+          T egb, temp6;
+          if (use_kappa) {
+            const T fgbk = -kap * fgb2 * fgbi;
+            const T expmkf = P::exp(fgbk) * inv_diel;
+            const T dielfac = T(1) - expmkf;
+            const T qf = qgb * fgbi;
+            egb = qf * dielfac;
+            temp6 = qf * fgbi * fgbi * (dielfac + (fgbk * expmkf));
+          }
+          else {
+            egb = qgb * fgbi * dielfac0;
+            temp6 = egb * fgbi * fgbi;
+          }
+          e_gb -= egb;
+          if (FORCES) {
+            fmag += temp6 * (T(1) - (T(0.25) * efac));
+            const T temp5 = T(0.5) * efac * temp6 * (rr + (T(0.25) * r2));
+            dri += temp5;
+            drj = row_rotate(drj) + temp5;
+          }
         }
+        const T qnb = nbok ? qiqj : T(0);
+        if (CLASH && clash_on) {
+          const T pj1 = row_ror<K>(p1j);
+          const T pj2 = row_ror<K>(p2j);
+          T lja, ljb, sigma;
+          if (LJTABLE) {
+            const double2 ab = k.ljab[lj_off + (static_cast<int>(p1i) * lj_nt) +
+                                      static_cast<int>(pj1)];
+            lja = nbok ? static_cast<T>(ab.x) : T(0);
+            ljb = nbok ? static_cast<T>(ab.y) : T(0);
+            sigma = (ljb > T(1.0e-6)) ? P::sqrt(P::cbrt(lja / ljb)) : T(0);
+          }
+          else {
+            const T sig = p1i + pj1;
+            const T sig2 = sig * sig;
+            const T sig6 = sig2 * sig2 * sig2;
+            const T e4 = nbok ? p2i * pj2 : T(0);
+            ljb = e4 * sig6;
+            lja = ljb * sig6;
+            sigma = (ljb > T(1.0e-6)) ? sig : T(0);
+          }
+          clash_pair<T, FORCES>(P::sqrt(r2), qnb, lja, ljb, sigma, cd, ca, cb, cratio, e_qq, e_lj,
+                                fmag);
+        }
+        else {
+          const T invr = P::rsqrt(r2);
+          const T invr2 = invr * invr;
+          const T eqq = qnb * invr;
+          e_qq += eqq;
+          if (LJTABLE) {
+            const double2 ab = k.ljab[lj_off + (static_cast<int>(p1i) * lj_nt) +
+                                      static_cast<int>(row_ror<K>(p1j))];
+            const T lja = nbok ? static_cast<T>(ab.x) : T(0);
+            const T ljb = nbok ? static_cast<T>(ab.y) : T(0);
+            const T invr4 = invr2 * invr2;
+            e_lj += ((lja * invr4 * invr4) - (ljb * invr2)) * invr4;
+            if (FORCES) {
+              fmag += ((T(6) * ljb) - (T(12) * lja * invr4 * invr2)) * invr4 * invr4;
+            }
+          }
+          else {
+            const T sig = row_ror<K>(p1j) + p1i;
+            const T s2 = sig * sig * invr2;
+            const T s6 = s2 * s2 * s2;
+            const T e4c = row_ror<K>(p2j) * p2i;
+            const T e4 = nbok ? e4c : T(0);
+            const T e4s6 = e4 * s6;
+            e_lj += e4s6 * (s6 - T(1));
+            if (FORCES) {
+              fmag += e4s6 * (T(6) - (T(12) * s6)) * invr2;
+            }
+          }
+          if (FORCES) {
+            fmag -= eqq * invr2;
+          }
+        }
+        if (FORCES) {
+          const T fdx = fmag * dx;
+          const T fdy = fmag * dy;
+          const T fdz = fmag * dz;
+          fxi += fdx;
+          fyi += fdy;
+          fzi += fdz;
+          fxj = row_rotate(fxj) - fdx;
+          fyj = row_rotate(fyj) - fdy;
+          fzj = row_rotate(fzj) - fdz;
+        }
+      };
+      unrolled_steps(step, std::make_integer_sequence<int, kGroup>{});
+      vbits >>= kGroup;
+      nbbits >>= kGroup;
+      xj = row_ror<kGroup>(xj);
+      yj = row_ror<kGroup>(yj);
+      zj = row_ror<kGroup>(zj);
+      qj = row_ror<kGroup>(qj);
+      p1j = row_ror<kGroup>(p1j);
+      p2j = row_ror<kGroup>(p2j);
+      if (GB) {
+        Rj = row_ror<kGroup>(Rj);
+        iRj = row_ror<kGroup>(iRj);
       }
     }
     if (FORCES) {
@@ -799,17 +861,18 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
         lds_add(&L.fy[tt.i], P::fixed(fyi * frc_scale));
         lds_add(&L.fz[tt.i], P::fixed(fzi * frc_scale));
         if (GB) {
-          lds_add(&L.dedr[tt.i], P::fixed(dri * dedr_scale));
+          lds_add(&L.dedr[tt.i], P::fixed(Ri * dri * dedr_scale));
         }
       }
-      const int jl_home = (il - tt.nsteps) & 15;
+      // After the last step the j sums sit nsteps - 1 lanes to the right of their atoms.
+      const int jl_home = (il - (tt.nsteps - 1)) & 15;
       if (jl_home < tt.nj) {
         const int j = tt.jb + jl_home;
         lds_add(&L.fx[j], P::fixed(fxj * frc_scale));
         lds_add(&L.fy[j], P::fixed(fyj * frc_scale));
         lds_add(&L.fz[j], P::fixed(fzj * frc_scale));
         if (GB) {
-          lds_add(&L.dedr[j], P::fixed(drj * dedr_scale));
+          lds_add(&L.dedr[j], P::fixed(L.rad[j] * drj * dedr_scale));
         }
       }
     }
@@ -880,8 +943,9 @@ __device__ __forceinline__ void deriv_drain(StageCtx<T> &c, const unsigned *q, i
 }
 
 // Forces from the Born-radius derivatives, src/Potential/nonbonded_potential.tpp:1018-1292,
-// with the same far-field / queued near-field split as stage_psi.  Far-field forces are summed
-// per lane in the working precision and converted to fixed precision once per task.
+// with the same far-field / queued near-field split and step groups as stage_psi.  Far-field
+// forces are summed per lane in the working precision and converted to fixed precision once per
+// task.
 template <typename T>
 // Always inlined: kept whole inside the kernels that evaluate forces.
 __device__ __forceinline__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
@@ -890,7 +954,7 @@ __device__ __forceinline__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
   const int il = c.tid & 15;
   const int lane = c.tid & 63;
   const int wave = c.tid >> 6;
-  unsigned *q = L.nearq + (wave * kNearQueue);
+  unsigned *q = L.nearq + (wave * kNearQueueStride);
   int qn = 0;
   const T frc_scale = T(ldexp(1.0, k.frc_bits));
   const int quads = (c.wu.task_count + 3) >> 2;
@@ -900,62 +964,79 @@ __device__ __forceinline__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
     tile_coords(c, tt, xi, yi, zi, xj, yj, zj);
     const T si = L.scr[tt.i], sdi = L.sd[tt.i];
     const T si2 = si * si;
-    T sj = L.scr[tt.j0], sdj = L.sd[tt.j0];
+    const T si3 = si2 * si;
+    const T sj = L.scr[tt.j0];
+    T sj2 = sj * sj;
+    T sj3 = sj2 * sj;
+    T sdj = L.sd[tt.j0];
     const unsigned ei = static_cast<unsigned>(tt.i);
-    unsigned ej = static_cast<unsigned>(tt.j0) << 9;
+    int ej = tt.j0 << 9;
     T fxi = T(0), fyi = T(0), fzi = T(0);
-    T fxj = T(0), fyj = T(0), fzj = T(0);
-    for (int ks = 0; ks < tt.nsteps; ks++) {
-      const bool valid = step_bit(tt.valid, ks);
-      const T dx = xj - xi;
-      const T dy = yj - yi;
-      const T dz = zj - zi;
-      const T r2 = valid ? ((dx * dx) + (dy * dy) + (dz * dz)) : T(1);
-      const T invr2 = P::rcp(r2);
-      const T r2q = T(0.0625) * r2;
-      const T sj2 = sj * sj;
-      const bool far_i = r2q > sj2;
-      const bool far_j = r2q > si2;
-      // Both series are evaluated on every lane and selected, which keeps the step free of
-      // branches.
-      const T ti = dpsi_far(sj, sj2, invr2) * sdi;
-      const T tj = dpsi_far(si, si2, invr2) * sdj;
-      const T fmag = ((valid && far_i) ? ti : T(0)) + ((valid && far_j) ? tj : T(0));
-      fxi -= fmag * dx;
-      fyi -= fmag * dy;
-      fzi -= fmag * dz;
-      fxj += fmag * dx;
-      fyj += fmag * dy;
-      fzj += fmag * dz;
-      const bool near_i = valid && !far_i;
-      const bool near_j = valid && !far_j;
-      nearq_push(q, qn, near_i || near_j,
-                 ei | ej | (near_i ? kNearI : 0u) | (near_j ? kNearJ : 0u));
-      if (qn >= 64) {
-        deriv_drain(c, q, 64, lane, k.frc_bits);
-        nearq_shift(q, qn, lane);
+    T gxj = T(0), gyj = T(0), gzj = T(0);
+    unsigned vbits = tt.valid;
+    for (int g = 0; g < tt.nsteps; g += kStepGroup) {
+      auto step = [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        if (g + K >= tt.nsteps) {
+          return;
+        }
+        // Absent pairs are put far away, as in stage_psi.
+        const bool valid = ((vbits >> K) & 1u) != 0u;
+        const T dx = row_ror<K>(xj) - xi;
+        const T dy = row_ror<K>(yj) - yi;
+        const T dz = row_ror<K>(zj) - zi;
+        const T r2c = (dx * dx) + (dy * dy) + (dz * dz);
+        const T r2 = valid ? r2c : FarAway<T>::kR2;
+        const T invr2 = P::rcp(r2);
+        const T invr6 = invr2 * invr2 * invr2;
+        const T x_i = row_ror<K>(sj2) * invr2;
+        const T x_j = si2 * invr2;
+        const bool far_i = x_i < T(0.0625);
+        const bool far_j = x_j < T(0.0625);
+        // Both series are evaluated on every lane and selected, which keeps the step free of
+        // branches.
+        const T ti = dpsi_series(x_i) * (row_ror<K>(sj3) * (invr6 * sdi));
+        const T tj = dpsi_series(x_j) * (row_ror<K>(sdj) * (invr6 * si3));
+        const T fmag = (far_i ? ti : T(0)) + (far_j ? tj : T(0));
+        const T fdx = fmag * dx;
+        const T fdy = fmag * dy;
+        const T fdz = fmag * dz;
+        fxi -= fdx;
+        fyi -= fdy;
+        fzi -= fdz;
+        gxj = row_rotate(gxj) + fdx;
+        gyj = row_rotate(gyj) + fdy;
+        gzj = row_rotate(gzj) + fdz;
+        nearq_push(q, qn, !(far_i && far_j),
+                   ei | static_cast<unsigned>(row_ror<K>(ej)) | (far_i ? 0u : kNearI) |
+                       (far_j ? 0u : kNearJ),
+                   lane);
+      };
+      unrolled_steps(step, std::integer_sequence<int, 0, 1, 2, 3>{});
+      vbits >>= kStepGroup;
+      xj = row_ror<kStepGroup>(xj);
+      yj = row_ror<kStepGroup>(yj);
+      zj = row_ror<kStepGroup>(zj);
+      sj2 = row_ror<kStepGroup>(sj2);
+      sj3 = row_ror<kStepGroup>(sj3);
+      sdj = row_ror<kStepGroup>(sdj);
+      ej = row_ror<kStepGroup>(ej);
+      while (qn >= 64) {
+        deriv_drain(c, q + (qn - 64), 64, lane, k.frc_bits);
+        qn -= 64;
       }
-      xj = row_rotate(xj);
-      yj = row_rotate(yj);
-      zj = row_rotate(zj);
-      sj = row_rotate(sj);
-      sdj = row_rotate(sdj);
-      fxj = row_rotate(fxj);
-      fyj = row_rotate(fyj);
-      fzj = row_rotate(fzj);
-      ej = static_cast<unsigned>(row_rotate(static_cast<int>(ej)));
     }
     if (tt.ivalid) {
       lds_add(&L.fx[tt.i], P::fixed(fxi * frc_scale));
       lds_add(&L.fy[tt.i], P::fixed(fyi * frc_scale));
       lds_add(&L.fz[tt.i], P::fixed(fzi * frc_scale));
     }
-    const int jl_home = (il - tt.nsteps) & 15;
+    const int jl_home = (il - (tt.nsteps - 1)) & 15;
     if (jl_home < tt.nj) {
       const int j = tt.jb + jl_home;
-      lds_add(&L.fx[j], P::fixed(fxj * frc_scale));
-      lds_add(&L.fy[j], P::fixed(fyj * frc_scale));
-      lds_add(&L.fz[j], P::fixed(fzj * frc_scale));
+      lds_add(&L.fx[j], P::fixed(gxj * frc_scale));
+      lds_add(&L.fy[j], P::fixed(gyj * frc_scale));
+      lds_add(&L.fz[j], P::fixed(gzj * frc_scale));
     }
   }
   deriv_drain(c, q, qn, lane, k.frc_bits);

@@ -273,62 +273,51 @@ hipError_t launch_conjugate_gradient(const CgKit &k, int step, int sd_cycles,
   return hipGetLastError();
 }
 
-// One line move per system, src/MolecularMechanics/line_movement.cui:4-415, one wavefront per
-// system and four systems per block.  Stopped systems (serving) keep their coordinates and line
-// state.
-constexpr int kLineWaves = 4;
+// Line-search state of system s.
+__device__ __forceinline__ LineState load_line(const LineKit &k, int s) {
+  LineState ls;
+  ls.l_move = k.l_move[s];
+  ls.s_move = k.s_move[s];
+  ls.mfac_a = k.mfac_a[s];
+  ls.mfac_b = k.mfac_b[s];
+  ls.mfac_c = k.mfac_c[s];
+  ls.nrg_a = k.nrg_a[s];
+  ls.nrg_b = k.nrg_b[s];
+  ls.nrg_c = k.nrg_c[s];
+  ls.nrg_d = k.nrg_d[s];
+  return ls;
+}
+__device__ __forceinline__ void store_line(const LineKit &k, int s, const LineState &ls) {
+  k.l_move[s] = ls.l_move;
+  k.s_move[s] = ls.s_move;
+  k.mfac_a[s] = ls.mfac_a;
+  k.mfac_b[s] = ls.mfac_b;
+  k.mfac_c[s] = ls.mfac_c;
+  k.nrg_a[s] = ls.nrg_a;
+  k.nrg_b[s] = ls.nrg_b;
+  k.nrg_c[s] = ls.nrg_c;
+  k.nrg_d[s] = ls.nrg_d;
+}
 
-__global__ void __launch_bounds__(64 * kLineWaves) k_line_advance(LineKit k, int move_number) {
-  const int lane = threadIdx.x & 63;
-  const int e = (blockIdx.x * kLineWaves) + static_cast<int>(threadIdx.x >> 6);
-  if (e >= k.nsys) {
-    return;
+// Line move `move` of system s at potential energy `cur` (src/MolecularMechanics/
+// line_movement.cui:67-330): records the energy, updates the state and returns the move in
+// fixed-precision coordinate units per fixed-precision direction unit (0 for no move).
+__device__ __forceinline__ double line_step(const LineKit &k, int s, int move, double cur) {
+  LineState ls = load_line(k, s);
+  double mf = line_move_factor(ls, move, cur);
+  store_line(k, s, ls);
+  if (fabs(mf) > 1.0e-12) {
+    mf *= (k.inv_frc_scale * k.gpos_scale) * ((move == 0) ? ls.l_move : ls.s_move);
   }
-  const int s = (k.slots != nullptr) ? k.slots[e] : e;
-  if (k.done != nullptr && k.done[s] != 0) {
-    return;
+  else {
+    mf = 0.0;
   }
-  // Potential energy: the 13 ScoreCard components, one per lane, summed exactly.
-  long long acc = (lane <= 12) ? k.scorecard[(static_cast<size_t>(s) * kScoreStride) + lane] : 0;
-  for (int m = 32; m >= 1; m >>= 1) {
-    acc += __shfl_xor(acc, m, 64);
-  }
-  double mf = 0.0;
-  if (lane == 0) {
-    const double cur = static_cast<double>(acc) * k.inv_nrg_scale;
-    LineState ls;
-    ls.l_move = k.l_move[s];
-    ls.s_move = k.s_move[s];
-    ls.mfac_a = k.mfac_a[s];
-    ls.mfac_b = k.mfac_b[s];
-    ls.mfac_c = k.mfac_c[s];
-    ls.nrg_a = k.nrg_a[s];
-    ls.nrg_b = k.nrg_b[s];
-    ls.nrg_c = k.nrg_c[s];
-    ls.nrg_d = k.nrg_d[s];
-    mf = line_move_factor(ls, move_number, cur);
-    k.l_move[s] = ls.l_move;
-    k.s_move[s] = ls.s_move;
-    k.mfac_a[s] = ls.mfac_a;
-    k.mfac_b[s] = ls.mfac_b;
-    k.mfac_c[s] = ls.mfac_c;
-    k.nrg_a[s] = ls.nrg_a;
-    k.nrg_b[s] = ls.nrg_b;
-    k.nrg_c[s] = ls.nrg_c;
-    k.nrg_d[s] = ls.nrg_d;
-    if (fabs(mf) > 1.0e-12) {
-      mf *= (k.inv_frc_scale * k.gpos_scale) * ((move_number == 0) ? ls.l_move : ls.s_move);
-    }
-    else {
-      mf = 0.0;
-    }
-  }
-  mf = __shfl(mf, 0, 64);
-  if (mf == 0.0) {
-    return;
-  }
-  const int start = k.sys_start[s];
-  const int n = k.sys_natom[s];
+  return mf;
+}
+
+// x += mf * direction over a system's atoms, one wavefront (line_movement.cui:343-406).
+__device__ __forceinline__ void line_apply(const LineKit &k, int start, int n, double mf,
+                                           int lane) {
   if (k.xcrd_ovrf == nullptr && k.xfrc_ovrf == nullptr) {
     for (int i = lane; i < n; i += 64) {
       const int g = start + i;
@@ -338,8 +327,8 @@ __global__ void __launch_bounds__(64 * kLineWaves) k_line_advance(LineKit k, int
     }
     return;
   }
-  // 95-bit coordinates and / or directions (line_movement.cui:343-406, the overflow forms): the
-  // increment is split as doubleToInt95 does and added with the carry of splitFPSum.
+  // 95-bit coordinates and / or directions (the overflow forms): the increment is split as
+  // doubleToInt95 does and added with the carry of splitFPSum.
   long long *crd[3] = {k.xcrd, k.ycrd, k.zcrd};
   int *crd_o[3] = {k.xcrd_ovrf, k.ycrd_ovrf, k.zcrd_ovrf};
   const long long *frc[3] = {k.xfrc, k.yfrc, k.zfrc};
@@ -366,6 +355,37 @@ __global__ void __launch_bounds__(64 * kLineWaves) k_line_advance(LineKit k, int
   }
 }
 
+// One line move per system, src/MolecularMechanics/line_movement.cui:4-415, one wavefront per
+// system and four systems per block.  Stopped systems (serving) keep their coordinates and line
+// state.
+constexpr int kLineWaves = 4;
+
+__global__ void __launch_bounds__(64 * kLineWaves) k_line_advance(LineKit k, int move_number) {
+  const int lane = threadIdx.x & 63;
+  const int e = (blockIdx.x * kLineWaves) + static_cast<int>(threadIdx.x >> 6);
+  if (e >= k.nsys) {
+    return;
+  }
+  const int s = (k.slots != nullptr) ? k.slots[e] : e;
+  if (k.done != nullptr && k.done[s] != 0) {
+    return;
+  }
+  // Potential energy: the 13 ScoreCard components, one per lane, summed exactly.
+  long long acc = (lane <= 12) ? k.scorecard[(static_cast<size_t>(s) * kScoreStride) + lane] : 0;
+  for (int m = 32; m >= 1; m >>= 1) {
+    acc += __shfl_xor(acc, m, 64);
+  }
+  double mf = 0.0;
+  if (lane == 0) {
+    mf = line_step(k, s, move_number, static_cast<double>(acc) * k.inv_nrg_scale);
+  }
+  mf = __shfl(mf, 0, 64);
+  if (mf == 0.0) {
+    return;
+  }
+  line_apply(k, k.sys_start[s], k.sys_natom[s], mf, lane);
+}
+
 hipError_t launch_line_advance(const LineKit &k, int move_number, hipStream_t stream) {
   if (k.nsys == 0) {
     return hipSuccess;
@@ -380,252 +400,98 @@ hipError_t launch_line_advance(const LineKit &k, int move_number, hipStream_t st
 namespace stormm_amd {
 
 // ------------------------------------------------------------------------------------------
-// Persistent minimizer.  One block owns one work unit for the whole minimization: parameters are
-// staged in LDS once, and every cycle runs the force + energy evaluation, the conjugate-gradient
-// direction update (one wave per system, deterministic butterfly reductions), and the four line
-// moves with three energy-only evaluations, exactly the sequence of the reference's host loop
-// (src/MolecularMechanics/hpc_minimization.cu:262-405) without returning to the host.
-// Coordinates stay in the global fixed-precision arrays (read back into LDS at each evaluation);
-// the normalized move direction is kept in the global force arrays as the reference does.  The
-// host launches it only for syntheses without restraints, so its evaluations leave the restraint
-// stage out.
+// Fused minimization cycle.  The reference moves the coordinates after each evaluation with a
+// launch of its own (line_movement.cui, launched four times per cycle from
+// hpc_minimization.cu:262-405).  Here the move that follows an evaluation is made by the next
+// evaluation while it loads the coordinates: its block takes each of its systems' move factor
+// from the previous evaluation's ScoreCard row and the line-search state, and adds the move to
+// each coordinate as it reads it, writing the moved coordinate back.  A cycle is then the force
+// evaluation, the conjugate-gradient kernel and three energy evaluations; the last move of the
+// cycle is made by the next cycle's force evaluation (or the final energy evaluation).  The
+// arithmetic of each move is k_line_advance's, so trajectories equal the per-step launches' bit
+// for bit.
 // ------------------------------------------------------------------------------------------
-struct MinLds {
-  LineState ls[kMaxWuSystems];
-  double mf[kMaxWuSystems];
-  double rms[kMaxWuSystems];
-  int done[kMaxWuSystems];
-  int cycles[kMaxWuSystems];
-};
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int m = 32; m >= 1; m >>= 1) {
-    v += __shfl_xor(v, m, 64);
-  }
-  return v;
-}
-
-template <typename T>
-__device__ __forceinline__ void apply_moves(StageCtx<T> &c, const MMKit &k, const MinLds *M) {
-  for (int a = c.tid; a < c.wu.lds_atoms; a += c.nthr) {
-    const int g = c.L.gidx[a];
-    if (g < 0) {
-      continue;
-    }
-    const double mf = M->mf[c.L.slot[a]];
-    if (mf != 0.0) {
-      k.xcrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.xfrc[g])));
-      k.ycrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.yfrc[g])));
-      k.zcrd[g] += static_cast<long long>(__builtin_rint(mf * static_cast<double>(k.zfrc[g])));
-    }
-  }
-  __syncthreads();
-}
-
-template <typename T>
-__device__ double wu_energy(const StageCtx<T> &c, int s) {
-  long long acc = 0;
-  for (int e = 0; e <= 12; e++) {
-    acc += c.L.nrg[s * 16 + e];
-  }
-  return static_cast<double>(acc);
-}
-
-// CLASH instantiations serve minimizations with clash-damping cycles: every evaluation takes
-// each system's damping from the cycle counter (off once the damped cycles are over, which gives
-// the plain potentials), so one inlined evaluation of each kind serves the whole run.
-template <typename T, bool CLASH>
-__global__ void __launch_bounds__(kMaxMinimizeBlock) k_minimize(MinKit mk) {
+// FORCES selects the force + energy evaluation, otherwise energy only.  pending_move (0..3) is
+// the line move to make while loading the coordinates, -1 none; a force evaluation makes move 3
+// only for systems that have completed a cycle (so one captured cycle serves the first cycle
+// too).  A unit whose systems have all stopped returns at once: its coordinates, forces and
+// ScoreCard rows already hold what an evaluation would write.
+template <typename T, bool FORCES, bool CLASH>
+__global__ void __launch_bounds__(EvalBlockLimit<T>::value)
+    k_min_cycle(MMKit k, CycleKit ck, int pending_move) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const MMKit &k = mk.mm;
-  const int waves = static_cast<int>(blockDim.x >> 6);
-  StageCtx<T> c(smem, k.maxa, waves);
-  MinLds *M = reinterpret_cast<MinLds *>(smem + WuLds<T>::bytes(k.maxa, waves));
+  const WuDesc &wu = k.wu[blockIdx.x];
+  const int tid = static_cast<int>(threadIdx.x);
+  const bool stopped = (tid >= wu.sys_count) ||
+                       (ck.cg.done[k.wu_sys[wu.sys_first + tid].sys_index] != 0);
+  if (__syncthreads_and(stopped ? 1 : 0) != 0) {
+    return;
+  }
   const int terms = kTermNonbonded | kTermValence;
+  StageCtx<T> c(smem, k.maxa, static_cast<int>(blockDim.x >> 6));
   init_ctx<T>(c, k, blockIdx.x, terms);
   stage_load_params<T>(c, k);
-  const int nsys = c.wu.sys_count;
-  const double inv_frc = ldexp(1.0, -k.frc_bits);
-  const double inv_nrg = ldexp(1.0, -k.nrg_bits);
-  const double move_scale = inv_frc * ldexp(1.0, k.gpos_bits);
-  const bool dp_mode = (sizeof(T) == 8);
-  if (c.tid < nsys) {
-    LineState ls;
-    ls.l_move = mk.dx0;
-    ls.s_move = mk.dx0;
-    ls.mfac_a = ls.mfac_b = ls.mfac_c = 0.0;
-    ls.nrg_a = ls.nrg_b = ls.nrg_c = ls.nrg_d = 0.0;
-    M->ls[c.tid] = ls;
-    M->mf[c.tid] = 0.0;
-    M->rms[c.tid] = 0.0;
-    M->done[c.tid] = 0;
-    M->cycles[c.tid] = 0;
-  }
-  // PhaseSpaceSynthesis::primeConjugateGradientCalculation for this unit's atoms.
-  for (int a = c.tid; a < c.wu.lds_atoms; a += c.nthr) {
-    const int g = c.L.gidx[a];
-    if (g >= 0) {
-      mk.xprv[g] = 0;
-      mk.yprv[g] = 0;
-      mk.zprv[g] = 0;
-      mk.xcg[g] = 0;
-      mk.ycg[g] = 0;
-      mk.zcg[g] = 0;
-    }
-  }
-  __syncthreads();
-  const int wave = c.tid >> 6;
-  const int lane = c.tid & 63;
-  const int nwaves = c.nthr >> 6;
-  for (int cyc = 0; cyc < mk.total_cycles; cyc++) {
-    bool all_done = true;
-    for (int s = 0; s < nsys; s++) {
-      all_done = all_done && (M->done[s] != 0);
-    }
-    if (all_done) {
-      break;
-    }
-    c.cycle = cyc;
-    evaluate_wu<T, true, CLASH, false>(c, k, terms);
-
-    // Convergence test and conjugate-gradient direction, one wave per system.  The three
-    // Cartesian components are handled by the same code on their own arrays.
-    for (int s = wave; s < nsys; s += nwaves) {
-      const WuSys ws = c.L.sys[s];
-      double ff = 0.0, gg = 0.0, dgg = 0.0;
-      auto sums = [&](const long long *lf, const long long *pv) {
-        for (int i = lane; i < ws.natom; i += 64) {
-          const long long lfv = lf[ws.lds_base + i];
-          const long long lpv = pv[ws.global_start + i];
-          const double df = static_cast<double>(lfv);
-          const double dp = static_cast<double>(lpv);
-          ff += df * df;
-          gg += dp * dp;
-          dgg += static_cast<double>(lfv - lpv) * df;
+  if (pending_move >= 0) {
+    // Move factors of the unit's systems (stopped systems do not move; line_step as in
+    // k_line_advance, on the exact sum of the 13 ScoreCard components).  The coordinate stage
+    // reads them after its first barrier.
+    if (c.tid < c.wu.sys_count) {
+      const int s = c.L.sys[c.tid].sys_index;
+      double mf = 0.0;
+      const bool first_cycle = FORCES && (ck.cg.steps[s] == 0);
+      if (ck.cg.done[s] == 0 && !first_cycle) {
+        long long acc = 0;
+        for (int e = 0; e < kNrgSlots; e++) {
+          acc += k.scorecard[(static_cast<size_t>(s) * kScoreStride) + e];
         }
-      };
-      sums(c.L.fx, mk.xprv);
-      sums(c.L.fy, mk.yprv);
-      sums(c.L.fz, mk.zprv);
-      ff = wave_sum(ff);
-      gg = wave_sum(gg);
-      dgg = wave_sum(dgg);
-      const double rms = sqrt(ff / (3.0 * ws.natom)) * inv_frc;
-      const bool done = (M->done[s] != 0) || (mk.conv_rms > 0.0 && rms <= mk.conv_rms);
-      double mf0 = 0.0;
-      if (!done) {
-        const int step = cyc + 1;
-        const double gam = (step >= mk.sd_cycles) ? (dgg / gg) : 0.0;
-        const bool use_gam = fabs(gam) > 1.0e-12;
-        double msum = 0.0;
-        auto direction = [&](const long long *lf, long long *pv, long long *cg, long long *gf) {
-          for (int i = lane; i < ws.natom; i += 64) {
-            const int g = ws.global_start + i;
-            const long long ifv = lf[ws.lds_base + i];
-            pv[g] = ifv;
-            if (use_gam) {
-              const double dfcg = static_cast<double>(ifv) + (gam * static_cast<double>(cg[g]));
-              msum += dfcg * dfcg;
-              const long long v = static_cast<long long>(__builtin_rint(dfcg));
-              cg[g] = v;
-              gf[g] = v;
-            }
-            else {
-              msum += static_cast<double>(ifv) * static_cast<double>(ifv);
-              cg[g] = ifv;
-              gf[g] = ifv;
-            }
-          }
-        };
-        direction(c.L.fx, mk.xprv, mk.xcg, k.xfrc);
-        direction(c.L.fy, mk.yprv, mk.ycg, k.yfrc);
-        direction(c.L.fz, mk.zprv, mk.zcg, k.zfrc);
-        msum = wave_sum(msum);
-        const double inv_factor = 1.0 / (sqrt(msum) * inv_frc);
-        auto normalize = [&](long long *gf) {
-          for (int i = lane; i < ws.natom; i += 64) {
-            const int g = ws.global_start + i;
-            const double v = static_cast<double>(gf[g]) * inv_factor;
-            gf[g] = dp_mode ? static_cast<long long>(__builtin_rint(v)) :
-                              static_cast<long long>(v);
-          }
-        };
-        normalize(k.xfrc);
-        normalize(k.yfrc);
-        normalize(k.zfrc);
+        mf = line_step(ck.line, s, pending_move,
+                       static_cast<double>(acc) * ck.line.inv_nrg_scale);
       }
-      if (lane == 0) {
-        M->rms[s] = rms;
-        if (!done) {
-          LineState ls = M->ls[s];
-          mf0 = line_move_factor(ls, 0, wu_energy(c, s) * inv_nrg);
-          M->ls[s] = ls;
-          M->mf[s] = (fabs(mf0) > 1.0e-12) ? mf0 * (move_scale * ls.l_move) : 0.0;
-          M->cycles[s] = cyc + 1;
-        }
-        else {
-          M->done[s] = 1;
-          M->mf[s] = 0.0;
-        }
-      }
+      c.L.move[c.tid] = mf;
     }
-    __syncthreads();
-    apply_moves<T>(c, k, M);
-    for (int move = 1; move < 4; move++) {
-      evaluate_wu<T, false, CLASH, false>(c, k, terms);
-      for (int s = c.tid; s < nsys; s += c.nthr) {
-        if (M->done[s] == 0) {
-          LineState ls = M->ls[s];
-          const double mf = line_move_factor(ls, move, wu_energy(c, s) * inv_nrg);
-          M->ls[s] = ls;
-          M->mf[s] = (fabs(mf) > 1.0e-12) ? mf * (move_scale * ls.s_move) : 0.0;
-        }
-        else {
-          M->mf[s] = 0.0;
-        }
-      }
-      __syncthreads();
-      apply_moves<T>(c, k, M);
-    }
+    c.move_pending = true;
   }
-  // Final energies of every system (hpc_minimization.cu:395-405), undamped.
-  c.cycle = k.clash_cycles;
-  evaluate_wu<T, false, CLASH, false>(c, k, terms);
-  stage_store<T, false>(c, k, true);
-  for (int s = c.tid; s < nsys; s += c.nthr) {
-    const int si = c.L.sys[s].sys_index;
-    if (mk.cycles_run != nullptr) {
-      mk.cycles_run[si] = M->cycles[s];
-    }
-    if (mk.rms_force != nullptr) {
-      mk.rms_force[si] = M->rms[s];
-    }
-    if (mk.l_move != nullptr) {
-      mk.l_move[si] = M->ls[s].l_move;
-    }
-  }
+  evaluate_wu<T, FORCES, CLASH, false>(c, k, terms);
+  stage_store<T, FORCES>(c, k, true);
 }
 
-template <typename T>
-static hipError_t launch_min_t(const MinKit &k, hipStream_t stream) {
-  const int block = std::min(k.mm.block, kMaxMinimizeBlock);
-  const size_t lds = WuLds<T>::bytes(k.mm.maxa, block / 64) + sizeof(MinLds);
-  auto fn = (k.mm.clash_variant != 0) ? k_minimize<T, true> : k_minimize<T, false>;
+template <typename T, bool FORCES, bool CLASH>
+static hipError_t launch_cycle_t(const MMKit &k, const CycleKit &ck, int pending_move,
+                                 hipStream_t stream) {
+  const int block = std::min(k.block, EvalBlockLimit<T>::value);
+  const size_t lds = WuLds<T>::bytes(k.maxa, block / 64);
+  auto fn = k_min_cycle<T, FORCES, CLASH>;
   hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        static_cast<int>(lds));
   if (err != hipSuccess) {
     return err;
   }
-  hipLaunchKernelGGL(fn, dim3(k.mm.nwu), dim3(block), lds, stream, k);
+  hipLaunchKernelGGL(fn, dim3(k.nwu), dim3(block), lds, stream, k, ck, pending_move);
   return hipGetLastError();
 }
 
-hipError_t launch_minimize_persistent(int precision, const MinKit &k, hipStream_t stream) {
-  if (k.mm.nwu == 0) {
+hipError_t launch_min_cycle(int precision, const MMKit &k, const CycleKit &ck, bool forces,
+                            int pending_move, hipStream_t stream) {
+  if (k.nwu == 0) {
     return hipSuccess;
   }
-  return (precision == 0) ? launch_min_t<double>(k, stream) : launch_min_t<float>(k, stream);
+  const bool clash = (k.clash_variant != 0);
+  if (precision == 0) {
+    if (forces) {
+      return clash ? launch_cycle_t<double, true, true>(k, ck, pending_move, stream) :
+                     launch_cycle_t<double, true, false>(k, ck, pending_move, stream);
+    }
+    return clash ? launch_cycle_t<double, false, true>(k, ck, pending_move, stream) :
+                   launch_cycle_t<double, false, false>(k, ck, pending_move, stream);
+  }
+  if (forces) {
+    return clash ? launch_cycle_t<float, true, true>(k, ck, pending_move, stream) :
+                   launch_cycle_t<float, true, false>(k, ck, pending_move, stream);
+  }
+  return clash ? launch_cycle_t<float, false, true>(k, ck, pending_move, stream) :
+                 launch_cycle_t<float, false, false>(k, ck, pending_move, stream);
 }
 
 __global__ void __launch_bounds__(256) k_pool_init_slots(SlotInitKit k) {

@@ -1530,6 +1530,177 @@ static int minimize_graph(int32_t precision, const stormm_synthesis *ag_c,
   return STORMM_OK;
 }
 
+// Per-system minimizer state of the fused cycle kernels: completed cycles, stop flags and the
+// RMS force of the last direction update.
+struct CycleState {
+  int *steps = nullptr;
+  int *done = nullptr;
+  double *rms = nullptr;
+};
+
+// How the fused cycle kernels are issued.
+enum class FusedIssue {
+  kDirect,      // launched one by one on the caller's stream; the call returns at once
+  kWholeGraph,  // the whole run captured in one HIP graph, launched once; returns when done
+  kCycleGraph   // one cycle captured (plus a clash-damped one) and replayed; returns when done
+};
+
+// Launches of one fused cycle: the force evaluation (making the previous cycle's last line move
+// first), the conjugate-gradient update, and three energy evaluations, each making the line move
+// that follows the evaluation before it.
+static hipError_t launch_fused_cycle(int prec, const MMKit &kf, const MMKit &ke,
+                                     const CycleKit &ck, int sd_cycles, hipStream_t st) {
+  hipError_t err = launch_min_cycle(prec, kf, ck, true, 3, st);
+  if (err == hipSuccess) {
+    err = launch_conjugate_gradient(ck.cg, 0, sd_cycles, st);
+  }
+  for (int move = 0; move < 3 && err == hipSuccess; move++) {
+    err = launch_min_cycle(prec, ke, ck, false, move, st);
+  }
+  return err;
+}
+
+// Captures one fused cycle into an executable graph.
+static hipError_t capture_fused_cycle(int prec, const MMKit &kf, const MMKit &ke,
+                                      const CycleKit &ck, int sd_cycles, hipStream_t cap,
+                                      hipGraph_t *graph, hipGraphExec_t *exec) {
+  hipError_t err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (err != hipSuccess) {
+    return err;
+  }
+  const hipError_t e2 = launch_fused_cycle(prec, kf, ke, ck, sd_cycles, cap);
+  err = hipStreamEndCapture(cap, graph);
+  if (err == hipSuccess) {
+    err = e2;
+  }
+  if (err == hipSuccess) {
+    err = hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0);
+  }
+  return err;
+}
+
+// The reference's minimization loop (hpc_minimization.cu:262-405) with each line move made by
+// the evaluation that follows it (launch_fused_cycle), then the final energy evaluation, which
+// makes the last move.  Clash damping follows each system's cycle count in cs.steps, as in the
+// graph replay, so one captured cycle serves every cycle of its kind.
+static int minimize_fused(int32_t precision, const stormm_synthesis *ag,
+                          const stormm_ps_writer &w, const stormm_minimize_controls *mc,
+                          const ScView &sc, const LmView &lm, const CycleState &cs,
+                          hipStream_t stream, FusedIssue how) {
+  if (!ag->plan.rst.empty()) {
+    return fail(STORMM_ERR_STATE, "the fused-cycle minimizer does not evaluate restraints; "
+                                  "minimize a restrained synthesis with the per-step (0) or "
+                                  "graph (2) mode");
+  }
+  if (w.gpos_bits > kMaxPlainGposBits || w.frc_bits > kMaxPlainFrcBits) {
+    return fail(STORMM_ERR_STATE, "the fused-cycle minimizer takes at most 38 coordinate and 40 "
+                                  "force bits; use the per-step (0) or graph (2) mode");
+  }
+  const int ns = ag->plan.nsys;
+  const int prec = (precision == STORMM_PREC_DOUBLE) ? 0 : 1;
+  MMKit k = make_kit(ag, w, &sc);
+  k.store_overwrite = 1;
+  set_clash(k, precision, 0.0, 0.0);
+  CycleKit ck;
+  ck.cg = make_cg_kit(ag, w, precision);
+  ck.cg.steps = cs.steps;
+  ck.cg.done = cs.done;
+  ck.cg.rms = cs.rms;
+  ck.cg.conv_rms = mc->convergence_rms_force;
+  ck.cg.max_steps = mc->total_cycles;
+  ck.line = make_line_kit(ag, w, sc, lm, precision);
+  ck.line.done = cs.done;
+  const int sd = mc->steepest_descent_cycles;
+  const int cdcyc = std::min(mc->clash_damping_cycles, mc->total_cycles);
+  // Damped evaluations read the cycle count before this cycle's direction update advances it
+  // (force evaluation) or after (energy evaluations).
+  MMKit kf_clash = k;
+  kf_clash.clash_variant = 1;
+  kf_clash.clash_cycles = cdcyc;
+  kf_clash.clash_step = cs.steps;
+  kf_clash.clash_step_offset = 0;
+  kf_clash.clash_r0 = mc->clash_minimum_distance;
+  kf_clash.clash_ratio = mc->clash_vdw_ratio;
+  MMKit ke_clash = kf_clash;
+  ke_clash.clash_step_offset = -1;
+  MMKit kfin = k;
+  kfin.rst_step = minimizer_step(mc->total_cycles, 1, cdcyc);
+  const int terms = kTermNonbonded | kTermValence;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipStream_t issue = stream;
+  hipError_t err = hipSuccess;
+  if (how != FusedIssue::kDirect) {
+    HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  }
+  if (how == FusedIssue::kWholeGraph) {
+    err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    issue = cap;
+  }
+  if (err == hipSuccess) {
+    err = (stormm_ps_writer_prime_conjugate_gradient(&w, ag->plan.padded_atoms, issue) ==
+           STORMM_OK) ? hipSuccess : hipErrorInvalidValue;
+  }
+  if (err == hipSuccess) {
+    err = launch_fill_int(cs.steps, cs.done, cs.done, static_cast<size_t>(ns), issue);
+  }
+  if (err == hipSuccess) {
+    err = launch_fill(reinterpret_cast<long long *>(cs.rms), nullptr, nullptr,
+                      static_cast<size_t>(ns), issue);
+  }
+  if (how == FusedIssue::kCycleGraph) {
+    if (err == hipSuccess) {
+      err = capture_fused_cycle(prec, k, k, ck, sd, cap, &graph[0], &exec[0]);
+    }
+    if (err == hipSuccess && cdcyc > 0) {
+      err = capture_fused_cycle(prec, kf_clash, ke_clash, ck, sd, cap, &graph[1], &exec[1]);
+    }
+    for (int c = 0; c < mc->total_cycles && err == hipSuccess; c++) {
+      err = hipGraphLaunch(exec[(c < cdcyc) ? 1 : 0], stream);
+    }
+  }
+  else {
+    for (int c = 0; c < mc->total_cycles && err == hipSuccess; c++) {
+      const bool damped = (c < cdcyc);
+      err = launch_fused_cycle(prec, damped ? kf_clash : k, damped ? ke_clash : k, ck, sd, issue);
+    }
+  }
+  if (err == hipSuccess) {
+    // The final energies, after the last cycle's last move.
+    err = (mc->total_cycles > 0) ? launch_min_cycle(prec, kfin, ck, false, 3, issue)
+                                 : launch_mm_eval(prec, kfin, terms, 0, 1, issue);
+  }
+  if (how == FusedIssue::kWholeGraph) {
+    const hipError_t e1 = hipStreamEndCapture(cap, &graph[0]);
+    if (err == hipSuccess) {
+      err = e1;
+    }
+    if (err == hipSuccess) {
+      err = hipGraphInstantiate(&exec[0], graph[0], nullptr, nullptr, 0);
+    }
+    if (err == hipSuccess) {
+      err = hipGraphLaunch(exec[0], stream);
+    }
+  }
+  if (how != FusedIssue::kDirect) {
+    if (err == hipSuccess) {
+      err = hipStreamSynchronize(stream);
+    }
+    for (int g = 0; g < 2; g++) {
+      if (exec[g] != nullptr) {
+        (void)hipGraphExecDestroy(exec[g]);
+      }
+      if (graph[g] != nullptr) {
+        (void)hipGraphDestroy(graph[g]);
+      }
+    }
+    (void)hipStreamDestroy(cap);
+  }
+  HIP_TRY(err);
+  return STORMM_OK;
+}
+
 static int minimize(int32_t precision, const stormm_synthesis *ag, const stormm_ps_writer &w,
                     const stormm_minimize_controls *mincon, const ScView &sc, const LmView &lm,
                     int32_t persistent, void *stream) {
@@ -1551,43 +1722,21 @@ static int minimize(int32_t precision, const stormm_synthesis *ag, const stormm_
   if (persistent == 2) {
     return minimize_graph(precision, ag, w, mincon, sc, lm, st);
   }
-  if (!ag->plan.rst.empty()) {
-    return fail(STORMM_ERR_STATE, "the persistent minimizer does not evaluate restraints; "
-                                  "minimize a restrained synthesis with the per-step (0) or "
-                                  "graph (2) mode");
+  if (persistent != 1 && persistent != 3 && persistent != 4) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "minimization mode must be 0, 1, 2, 3 or 4");
   }
-  if (w.gpos_bits > kMaxPlainGposBits || w.frc_bits > kMaxPlainFrcBits) {
-    return fail(STORMM_ERR_STATE, "the persistent minimizer takes at most 38 coordinate and 40 "
-                                  "force bits; use the per-step (0) or graph (2) mode");
+  stormm_synthesis *agm = const_cast<stormm_synthesis *>(ag);
+  CycleState cs;
+  cs.steps = agm->cycles_run.as<int>();
+  cs.done = agm->stop_flags.as<int>();
+  cs.rms = agm->rms_force.as<double>();
+  const FusedIssue how = (persistent == 1) ? FusedIssue::kWholeGraph :
+                         (persistent == 3) ? FusedIssue::kCycleGraph : FusedIssue::kDirect;
+  rc = minimize_fused(precision, ag, w, mincon, sc, lm, cs, st, how);
+  if (rc == STORMM_OK && how == FusedIssue::kDirect) {
+    HIP_TRY(hipStreamSynchronize(st));
   }
-  MinKit k;
-  k.mm = make_kit(ag, w, &sc);
-  // The persistent kernel switches to its damped evaluations for cycles below clash_cycles and
-  // takes the schedule from its own cycle counter.
-  set_clash(k.mm, precision, 0.0, 0.0);
-  if (mincon->clash_damping_cycles > 0) {
-    k.mm.clash_variant = 1;
-    k.mm.clash_cycles = mincon->clash_damping_cycles;
-    k.mm.clash_r0 = mincon->clash_minimum_distance;
-    k.mm.clash_ratio = mincon->clash_vdw_ratio;
-  }
-  k.xprv = reinterpret_cast<long long *>(w.xprv);
-  k.yprv = reinterpret_cast<long long *>(w.yprv);
-  k.zprv = reinterpret_cast<long long *>(w.zprv);
-  k.xcg = reinterpret_cast<long long *>(w.x_cg_temp);
-  k.ycg = reinterpret_cast<long long *>(w.y_cg_temp);
-  k.zcg = reinterpret_cast<long long *>(w.z_cg_temp);
-  k.total_cycles = mincon->total_cycles;
-  k.sd_cycles = mincon->steepest_descent_cycles;
-  k.dx0 = mincon->initial_step;
-  k.conv_rms = mincon->convergence_rms_force;
-  k.l_move = lm.v[0];
-  k.cycles_run = const_cast<stormm_synthesis *>(ag)->cycles_run.as<int>();
-  k.rms_force = const_cast<stormm_synthesis *>(ag)->rms_force.as<double>();
-  HIP_TRY(launch_fill(sc.acc, nullptr, nullptr, static_cast<size_t>(sc.nsys) * kScoreStride,
-                      st));
-  HIP_TRY(launch_minimize_persistent(precision == STORMM_PREC_DOUBLE ? 0 : 1, k, st));
-  return STORMM_OK;
+  return rc;
 }
 
 extern "C" int stormm_launch_minimization(int32_t precision, const stormm_synthesis *ag,

@@ -22,7 +22,7 @@ from test_oracle_minimizer import CG_CASE, bowl, textbook_cg
 pytestmark = pytest.mark.gpu
 
 MODES = {"double": (32, 40, 32), "single": (32, 23, 26)}
-MODE_NAMES = {0: "per-step launches", 1: "persistent", 2: "graph replay"}
+MODE_NAMES = {0: "per-step launches", 1: "fused cycles, whole-run graph", 2: "graph replay"}
 
 
 def gpu_minimize(topos, coords, prec, cycles, persistent, sd, dx0=0.01, clash=None,
@@ -90,13 +90,20 @@ def test_single_precision_conjugate_gradient_tracks_the_oracle(persistent):
     gpos, frc, nrg = MODES["single"]
     cycles, sd = 20, 5
     xf, e, _ = gpu_minimize(topos, coords, "single", cycles, persistent, sd, clash=0)
-    worst_x, worst_e = 0.0, 0.0
+    worst_x, worst_e, worst_here = 0.0, 0.0, 0.0
     for s, t in enumerate(topos):
         fixed = np.rint(np.asarray(coords[s]) * 2.0 ** gpos).astype(np.int64)
         out, trace = oracle.minimize(t, fixed, gpos, frc, nrg, cycles, sd, 0.01, dp_mode=False)
+        # Energy scale S = sum_k |E_k| at the device's end point: the total alone can nearly
+        # cancel, so errors are stated relative to S, as for single evaluations.
+        eo, _, _ = oracle.eval_mm(t, xf[s] * 2.0 ** -gpos, False, nrg)
+        scale = sum(abs(eo[k]) for k in oracle.SLOT_NAMES)
         worst_x = max(worst_x, np.abs(xf[s] - out).max() * 2.0 ** -gpos)
-        worst_e = max(worst_e, abs(e[s] - trace[-1]) / abs(trace[-1]))
+        worst_e = max(worst_e, abs(e[s] - trace[-1]) / scale)
+        # The reported energy is the FP32 evaluation at the end point (single-evaluation bar).
+        worst_here = max(worst_here, abs(e[s] - oracle.total_potential(eo)) / scale)
         assert e[s] < trace[0]
     print(f"[{MODE_NAMES[persistent]}] FP32 vs oracle after {cycles} cycles: coordinates "
-          f"{worst_x:.2e} A, energy {worst_e:.2e} relative")
-    assert worst_x < 2e-3 and worst_e < 1e-4
+          f"{worst_x:.2e} A, end energy {worst_e:.2e} of S (oracle trajectory), "
+          f"{worst_here:.2e} of S (oracle at the device's end point)")
+    assert worst_x < 2e-3 and worst_e < 1e-5 and worst_here < 1e-6

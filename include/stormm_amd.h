@@ -420,6 +420,20 @@ int stormm_launch_minimization_kit(int32_t precision, const stormm_synthesis *ag
                                    const stormm_scorecard_writer *scw,
                                    const stormm_linmin_writer *lmw, int32_t persistent,
                                    void *stream);
+/* Allocation-free minimization on caller-owned arrays.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under stream capture, tight serving
+ * loops).  Runs mode 1's fused-cycle kernels, queued one by one on `stream` without a graph and
+ * without waiting (the call returns once the launches are queued), with each system's cycle
+ * count, stop flag and RMS force in a static device array of 1 MiB per device that the library
+ * holds until the process ends: at most 65536 systems (more returns STORMM_ERR_CAPACITY).  The
+ * array starts every call cleared; calls that overlap in time on one device share it and race.
+ * Mode 1's limits apply: no restraints, at most 38 coordinate and 40 force bits
+ * (STORMM_ERR_STATE otherwise).  Results equal stormm_launch_minimization_kit in mode 1. */
+int stormm_launch_minimization_kit_noalloc(int32_t precision, const stormm_synthesis *ag,
+                                           const stormm_ps_writer *psw,
+                                           const stormm_minimize_controls *mincon,
+                                           const stormm_scorecard_writer *scw,
+                                           const stormm_linmin_writer *lmw, void *stream);
 /* PhaseSpaceSynthesis::initializeForces / primeConjugateGradientCalculation and
  * ScoreCard::initialize on caller-owned arrays. */
 int stormm_ps_writer_initialize_forces(const stormm_ps_writer *psw, int32_t padded_atoms,

@@ -161,6 +161,10 @@ _SIGS = {
     "stormm_launch_minimization_kit": (c_int32, [c_int32, vp, P(PsWriter),
                                                  P(MinimizeControls), P(ScoreCardWriter),
                                                  P(LinMinWriter), c_int32, vp]),
+    "stormm_launch_minimization_kit_noalloc": (c_int32, [c_int32, vp, P(PsWriter),
+                                                         P(MinimizeControls),
+                                                         P(ScoreCardWriter), P(LinMinWriter),
+                                                         vp]),
     "stormm_ps_writer_initialize_forces": (c_int32, [P(PsWriter), c_int32, vp]),
     "stormm_ps_writer_prime_conjugate_gradient": (c_int32, [P(PsWriter), c_int32, vp]),
     "stormm_scorecard_writer_initialize": (c_int32, [P(ScoreCardWriter), vp]),

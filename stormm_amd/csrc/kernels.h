@@ -65,6 +65,13 @@ struct CycleKit {
 hipError_t launch_min_cycle(int precision, const MMKit &k, const CycleKit &ck, bool forces,
                             int pending_move, hipStream_t stream);
 
+// Static per-device state of the allocation-free minimization (mm_minimize.hip): up to
+// kNoAllocMaxSystems systems, 16 bytes each (cycle count, stop flag, RMS force).
+constexpr int kNoAllocMaxSystems = 65536;
+constexpr size_t kNoAllocScratchBytes = static_cast<size_t>(kNoAllocMaxSystems) * 16;
+// Device address of the calling thread's current device's copy of that state.
+hipError_t noalloc_scratch(void **ptr);
+
 // Fresh per-system state of newly admitted pool slots: layout, zero cycle count and stop flag,
 // line-search state primed with dx0, zeroed ScoreCard row.
 struct SlotInitKit {

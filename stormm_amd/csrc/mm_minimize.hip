@@ -2,6 +2,7 @@
 // the reference's launch-per-step minimizer, plus the persistent minimizer that runs every cycle
 // of a work unit's systems inside one block.
 #include <algorithm>
+#include <mutex>
 
 #include "minimize_device.h"
 #include "mm_eval.h"
@@ -492,6 +493,40 @@ hipError_t launch_min_cycle(int precision, const MMKit &k, const CycleKit &ck, b
   }
   return clash ? launch_cycle_t<float, false, true>(k, ck, pending_move, stream) :
                  launch_cycle_t<float, false, false>(k, ck, pending_move, stream);
+}
+
+// ------------------------------------------------------------------------------------------
+// State of the allocation-free minimization entry point: per-system cycle counts (int), stop
+// flags (int) and RMS forces (double) for up to kNoAllocMaxSystems systems, 16 bytes each.  Each
+// device holds its own copy from the library's load until the process ends.  Host code must not
+// pass the array's name to a kernel: it stands for the host-side shadow, not the device memory.
+// noalloc_scratch() returns the device address.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch(void **ptr) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void *addr[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t err = hipGetDevice(&dev);
+  if (err != hipSuccess) {
+    return err;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    return hipErrorInvalidDevice;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr[dev] == nullptr) {
+    // Looked up once per device, with that device current.
+    void *p = nullptr;
+    err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (err != hipSuccess) {
+      return err;
+    }
+    addr[dev] = p;
+  }
+  *ptr = addr[dev];
+  return hipSuccess;
 }
 
 __global__ void __launch_bounds__(256) k_pool_init_slots(SlotInitKit k) {

@@ -1778,6 +1778,54 @@ extern "C" int stormm_launch_minimization_kit(int32_t precision, const stormm_sy
   return minimize(precision, ag, *psw, mincon, sc, lm, persistent, stream);
 }
 
+// The fused-cycle minimizer on caller-owned arrays with its per-system state in the library's
+// static device array: no device allocation, no graph, no stream creation and no
+// synchronization; the launches are queued on `stream` and the call returns.
+extern "C" int stormm_launch_minimization_kit_noalloc(int32_t precision,
+                                                      const stormm_synthesis *ag,
+                                                      const stormm_ps_writer *psw,
+                                                      const stormm_minimize_controls *mincon,
+                                                      const stormm_scorecard_writer *scw,
+                                                      const stormm_linmin_writer *lmw,
+                                                      void *stream) {
+  if (psw == nullptr || scw == nullptr || lmw == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null writer");
+  }
+  ScView sc;
+  LmView lm;
+  int rc = sc_view(scw, &sc);
+  if (rc == STORMM_OK) {
+    rc = lm_view(lmw, &lm);
+  }
+  if (rc == STORMM_OK) {
+    rc = check_writer(ag, psw, precision, &sc, true);
+  }
+  if (rc != STORMM_OK) {
+    return rc;
+  }
+  if (mincon == nullptr || sc.acc == nullptr || lm.nsys != ag->plan.nsys) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "minimization needs controls, a ScoreCard and a "
+                                             "matching LineMinimization record");
+  }
+  if ((rc = check_mincon(mincon)) != STORMM_OK) {
+    return rc;
+  }
+  if (ag->plan.nsys > kNoAllocMaxSystems) {
+    return fail(STORMM_ERR_CAPACITY, "the allocation-free minimization takes at most " +
+                                         std::to_string(kNoAllocMaxSystems) + " systems (" +
+                                         std::to_string(ag->plan.nsys) + " given)");
+  }
+  void *base = nullptr;
+  HIP_TRY(noalloc_scratch(&base));
+  CycleState cs;
+  cs.steps = static_cast<int *>(base);
+  cs.done = cs.steps + kNoAllocMaxSystems;
+  cs.rms = reinterpret_cast<double *>(static_cast<unsigned char *>(base) +
+                                      (static_cast<size_t>(kNoAllocMaxSystems) * 8));
+  return minimize_fused(precision, ag, *psw, mincon, sc, lm, cs,
+                        static_cast<hipStream_t>(stream), FusedIssue::kDirect);
+}
+
 extern "C" int stormm_check_ps_writer(const stormm_synthesis *ag, const stormm_ps_writer *psw) {
   if (ag == nullptr || psw == nullptr || psw->atom_starts == nullptr ||
       psw->atom_counts == nullptr) {

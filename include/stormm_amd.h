@@ -185,12 +185,20 @@ typedef struct {
 } stormm_ps_writer;
 
 /* ScoreCardWriter layout (src/Potential/scorecard.h:64-95): instantaneous accumulators, one row
- * of data_stride (32) int64 slots per system in StateVariable order, at nrg_bits. */
+ * of data_stride (32) int64 slots per system in StateVariable order, at nrg_bits, and the
+ * energy history: running sums and sums of squares of the committed samples (kcal/mol) and the
+ * time series, sample after sample, each sample one row per system.  A caller without history
+ * leaves the last five fields zero. */
 typedef struct {
   int32_t system_count;
   int32_t data_stride;
   int32_t nrg_bits;
   int64_t *instantaneous_accumulators;
+  int32_t sampled_step_count;      /* samples already in time_series                          */
+  int32_t sample_capacity;         /* samples time_series holds                               */
+  double *running_accumulators;    /* [system_count][32], or NULL                             */
+  double *squared_accumulators;    /* [system_count][32], NULL exactly when running is NULL   */
+  int64_t *time_series;            /* [sample_capacity][system_count][32], or NULL            */
 } stormm_scorecard_writer;
 
 /* LinMinWriter layout (src/MolecularMechanics/line_minimization.h:15-50): per-system line-search
@@ -222,6 +230,7 @@ typedef struct {
   int32_t clash_damping_cycles;    /* cdcyc                                                     */
   double clash_minimum_distance;   /* clash_r0 (A): Coulomb is softened inside this distance     */
   double clash_vdw_ratio;          /* clash_vdw_ratio: LJ is softened inside ratio * sigma_ij    */
+  int32_t diagnostic_print_frequency; /* ntpr: commit the ScoreCard every ntpr cycles (0: never) */
 } stormm_minimize_controls;
 
 const char *stormm_last_error(void);
@@ -273,9 +282,14 @@ int stormm_phasespace_upload_systems(stormm_phasespace *ps, int32_t lo, int32_t 
                                      const double *xyz);
 int stormm_phasespace_download_systems(const stormm_phasespace *ps, int32_t lo, int32_t hi,
                                        double *xyz, double *fxyz);
-/* One system: PhaseSpaceSynthesis::importSystem (phasespace_synthesis.h:917-923,
- * .cpp:3410-3436) written straight to the device, and exportSystem (phasespace_synthesis.h:698,
- * .cpp:2472-) read straight from it (coordinates and forces; xyz or fxyz may be NULL). */
+/* One system: PhaseSpaceSynthesis::importSystem with its default TrajectoryKind::POSITIONS
+ * (the raw-array and CoordinateFrame overloads, phasespace_synthesis.h:908-923, .tpp:9-) written
+ * straight to the device: positions only.  The system's forces, prior positions and
+ * conjugate-gradient history keep whatever they held, as in the reference, where forces and
+ * velocities are imported only by the PhaseSpace overloads (.cpp:3410-3436); an evaluation
+ * initialises forces itself and a minimization restarts its line search and gradient history.
+ * exportSystem (phasespace_synthesis.h:698, .cpp:2472-) reads straight from the device
+ * (coordinates and forces; xyz or fxyz may be NULL). */
 int stormm_phasespace_import_system(stormm_phasespace *ps, int32_t system_index,
                                     const double *xyz);
 int stormm_phasespace_export_system(const stormm_phasespace *ps, int32_t system_index,
@@ -303,9 +317,27 @@ int stormm_scorecard_download(const stormm_scorecard *sc, int64_t *accumulators)
 /* running sums and sums of squares of committed samples, [nsys][32] double each */
 int stormm_scorecard_download_history(const stormm_scorecard *sc, double *running,
                                       double *squared, int32_t *sample_count);
+/* The time series (ScoreCard::time_series_accumulators, scorecard.h:503; reportHistory,
+ * scorecard.cpp:1084-1134).  stormm_scorecard_commit is ScoreCard::commit followed by
+ * incrementSampleCount (scorecard.cpp:366-, 608): it appends one sample, every system's row
+ * with the potential energy sum in slot 28 and the total in slot 29, doubling the series'
+ * capacity when it is full.  reserve sizes the series for `capacity` samples (ScoreCard::reserve,
+ * scorecard.cpp:236-250); reset_sample_count keeps the first `count` samples (0 <= count <=
+ * samples; resetSampleCount, :613) and leaves the running sums alone; set_last_time_step labels
+ * the last sample (setLastTimeStep, :1142).  download_series copies samples [first, first +
+ * count) as [count][nsys][32] int64 at nrg_bits, and their time steps when time_steps is not
+ * NULL.  sample_count reports the samples held and the capacity. */
+int stormm_scorecard_reserve(stormm_scorecard *sc, int32_t capacity);
+int stormm_scorecard_reset_sample_count(stormm_scorecard *sc, int32_t count);
+int stormm_scorecard_set_last_time_step(stormm_scorecard *sc, int32_t time_step);
+int stormm_scorecard_sample_count(const stormm_scorecard *sc, int32_t *samples,
+                                  int32_t *capacity);
+int stormm_scorecard_download_series(const stormm_scorecard *sc, int32_t first, int32_t count,
+                                     int64_t *series, int32_t *time_steps);
 int64_t *stormm_scorecard_device_accumulators(stormm_scorecard *sc);
-/* The handle's accumulators in the ScoreCardWriter layout (ScoreCard::data(), scorecard.h:
- * 290-296). */
+/* The handle's accumulators and history in the ScoreCardWriter layout (ScoreCard::data(),
+ * scorecard.h:290-296); the time-series pointer holds until a commit or reserve grows the
+ * series. */
 int stormm_scorecard_data(stormm_scorecard *sc, stormm_scorecard_writer *w);
 
 /* ---- MolecularMechanicsControls ----------------------------------------------------------- */
@@ -370,7 +402,14 @@ int stormm_launch_line_advance(int32_t precision, const stormm_synthesis *ag,
  * (stormm_minimization_status).  Final energies land in the ScoreCard.  Modes 1, 3 and 4 take
  * no restraints and at most 38 coordinate and 40 force bits (STORMM_ERR_STATE otherwise); a
  * restrained synthesis is minimized in modes 0 and 2.  Mode 0 returns once its launches are
- * queued on the stream; the other modes return when the minimization is done. */
+ * queued on the stream; the other modes return when the minimization is done.
+ * With diagnostic_print_frequency = ntpr > 0 every mode commits the ScoreCard after the force
+ * evaluation of each cycle that is a multiple of ntpr (hpc_minimization.cu:292-300): the
+ * energies at the start of that cycle, appended to the time series and the running sums, with
+ * the cycle as the sample's time step; ceil(total_cycles / ntpr) samples per call.  A system
+ * that stops early repeats its last sample in the ones it did not reach.  The kit forms write
+ * the writer's time_series from sampled_step_count on (the caller advances its own count; a
+ * series too small returns STORMM_ERR_CAPACITY; none given: no samples). */
 int stormm_launch_minimization(int32_t precision, const stormm_synthesis *ag,
                                stormm_phasespace *ps, const stormm_minimize_controls *mincon,
                                stormm_scorecard *sc, stormm_line_min *lm, int32_t persistent,
@@ -448,6 +487,14 @@ int stormm_scorecard_writer_initialize(const stormm_scorecard_writer *scw, void 
 int stormm_time_mm_evaluation(int32_t precision, const stormm_synthesis *ag,
                               stormm_phasespace *ps, stormm_scorecard *sc, int32_t eval_force,
                               int32_t repeats, void *stream, double *mean_ms);
+/* The same for one choice of terms (the launches of stormm_launch_nonbonded,
+ * stormm_launch_valence or both fused): the benchmark's breakdown of where the time goes. */
+#define STORMM_TERMS_NONBONDED 1
+#define STORMM_TERMS_VALENCE 2
+#define STORMM_TERMS_ALL 3
+int stormm_time_launch(int32_t precision, const stormm_synthesis *ag, stormm_phasespace *ps,
+                       stormm_scorecard *sc, int32_t terms, int32_t eval_force,
+                       int32_t repeats, void *stream, double *mean_ms);
 
 /* ---- Serving pool ------------------------------------------------------------------------- */
 /* No counterpart in the reference, which minimizes one fixed batch per launchMinimization call
@@ -493,6 +540,16 @@ int stormm_pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *running, vo
  * (kcal/mol/A) and potential energy (kcal/mol) at the current coordinates. */
 int stormm_pool_status(const stormm_pool *pool, const int32_t *slots, int32_t count,
                        int32_t *steps, int32_t *done, double *rms, double *energy);
+/* Energy history of one slot, the per-system form of the ScoreCard time series (ScoreCard::
+ * reportHistory, src/Potential/scorecard.cpp:1090-1102): with minimize.
+ * diagnostic_print_frequency = ntpr > 0 the pool keeps ceil(total_cycles / ntpr) samples per
+ * slot, sample k being the energies at the start of the slot's cycle k * ntpr counted from its
+ * admission (plus the cycle at which the RMS force stopped it).  Writes the samples recorded so
+ * far to count and, for each, the potential energy (kcal/mol) to energy and the ScoreCard row
+ * (32 int64 at nrg_bits, potential energy in slot 28) to rows; either may be NULL.  More
+ * samples than `capacity` returns STORMM_ERR_CAPACITY; no history configured gives count 0. */
+int stormm_pool_history(const stormm_pool *pool, int32_t slot, int32_t capacity, int32_t *count,
+                        double *energy, int64_t *rows);
 /* Coordinates of one system, [N][3] in A and/or fixed precision (either pointer may be null). */
 int stormm_pool_coordinates(const stormm_pool *pool, int32_t slot, double *xyz,
                             int64_t *fixed_xyz);

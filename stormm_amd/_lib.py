@@ -78,7 +78,9 @@ class PsWriter(ctypes.Structure):
 
 class ScoreCardWriter(ctypes.Structure):
     _fields_ = [("system_count", c_int32), ("data_stride", c_int32), ("nrg_bits", c_int32),
-                ("instantaneous_accumulators", vp)]
+                ("instantaneous_accumulators", vp), ("sampled_step_count", c_int32),
+                ("sample_capacity", c_int32), ("running_accumulators", vp),
+                ("squared_accumulators", vp), ("time_series", vp)]
 
 
 class LinMinWriter(ctypes.Structure):
@@ -95,7 +97,7 @@ class MinimizeControls(ctypes.Structure):
     _fields_ = [("total_cycles", c_int32), ("steepest_descent_cycles", c_int32),
                 ("initial_step", c_double), ("convergence_rms_force", c_double),
                 ("clash_damping_cycles", c_int32), ("clash_minimum_distance", c_double),
-                ("clash_vdw_ratio", c_double)]
+                ("clash_vdw_ratio", c_double), ("diagnostic_print_frequency", c_int32)]
 
 
 class PoolConfig(ctypes.Structure):
@@ -118,6 +120,7 @@ _SIGS = {
     "stormm_pool_minimize": (c_int32, [vp, c_int32, P(c_int32), vp]),
     "stormm_pool_status": (c_int32, [vp, vp, c_int32, vp, vp, vp, vp]),
     "stormm_pool_coordinates": (c_int32, [vp, c_int32, vp, vp]),
+    "stormm_pool_history": (c_int32, [vp, c_int32, c_int32, P(c_int32), vp, vp]),
     "stormm_device_info": (c_int32, [P(c_int32), P(c_int32), ctypes.c_char_p, c_int32]),
     "stormm_device_synchronize": (c_int32, []),
     "stormm_synthesis_create": (c_int32, [P(TopologyBatch), P(LaunchShape), P(vp)]),
@@ -176,6 +179,11 @@ _SIGS = {
     "stormm_scorecard_commit": (c_int32, [vp, vp]),
     "stormm_scorecard_download": (c_int32, [vp, vp]),
     "stormm_scorecard_download_history": (c_int32, [vp, vp, vp, P(c_int32)]),
+    "stormm_scorecard_reserve": (c_int32, [vp, c_int32]),
+    "stormm_scorecard_reset_sample_count": (c_int32, [vp, c_int32]),
+    "stormm_scorecard_set_last_time_step": (c_int32, [vp, c_int32]),
+    "stormm_scorecard_sample_count": (c_int32, [vp, P(c_int32), P(c_int32)]),
+    "stormm_scorecard_download_series": (c_int32, [vp, c_int32, c_int32, vp, vp]),
     "stormm_scorecard_device_accumulators": (vp, [vp]),
     "stormm_mm_controls_create": (c_int32, [P(MinimizeControls), P(vp)]),
     "stormm_mm_controls_destroy": (c_int32, [vp]),
@@ -197,6 +205,8 @@ _SIGS = {
     "stormm_minimization_status": (c_int32, [vp, vp, vp]),
     "stormm_copy_bandwidth": (c_int32, [c_int64, c_int32, P(c_double)]),
     "stormm_fma_throughput": (c_int32, [c_int32, P(c_double)]),
+    "stormm_time_launch": (c_int32, [c_int32, vp, vp, vp, c_int32, c_int32, c_int32, vp,
+                                     P(c_double)]),
     "stormm_time_mm_evaluation": (c_int32, [c_int32, vp, vp, vp, c_int32, c_int32, vp,
                                             P(c_double)]),
 }

@@ -20,10 +20,18 @@ constexpr int kGlobalPad = 32;
 constexpr int kMaxWuAtoms = 512;
 // Largest number of systems packed into one work unit.
 constexpr int kMaxWuSystems = 32;
+// Largest number of tile tasks of one work unit: its systems take at most kMaxWuAtoms / kTile
+// atom blocks between them, and b blocks hold b (b + 1) / 2 tiles.
+constexpr int kMaxWuTasks = (kMaxWuAtoms / kTile) * ((kMaxWuAtoms / kTile) + 1) / 2;
 // Energy components tracked per system inside a work unit (StateVariable slots 0..12).
 constexpr int kNrgSlots = 13;
 // ScoreCard stride (slots per system), src/Potential/scorecard.cpp:66 with 32-wide warps.
 constexpr int kScoreStride = 32;
+// StateVariable slots of the kinetic energy and of the potential and total energy sums
+// (src/Potential/energy_enumerators.h:36-78).
+constexpr int kKineticSlot = 14;
+constexpr int kPotentialSlot = 28;
+constexpr int kTotalSlot = 29;
 // Fixed-precision scales of the LDS accumulators for the Born-radius integral and dE/dR.
 constexpr int kPsiBits = 40;
 constexpr int kDedrBits = 32;

@@ -10,8 +10,26 @@ hipError_t launch_mm_eval(int precision, const MMKit &k, int terms, int eval_for
 hipError_t launch_fill(long long *a, long long *b, long long *c, size_t n, hipStream_t stream);
 // Zeroes three int arrays of n entries (overflow words).
 hipError_t launch_fill_int(int *a, int *b, int *c, size_t n, hipStream_t stream);
+// ScoreCard::commit: running sums and sums of squares, and (series non-null) the time-series
+// row of this sample, series[s][32] for each system s.
 hipError_t launch_scorecard_commit(const long long *inst, double *running, double *squared,
-                                   int nsys, double inv_scale, hipStream_t stream);
+                                   long long *series, int nsys, double inv_scale,
+                                   hipStream_t stream);
+
+// Energy snapshots of a minimization, taken by the conjugate-gradient kernel from the ScoreCard
+// rows of the evaluation that precedes it, every `every` cycles (ScoreCard::commit at the
+// reference's diagnostic print frequency).  The snapshot of system s at cycle c goes to sample
+// first + c / every, at rows + sample * sample_stride + s * sys_stride, and is added to the
+// running sums (when present).  rows == null: no snapshots.
+struct HistKit {
+  long long *rows;
+  double *running, *squared;
+  const long long *scorecard;   // instantaneous rows, 32 slots per system
+  long long sample_stride, sys_stride;
+  int every, first, capacity;   // capacity: samples the rows hold
+  int max_cycle;                // no snapshot at or past this cycle (0: no limit)
+  double inv_nrg_scale;
+};
 
 // Minimizer pieces (mm_minimize.hip).
 struct CgKit {
@@ -33,6 +51,7 @@ struct CgKit {
   int *xcg_ovrf, *ycg_ovrf, *zcg_ovrf;
   double inv_frc_scale;
   int dp_mode;
+  HistKit hist;
 };
 struct LineKit {
   int nsys;
@@ -51,6 +70,10 @@ struct LineKit {
 };
 hipError_t launch_conjugate_gradient(const CgKit &k, int step, int sd_cycles,
                                      hipStream_t stream);
+// After a minimization of `cycles` cycles with per-system cycle counts `steps`: the samples of
+// a system that stopped early repeat its last snapshot up to sample first + count - 1.
+hipError_t launch_history_fill(const HistKit &h, const int *steps, int nsys, int count,
+                               hipStream_t stream);
 hipError_t launch_line_advance(const LineKit &k, int move_number, hipStream_t stream);
 
 // Fused minimization cycle (mm_minimize.hip): evaluations that make the pending line move of

@@ -253,6 +253,7 @@ template <> struct EvalBlockLimit<double> {
 // ------------------------------------------------------------------------------------------
 template <typename T> struct WuLds {
   using crd_t = typename Prec<T>::crd_t;
+  int4 *tasks;              // the unit's tile tasks, placed at their systems' LDS bases
   long long *psi, *dedr, *fx, *fy, *fz, *nrg, *org;
   double *move;             // per system: pending line move (coordinate units per direction unit)
   T *q, *p1, *p2, *rho, *irho, *scr, *rad, *irad, *sd;
@@ -267,7 +268,8 @@ template <typename T> struct WuLds {
 
   static __host__ __device__ size_t bytes(int maxa, int waves) {
     const size_t m = static_cast<size_t>(maxa);
-    return (5 * m * sizeof(long long)) + (kMaxWuSystems * 16 * sizeof(long long)) +
+    return (kMaxWuTasks * sizeof(int4)) +
+           (5 * m * sizeof(long long)) + (kMaxWuSystems * 16 * sizeof(long long)) +
            (3 * kMaxWuSystems * sizeof(long long)) + (kMaxWuSystems * sizeof(double)) +
            (9 * m * sizeof(T)) +
            (3 * m * sizeof(crd_t)) + (2 * m * sizeof(int)) + (kMaxWuSystems * sizeof(WuSys)) +
@@ -284,6 +286,7 @@ template <typename T> struct WuLds {
       return r;
     };
     const size_t m = static_cast<size_t>(maxa);
+    tasks = reinterpret_cast<int4 *>(take(kMaxWuTasks * sizeof(int4)));
     psi = reinterpret_cast<long long *>(take(m * sizeof(long long)));
     dedr = reinterpret_cast<long long *>(take(m * sizeof(long long)));
     fx = reinterpret_cast<long long *>(take(m * sizeof(long long)));

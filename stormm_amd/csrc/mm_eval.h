@@ -31,6 +31,22 @@ template <typename T> struct StageCtx {
   __device__ StageCtx(unsigned char *smem, int maxa, int waves) : L(smem, maxa, waves) {}
 };
 
+// Slot s of the unit's system that holds item t, given ascending per-slot starts pre[0..n]
+// (pre[0] = 0 <= t < pre[n]): the s with pre[s] <= t < pre[s + 1].
+__device__ __forceinline__ int find_slot(const int *pre, int n, int t) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= t) {
+      lo = mid;
+    }
+    else {
+      hi = mid;
+    }
+  }
+  return lo;
+}
+
 // ------------------------------------------------------------------------------------------
 // Work-unit set-up: system table, per-atom parameters, and the coordinate origins.
 template <typename T>
@@ -54,6 +70,26 @@ __device__ __forceinline__ void stage_load_params(StageCtx<T> &c, const MMKit &k
     L.opre[wu.sys_count] = o;
     L.dpre[wu.sys_count] = d;
     L.vpre[wu.sys_count] = v;
+  }
+  __syncthreads();
+  // The unit's tile tasks, read once per evaluation: off-diagonal tasks of every system, then
+  // the diagonal ones, each with its system's LDS base added and its slot in the low bits of y.
+  for (int t = c.tid; t < wu.task_count; t += c.nthr) {
+    int s, idx;
+    if (t < wu.offd_count) {
+      s = find_slot(L.opre, wu.sys_count, t);
+      idx = L.sys[s].task_first + (t - L.opre[s]);
+    }
+    else {
+      const int td = t - wu.offd_count;
+      s = find_slot(L.dpre, wu.sys_count, td);
+      idx = L.sys[s].task_first + L.sys[s].offd_count + (td - L.dpre[s]);
+    }
+    int4 tk = k.tasks[idx];
+    const int base = L.sys[s].lds_base;
+    tk.x += base | (base << 16);
+    tk.y |= s;
+    L.tasks[t] = tk;
   }
   for (int a = c.tid; a < wu.lds_atoms; a += c.nthr) {
     int s = 0;
@@ -321,41 +357,12 @@ __device__ __forceinline__ bool step_bit(unsigned mask, int ks) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Slot s of the unit's system that holds item t, given ascending per-slot starts pre[0..n]
-// (pre[0] = 0 <= t < pre[n]): the s with pre[s] <= t < pre[s + 1].
-__device__ __forceinline__ int find_slot(const int *pre, int n, int t) {
-  int lo = 0, hi = n;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (pre[mid] <= t) {
-      lo = mid;
-    }
-    else {
-      hi = mid;
-    }
-  }
-  return lo;
-}
 
-// Tile task t of the work unit, placed at its system's LDS base and tagged with its slot.
+// Tile task t of the work unit, placed at its system's LDS base and tagged with its slot
+// (staged in LDS by stage_load_params).
 template <typename T>
-__device__ __forceinline__ Task fetch_task(const StageCtx<T> &c, const MMKit &k, int t) {
-  const WuLds<T> &L = c.L;
-  int s, idx;
-  if (t < c.wu.offd_count) {
-    s = find_slot(L.opre, c.wu.sys_count, t);
-    idx = L.sys[s].task_first + (t - L.opre[s]);
-  }
-  else {
-    const int td = t - c.wu.offd_count;
-    s = find_slot(L.dpre, c.wu.sys_count, td);
-    idx = L.sys[s].task_first + L.sys[s].offd_count + (td - L.dpre[s]);
-  }
-  Task tk = k.tasks[idx];
-  const int base = L.sys[s].lds_base;
-  tk.x += base | (base << 16);
-  tk.y |= s;
-  return tk;
+__device__ __forceinline__ Task fetch_task(const StageCtx<T> &c, const MMKit &, int t) {
+  return c.L.tasks[t];
 }
 
 // Task dealing for the Born-integral stages: pass tb gives the four 16-lane rows of a

@@ -105,34 +105,42 @@ hipError_t launch_fill_int(int *a, int *b, int *c, size_t n, hipStream_t stream)
   return hipGetLastError();
 }
 
-// ScoreCard::commit (kScoreCardCommit, src/Potential/hpc_scorecard.cu:72): fold the
-// instantaneous accumulators into running sums and sums of squares, and store the potential
-// energy total in its slot.
+// ScoreCard::commit (kScoreCardCommit, src/Potential/hpc_scorecard.cu:170-215) with the
+// potential and total energy sums of kScoreCardSumEnergy (:72-157): fold the instantaneous
+// accumulators into running sums and sums of squares and, when series is given, store them as
+// this sample's time-series row, the potential energy (BOND..RESTRAINT) in slot 28 and the total
+// (potential + KINETIC, slot 14) in slot 29.
 __global__ void k_scorecard_commit(const long long *inst, double *running, double *squared,
-                                   int nsys, double inv_scale) {
+                                   long long *series, int nsys, double inv_scale) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsys) {
     return;
   }
+  const long long *row = inst + static_cast<size_t>(s) * kScoreStride;
   long long pe = 0;
-  for (int k = 0; k <= 12; k++) {
-    pe += inst[static_cast<size_t>(s) * kScoreStride + k];
+  for (int k = 0; k < kNrgSlots; k++) {
+    pe += row[k];
   }
   for (int k = 0; k < kScoreStride; k++) {
-    const long long v = (k == 28) ? pe : inst[static_cast<size_t>(s) * kScoreStride + k];
+    const long long v = (k == kPotentialSlot) ? pe : (k == kTotalSlot) ? pe + row[kKineticSlot] :
+                                                                         row[k];
     const double d = static_cast<double>(v) * inv_scale;
     running[static_cast<size_t>(s) * kScoreStride + k] += d;
     squared[static_cast<size_t>(s) * kScoreStride + k] += d * d;
+    if (series != nullptr) {
+      series[static_cast<size_t>(s) * kScoreStride + k] = v;
+    }
   }
 }
 
 hipError_t launch_scorecard_commit(const long long *inst, double *running, double *squared,
-                                   int nsys, double inv_scale, hipStream_t stream) {
+                                   long long *series, int nsys, double inv_scale,
+                                   hipStream_t stream) {
   if (nsys == 0) {
     return hipSuccess;
   }
   hipLaunchKernelGGL(k_scorecard_commit, dim3((nsys + 127) / 128), dim3(128), 0, stream, inst,
-                     running, squared, nsys, inv_scale);
+                     running, squared, series, nsys, inv_scale);
   return hipGetLastError();
 }
 

@@ -131,12 +131,53 @@ __device__ void cg_wide(const CgKit &k, int s, int lane, int start, int n, int s
   }
 }
 
+// Sample `sample` of system s in the energy history: its ScoreCard row with the potential
+// energy sum in slot 28 and the total in slot 29, added to the running sums when they are kept
+// (ScoreCard::commit, src/Potential/scorecard.cpp:366-396, and its device form, src/Potential/
+// hpc_scorecard.cu:170-215).  Lanes 0..31 take one slot each.
+__device__ __forceinline__ void write_snapshot(const HistKit &h, int s, int sample,
+                                               const long long *row, int lane) {
+  if (lane >= kScoreStride) {
+    return;
+  }
+  long long pe = 0;
+  for (int k = 0; k < kNrgSlots; k++) {
+    pe += row[k];
+  }
+  const long long v = (lane == kPotentialSlot) ? pe :
+                      (lane == kTotalSlot) ? pe + row[kKineticSlot] : row[lane];
+  h.rows[static_cast<long long>(sample) * h.sample_stride +
+         static_cast<long long>(s) * h.sys_stride + lane] = v;
+  if (h.running != nullptr) {
+    const double d = static_cast<double>(v) * h.inv_nrg_scale;
+    const size_t at = static_cast<size_t>(s) * kScoreStride + lane;
+    h.running[at] += d;
+    h.squared[at] += d * d;
+  }
+}
+
+// The snapshot of cycle `cycle` (0-based), taken when the cycle is a multiple of the history's
+// interval (src/MolecularMechanics/hpc_minimization.cu:292-300).
+__device__ __forceinline__ void record_snapshot(const HistKit &h, int s, int cycle, int lane) {
+  if (h.rows == nullptr || (cycle % h.every) != 0 ||
+      (h.max_cycle > 0 && cycle >= h.max_cycle)) {
+    return;
+  }
+  const int sample = h.first + (cycle / h.every);
+  if (sample < h.capacity) {
+    write_snapshot(h, s, sample, h.scorecard + static_cast<size_t>(s) * kScoreStride, lane);
+  }
+}
+
 __global__ void __launch_bounds__(64) k_conjugate_gradient(CgKit k, int step, int sd_cycles) {
   const int s = (k.slots != nullptr) ? k.slots[blockIdx.x] : static_cast<int>(blockIdx.x);
   const int lane = threadIdx.x;
   if (k.done != nullptr && k.done[s] != 0) {
     return;
   }
+  // The evaluation before this kernel left the energies at the start of the cycle in the
+  // ScoreCard: cycles counted on the device, or `step` = cycle + 1 from the host.
+  record_snapshot(k.hist, s, (k.steps != nullptr) ? k.steps[s] : step - 1, lane);
   const int start = k.sys_start[s];
   const int n = k.sys_natom[s];
   if (k.xfrc_ovrf != nullptr) {
@@ -271,6 +312,30 @@ hipError_t launch_conjugate_gradient(const CgKit &k, int step, int sd_cycles,
   }
   hipLaunchKernelGGL(k_conjugate_gradient, dim3(k.nsys), dim3(64), 0, stream, k, step,
                      sd_cycles);
+  return hipGetLastError();
+}
+
+// Samples first + steps[s] / every + 1 .. first + count - 1 of system s (the ones it did not
+// reach after stopping) repeat its last snapshot.
+__global__ void __launch_bounds__(64) k_history_fill(HistKit h, const int *steps, int count) {
+  const int s = blockIdx.x;
+  const int last = h.first + steps[s] / h.every;
+  const int end = min(h.first + count, h.capacity);
+  for (int j = last + 1; j < end; j++) {
+    const long long *row = h.rows + static_cast<long long>(last) * h.sample_stride +
+                           static_cast<long long>(s) * h.sys_stride;
+    // Slots 28 and 29 of the row already hold the sums, which write_snapshot recomputes from
+    // slots 0..12 and 14 unchanged.
+    write_snapshot(h, s, j, row, threadIdx.x);
+  }
+}
+
+hipError_t launch_history_fill(const HistKit &h, const int *steps, int nsys, int count,
+                               hipStream_t stream) {
+  if (nsys == 0 || h.rows == nullptr || count <= 0) {
+    return hipSuccess;
+  }
+  hipLaunchKernelGGL(k_history_fill, dim3(nsys), dim3(64), 0, stream, h, steps, count);
   return hipGetLastError();
 }
 

@@ -944,6 +944,10 @@ int build_plan(const stormm_topology_batch *b, const stormm_launch_shape *shape_
     P.maxa = std::max(P.maxa, append_work_unit(bins[u], bin_mode[u], sys_index, P.atom_starts,
                                                P.atom_counts, P.sys_template, P.ts, P.wu,
                                                P.wu_sys, &P.rst_first, &P.rst_count));
+    if (P.wu.back().task_count > kMaxWuTasks) {
+      return fail(STORMM_ERR_CAPACITY, "a work unit holds more than " +
+                                           std::to_string(kMaxWuTasks) + " tile tasks");
+    }
   }
   return STORMM_OK;
 }

@@ -85,6 +85,10 @@ struct stormm_pool {
   DevMem xcrd, ycrd, zcrd, xfrc, yfrc, zfrc, xprv, yprv, zprv, xcg, ycg, zcg;
   DevMem q, ljp1, ljp2, rho, scr, pbr, alpha, beta, gamma, lj_type;
   DevMem starts, natoms, steps, done, rms, line, scorecard, status;
+  // Energy history per slot: hist_len ScoreCard rows, one every cfg.minimize.
+  // diagnostic_print_frequency cycles (none when that is 0).
+  int hist_len = 0;
+  DevMem hist;
   DevMem tasks, masks, vterms, bond_par, angl_par, dihe_par, dihe14, ljab, ljab14;
   size_t up_tasks = 0, up_masks = 0, up_vterms = 0, up_bond = 0, up_angl = 0, up_dihe = 0,
          up_d14 = 0, up_ljab = 0, up_ljab14 = 0;
@@ -105,10 +109,20 @@ static int check_config(const stormm_pool_config *c) {
   if (c->minimize.total_cycles < 0 || c->minimize.steepest_descent_cycles < 0 ||
       !(c->minimize.initial_step > 0.0) || c->minimize.clash_damping_cycles < 0 ||
       c->minimize.clash_damping_cycles > c->minimize.total_cycles ||
-      c->minimize.clash_minimum_distance < 0.0 || c->minimize.clash_vdw_ratio < 0.0) {
+      c->minimize.clash_minimum_distance < 0.0 || c->minimize.clash_vdw_ratio < 0.0 ||
+      c->minimize.diagnostic_print_frequency < 0) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "invalid minimization controls");
   }
+  if (c->minimize.diagnostic_print_frequency > 0 && c->minimize.total_cycles == 0) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "an energy history needs a cycle limit");
+  }
   return STORMM_OK;
+}
+
+// Samples of each slot's energy history.
+static int history_length(const stormm_pool_config *c) {
+  const int every = c->minimize.diagnostic_print_frequency;
+  return (every > 0) ? (c->minimize.total_cycles + every - 1) / every : 0;
 }
 
 extern "C" int stormm_pool_create(const stormm_pool_config *cfg, stormm_pool **out) {
@@ -132,7 +146,10 @@ extern "C" int stormm_pool_create(const stormm_pool_config *cfg, stormm_pool **o
     return rc;
   }
   const size_t budget = static_cast<size_t>(cfg->hbm_budget_bytes);
-  const size_t slot_bytes = static_cast<size_t>(cfg->max_systems) * kPoolBytesPerSlot;
+  p->hist_len = history_length(cfg);
+  const size_t hist_bytes = static_cast<size_t>(p->hist_len) * kScoreStride * sizeof(long long);
+  const size_t slot_bytes = static_cast<size_t>(cfg->max_systems) *
+                            (kPoolBytesPerSlot + hist_bytes);
   p->template_budget = static_cast<size_t>(kTemplateShare * static_cast<double>(budget));
   if (budget <= slot_bytes + p->template_budget) {
     return fail(STORMM_ERR_CAPACITY, "budget too small for the requested system slots");
@@ -163,6 +180,9 @@ extern "C" int stormm_pool_create(const stormm_pool_config *cfg, stormm_pool **o
   HIP_TRY(p->line.alloc(9 * ns * sizeof(double)));
   HIP_TRY(p->scorecard.alloc(ns * kScoreStride * sizeof(long long)));
   HIP_TRY(hipMemset(p->scorecard.ptr, 0, ns * kScoreStride * sizeof(long long)));
+  if (p->hist_len > 0) {
+    HIP_TRY(p->hist.alloc(ns * hist_bytes));
+  }
   HIP_TRY(p->status.alloc(sizeof(int)));
   HIP_TRY(hipMemset(p->status.ptr, 0, sizeof(int)));
   HIP_TRY(p->active.alloc(ns * sizeof(int)));
@@ -679,6 +699,10 @@ static int pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *running, vo
   for (size_t u = 0; u < bins.size(); u++) {
     maxa = std::max(maxa, append_work_unit(bins[u], modes[u], index, start, natom, tmpl, p->ts,
                                            wu, wu_sys));
+    if (wu.back().task_count > kMaxWuTasks) {
+      return fail(STORMM_ERR_CAPACITY, "a work unit holds more than " +
+                                           std::to_string(kMaxWuTasks) + " tile tasks");
+    }
   }
   const size_t wu_bytes = wu.size() * sizeof(WuDesc);
   const size_t wu_sys_bytes = wu_sys.size() * sizeof(WuSys);
@@ -738,6 +762,20 @@ static int pool_minimize(stormm_pool *pool, int32_t cycles, int32_t *running, vo
   cg.zcg = p->zcg.as<long long>();
   cg.inv_frc_scale = std::ldexp(1.0, -p->cfg.frc_bits);
   cg.dp_mode = (prec == 0) ? 1 : 0;
+  if (p->hist_len > 0) {
+    // Each slot's sample k is its energy at the start of cycle k * every, counted from its
+    // admission.
+    HistKit &h = cg.hist;
+    h.rows = p->hist.as<long long>();
+    h.scorecard = mm.scorecard;
+    h.sample_stride = kScoreStride;
+    h.sys_stride = static_cast<long long>(p->hist_len) * kScoreStride;
+    h.every = p->cfg.minimize.diagnostic_print_frequency;
+    h.first = 0;
+    h.capacity = p->hist_len;
+    h.max_cycle = max_steps;
+    h.inv_nrg_scale = std::ldexp(1.0, -p->cfg.nrg_bits);
+  }
   LineKit lk;
   std::memset(&lk, 0, sizeof(lk));
   lk.nsys = na;
@@ -851,6 +889,55 @@ extern "C" int stormm_pool_status(const stormm_pool *pool, const int32_t *slots,
       energy[i] = static_cast<double>(acc) * inv_nrg;
     }
   }
+  return STORMM_OK;
+}
+
+extern "C" int stormm_pool_history(const stormm_pool *pool, int32_t slot, int32_t capacity,
+                                   int32_t *count, double *energy, int64_t *rows) {
+  if (pool == nullptr || count == nullptr || capacity < 0 || slot < 0 ||
+      slot >= pool->cap_slots || !pool->slot_live[slot]) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "invalid history request");
+  }
+  const stormm_pool *p = pool;
+  *count = 0;
+  if (p->hist_len == 0) {
+    return STORMM_OK;
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  int steps = 0, done = 0;
+  HIP_TRY(hipMemcpy(&steps, p->steps.as<int>() + slot, sizeof(int), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(&done, p->done.as<int>() + slot, sizeof(int), hipMemcpyDeviceToHost));
+  // Snapshots of the cycles run (0, every, ... below steps) and of the cycle at which the RMS
+  // force stopped the system.
+  const int every = p->cfg.minimize.diagnostic_print_frequency;
+  const int max_steps = p->cfg.minimize.total_cycles;
+  int n = (steps + every - 1) / every;
+  if (done != 0 && steps % every == 0 && steps < max_steps) {
+    n += 1;
+  }
+  n = std::min(n, p->hist_len);
+  if (n > capacity) {
+    return fail(STORMM_ERR_CAPACITY, "slot " + std::to_string(slot) + " holds " +
+                                         std::to_string(n) + " samples; room for " +
+                                         std::to_string(capacity) + " given");
+  }
+  std::vector<long long> h(static_cast<size_t>(n) * kScoreStride);
+  const size_t first = static_cast<size_t>(slot) * p->hist_len * kScoreStride;
+  HIP_TRY(hipMemcpy(h.data(), p->hist.as<long long>() + first, h.size() * sizeof(long long),
+                    hipMemcpyDeviceToHost));
+  const double inv_nrg = std::ldexp(1.0, -p->cfg.nrg_bits);
+  for (int i = 0; i < n; i++) {
+    if (energy != nullptr) {
+      energy[i] = static_cast<double>(h[static_cast<size_t>(i) * kScoreStride + kPotentialSlot]) *
+                  inv_nrg;
+    }
+    if (rows != nullptr) {
+      std::memcpy(rows + static_cast<size_t>(i) * kScoreStride,
+                  h.data() + static_cast<size_t>(i) * kScoreStride,
+                  kScoreStride * sizeof(long long));
+    }
+  }
+  *count = n;
   return STORMM_OK;
 }
 

@@ -61,7 +61,11 @@ struct stormm_phasespace {
 
 struct stormm_scorecard {
   int nsys = 0, nrg_bits = 26, samples = 0;
+  // Samples the time series holds (the reference's default of 16 until it is first used).
+  int capacity = 16;
   DevMem inst, running, squared;
+  DevMem series;               // [capacity][nsys][32] int64, allocated on first use
+  std::vector<int> time_steps; // one per sample
 };
 
 struct stormm_mm_controls {
@@ -699,15 +703,110 @@ extern "C" int stormm_scorecard_initialize(stormm_scorecard *sc, void *stream) {
   return STORMM_OK;
 }
 
+// Sizes the time series for at least `need` samples and allocates it if it is not yet; the
+// samples held are kept (their copy waits for the device to finish its queued work).
+static int scorecard_series(stormm_scorecard *sc, int need) {
+  if (sc->series.ptr != nullptr && need <= sc->capacity) {
+    return STORMM_OK;
+  }
+  const int cap = std::max(need, sc->capacity);
+  const size_t row = static_cast<size_t>(sc->nsys) * kScoreStride * sizeof(long long);
+  DevMem fresh;
+  HIP_TRY(fresh.alloc(row * static_cast<size_t>(cap)));
+  HIP_TRY(hipMemset(fresh.ptr, 0, fresh.bytes));
+  if (sc->series.ptr != nullptr && sc->samples > 0) {
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(fresh.ptr, sc->series.ptr, row * static_cast<size_t>(sc->samples),
+                      hipMemcpyDeviceToDevice));
+  }
+  std::swap(fresh.ptr, sc->series.ptr);
+  std::swap(fresh.bytes, sc->series.bytes);
+  sc->capacity = cap;
+  return STORMM_OK;
+}
+
 extern "C" int stormm_scorecard_commit(stormm_scorecard *sc, void *stream) {
   if (sc == nullptr) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
   }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // A full series doubles, as ScoreCard::commit does (src/Potential/scorecard.cpp:368-372).
+  int rc = scorecard_series(sc, (sc->samples < sc->capacity) ? sc->capacity : 2 * sc->capacity);
+  if (rc != STORMM_OK) {
+    return rc;
+  }
+  long long *row = sc->series.as<long long>() +
+                   static_cast<size_t>(sc->samples) * sc->nsys * kScoreStride;
   HIP_TRY(launch_scorecard_commit(sc->inst.as<long long>(), sc->running.as<double>(),
-                                  sc->squared.as<double>(), sc->nsys,
-                                  std::ldexp(1.0, -sc->nrg_bits),
-                                  static_cast<hipStream_t>(stream)));
+                                  sc->squared.as<double>(), row, sc->nsys,
+                                  std::ldexp(1.0, -sc->nrg_bits), st));
   sc->samples += 1;
+  sc->time_steps.push_back(0);
+  return STORMM_OK;
+}
+
+extern "C" int stormm_scorecard_reserve(stormm_scorecard *sc, int32_t capacity) {
+  if (sc == nullptr || capacity < 0) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "invalid reserve request");
+  }
+  return scorecard_series(sc, std::max(capacity, sc->samples));
+}
+
+extern "C" int stormm_scorecard_reset_sample_count(stormm_scorecard *sc, int32_t count) {
+  if (sc == nullptr || count < 0 || count > sc->samples) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "the sample count can be reset to a value between 0 "
+                                             "and the samples held");
+  }
+  sc->samples = count;
+  sc->time_steps.resize(static_cast<size_t>(count));
+  return STORMM_OK;
+}
+
+extern "C" int stormm_scorecard_set_last_time_step(stormm_scorecard *sc, int32_t time_step) {
+  if (sc == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  if (sc->samples <= 0) {
+    return fail(STORMM_ERR_STATE, "no energies have been committed to the history");
+  }
+  sc->time_steps[static_cast<size_t>(sc->samples) - 1] = time_step;
+  return STORMM_OK;
+}
+
+extern "C" int stormm_scorecard_sample_count(const stormm_scorecard *sc, int32_t *samples,
+                                             int32_t *capacity) {
+  if (sc == nullptr) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  if (samples != nullptr) {
+    *samples = sc->samples;
+  }
+  if (capacity != nullptr) {
+    *capacity = sc->capacity;
+  }
+  return STORMM_OK;
+}
+
+extern "C" int stormm_scorecard_download_series(const stormm_scorecard *sc, int32_t first,
+                                                int32_t count, int64_t *series,
+                                                int32_t *time_steps) {
+  if (sc == nullptr || first < 0 || count < 0 || first + count > sc->samples ||
+      (count > 0 && series == nullptr)) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "the series request must lie within the samples "
+                                             "held");
+  }
+  if (count == 0) {
+    return STORMM_OK;
+  }
+  const size_t row = static_cast<size_t>(sc->nsys) * kScoreStride;
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(series, sc->series.as<long long>() + row * first,
+                    row * count * sizeof(long long), hipMemcpyDeviceToHost));
+  if (time_steps != nullptr) {
+    for (int i = 0; i < count; i++) {
+      time_steps[i] = sc->time_steps[static_cast<size_t>(first + i)];
+    }
+  }
   return STORMM_OK;
 }
 
@@ -746,10 +845,19 @@ extern "C" int stormm_scorecard_data(stormm_scorecard *sc, stormm_scorecard_writ
   if (sc == nullptr || w == nullptr) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "null argument");
   }
+  int rc = scorecard_series(sc, sc->capacity);
+  if (rc != STORMM_OK) {
+    return rc;
+  }
   w->system_count = sc->nsys;
   w->data_stride = kScoreStride;
   w->nrg_bits = sc->nrg_bits;
   w->instantaneous_accumulators = sc->inst.as<int64_t>();
+  w->sampled_step_count = sc->samples;
+  w->sample_capacity = sc->capacity;
+  w->running_accumulators = sc->running.as<double>();
+  w->squared_accumulators = sc->squared.as<double>();
+  w->time_series = sc->series.as<int64_t>();
   return STORMM_OK;
 }
 
@@ -872,6 +980,13 @@ extern "C" int stormm_line_min_download(const stormm_line_min *lm, double *value
 struct ScView {
   int nsys = 0, nrg_bits = 26;
   long long *acc = nullptr;
+  // Energy history: the time series, `samples` of its `capacity` samples used, and the running
+  // sums (null series: no history).
+  long long *series = nullptr;
+  double *running = nullptr, *squared = nullptr;
+  int samples = 0, capacity = 0;
+  stormm_scorecard *owner = nullptr;  // the handle, when the view is of one
+  HistKit hist = {};                  // a minimization's energy snapshots (null rows: none)
 };
 struct LmView {
   int nsys = 0;
@@ -884,6 +999,12 @@ static ScView sc_view(stormm_scorecard *sc) {
     v.nsys = sc->nsys;
     v.nrg_bits = sc->nrg_bits;
     v.acc = sc->inst.as<long long>();
+    v.series = sc->series.as<long long>();
+    v.running = sc->running.as<double>();
+    v.squared = sc->squared.as<double>();
+    v.samples = sc->samples;
+    v.capacity = sc->capacity;
+    v.owner = sc;
   }
   return v;
 }
@@ -897,9 +1018,87 @@ static int sc_view(const stormm_scorecard_writer *w, ScView *out) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "ScoreCard writer needs accumulators, a data "
                                              "stride of 32 and energy bits in [22, 40]");
   }
+  if ((w->running_accumulators == nullptr) != (w->squared_accumulators == nullptr) ||
+      (w->time_series != nullptr &&
+       (w->sampled_step_count < 0 || w->sample_capacity < w->sampled_step_count))) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "ScoreCard writer history needs both running-sum "
+                                             "arrays or neither, and 0 <= sampled_step_count "
+                                             "<= sample_capacity");
+  }
   out->nsys = w->system_count;
   out->nrg_bits = w->nrg_bits;
   out->acc = reinterpret_cast<long long *>(w->instantaneous_accumulators);
+  out->series = reinterpret_cast<long long *>(w->time_series);
+  out->running = w->running_accumulators;
+  out->squared = w->squared_accumulators;
+  out->samples = w->sampled_step_count;
+  out->capacity = w->sample_capacity;
+  return STORMM_OK;
+}
+
+// Samples a minimization of mc->total_cycles cycles commits at its diagnostic interval.
+static int history_samples(const stormm_minimize_controls *mc) {
+  const int every = mc->diagnostic_print_frequency;
+  return (every > 0) ? (mc->total_cycles + every - 1) / every : 0;
+}
+
+// Readies sc's history for a minimization's snapshots (sc->hist): a handle's series grows as
+// needed, a writer's must have room.  No interval or no series: no snapshots.
+static int prepare_history(const stormm_minimize_controls *mc, ScView *sc) {
+  sc->hist = HistKit{};
+  const int count = history_samples(mc);
+  if (count == 0) {
+    return STORMM_OK;
+  }
+  if (sc->owner != nullptr) {
+    const int rc = scorecard_series(sc->owner, sc->owner->samples + count);
+    if (rc != STORMM_OK) {
+      return rc;
+    }
+    *sc = sc_view(sc->owner);
+  }
+  if (sc->series == nullptr) {
+    return STORMM_OK;
+  }
+  if (sc->samples + count > sc->capacity) {
+    return fail(STORMM_ERR_CAPACITY,
+                "the ScoreCard's time series holds " + std::to_string(sc->capacity) +
+                    " samples; the minimization adds " + std::to_string(count) + " to " +
+                    std::to_string(sc->samples));
+  }
+  HistKit &h = sc->hist;
+  h.rows = sc->series;
+  h.running = sc->running;
+  h.squared = sc->squared;
+  h.scorecard = sc->acc;
+  h.sample_stride = static_cast<long long>(sc->nsys) * kScoreStride;
+  h.sys_stride = kScoreStride;
+  h.every = mc->diagnostic_print_frequency;
+  h.first = sc->samples;
+  h.capacity = sc->capacity;
+  h.max_cycle = 0;
+  h.inv_nrg_scale = std::ldexp(1.0, -sc->nrg_bits);
+  return STORMM_OK;
+}
+
+// After the minimization's launches: samples of systems that stopped early repeat their last
+// snapshot (steps: per-system cycle counts, null when every system ran every cycle), and a
+// handle counts the new samples, labelled with their cycles.
+static int finish_history(const stormm_minimize_controls *mc, const ScView &sc,
+                          const int *steps, hipStream_t stream) {
+  if (sc.hist.rows == nullptr) {
+    return STORMM_OK;
+  }
+  const int count = history_samples(mc);
+  if (steps != nullptr && mc->convergence_rms_force > 0.0) {
+    HIP_TRY(launch_history_fill(sc.hist, steps, sc.nsys, count, stream));
+  }
+  if (sc.owner != nullptr) {
+    for (int i = 0; i < count; i++) {
+      sc.owner->time_steps.push_back(i * mc->diagnostic_print_frequency);
+    }
+    sc.owner->samples += count;
+  }
   return STORMM_OK;
 }
 
@@ -1077,6 +1276,10 @@ static int check_mincon(const stormm_minimize_controls *mc) {
   }
   if (mc->clash_minimum_distance < 0.0 || mc->clash_vdw_ratio < 0.0) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "clash parameters must be non-negative");
+  }
+  if (mc->diagnostic_print_frequency < 0) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "the diagnostic print frequency must be "
+                                             "non-negative");
   }
   return STORMM_OK;
 }
@@ -1372,6 +1575,8 @@ static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag,
   const int prec = (precision == STORMM_PREC_DOUBLE) ? 0 : 1;
   MMKit k = make_kit(ag, w, &sc);
   k.store_overwrite = 1;
+  CgKit cg = make_cg_kit(ag, w, precision);
+  cg.hist = sc.hist;
   const int terms = kTermNonbonded | kTermValence;
   const int cdcyc = mc->clash_damping_cycles;
   for (int cyc = 0; cyc < mc->total_cycles; cyc++) {
@@ -1386,10 +1591,7 @@ static int minimize_stepwise(int32_t precision, const stormm_synthesis *ag,
     k.rst_step = minimizer_step(cyc, 0, cdcyc);
     HIP_TRY(launch_mm_eval(prec, k, terms, 1, 1, stream));
     step += 1;
-    if ((rc = conjugate_gradient(precision, ag, w, step, mc->steepest_descent_cycles, stream)) !=
-        STORMM_OK) {
-      return rc;
-    }
+    HIP_TRY(launch_conjugate_gradient(cg, step, mc->steepest_descent_cycles, stream));
     for (int move = 0; move < 4; move++) {
       if (move > 0) {
         k.rst_step = minimizer_step(cyc, move, cdcyc);
@@ -1459,6 +1661,7 @@ static int minimize_graph(int32_t precision, const stormm_synthesis *ag_c,
   k.store_overwrite = 1;
   set_clash(k, precision, 0.0, 0.0);
   CgKit cg = make_cg_kit(ag, w, precision);
+  cg.hist = sc.hist;
   cg.steps = ag->cycles_run.as<int>();
   cg.done = ag->stop_flags.as<int>();
   cg.rms = ag->rms_force.as<double>();
@@ -1603,6 +1806,7 @@ static int minimize_fused(int32_t precision, const stormm_synthesis *ag,
   set_clash(k, precision, 0.0, 0.0);
   CycleKit ck;
   ck.cg = make_cg_kit(ag, w, precision);
+  ck.cg.hist = sc.hist;
   ck.cg.steps = cs.steps;
   ck.cg.done = cs.done;
   ck.cg.rms = cs.rms;
@@ -1715,25 +1919,35 @@ static int minimize(int32_t precision, const stormm_synthesis *ag, const stormm_
   if ((rc = check_mincon(mincon)) != STORMM_OK) {
     return rc;
   }
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  if (persistent == 0) {
-    return minimize_stepwise(precision, ag, w, mincon, sc, lm, st);
-  }
-  if (persistent == 2) {
-    return minimize_graph(precision, ag, w, mincon, sc, lm, st);
-  }
-  if (persistent != 1 && persistent != 3 && persistent != 4) {
+  if (persistent < 0 || persistent > 4) {
     return fail(STORMM_ERR_INVALID_ARGUMENT, "minimization mode must be 0, 1, 2, 3 or 4");
   }
+  ScView sch = sc;
+  if ((rc = prepare_history(mincon, &sch)) != STORMM_OK) {
+    return rc;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
   stormm_synthesis *agm = const_cast<stormm_synthesis *>(ag);
-  CycleState cs;
-  cs.steps = agm->cycles_run.as<int>();
-  cs.done = agm->stop_flags.as<int>();
-  cs.rms = agm->rms_force.as<double>();
-  const FusedIssue how = (persistent == 1) ? FusedIssue::kWholeGraph :
-                         (persistent == 3) ? FusedIssue::kCycleGraph : FusedIssue::kDirect;
-  rc = minimize_fused(precision, ag, w, mincon, sc, lm, cs, st, how);
-  if (rc == STORMM_OK && how == FusedIssue::kDirect) {
+  if (persistent == 0) {
+    rc = minimize_stepwise(precision, ag, w, mincon, sch, lm, st);
+  }
+  else if (persistent == 2) {
+    rc = minimize_graph(precision, ag, w, mincon, sch, lm, st);
+  }
+  else {
+    CycleState cs;
+    cs.steps = agm->cycles_run.as<int>();
+    cs.done = agm->stop_flags.as<int>();
+    cs.rms = agm->rms_force.as<double>();
+    const FusedIssue how = (persistent == 1) ? FusedIssue::kWholeGraph :
+                           (persistent == 3) ? FusedIssue::kCycleGraph : FusedIssue::kDirect;
+    rc = minimize_fused(precision, ag, w, mincon, sch, lm, cs, st, how);
+  }
+  if (rc == STORMM_OK) {
+    rc = finish_history(mincon, sch, (persistent == 0) ? nullptr : agm->cycles_run.as<int>(),
+                        st);
+  }
+  if (rc == STORMM_OK && persistent == 4) {
     HIP_TRY(hipStreamSynchronize(st));
   }
   return rc;
@@ -1815,6 +2029,10 @@ extern "C" int stormm_launch_minimization_kit_noalloc(int32_t precision,
                                          std::to_string(kNoAllocMaxSystems) + " systems (" +
                                          std::to_string(ag->plan.nsys) + " given)");
   }
+  // A writer's history is the caller's: checked for room, never grown.
+  if ((rc = prepare_history(mincon, &sc)) != STORMM_OK) {
+    return rc;
+  }
   void *base = nullptr;
   HIP_TRY(noalloc_scratch(&base));
   CycleState cs;
@@ -1822,8 +2040,12 @@ extern "C" int stormm_launch_minimization_kit_noalloc(int32_t precision,
   cs.done = cs.steps + kNoAllocMaxSystems;
   cs.rms = reinterpret_cast<double *>(static_cast<unsigned char *>(base) +
                                       (static_cast<size_t>(kNoAllocMaxSystems) * 8));
-  return minimize_fused(precision, ag, *psw, mincon, sc, lm, cs,
-                        static_cast<hipStream_t>(stream), FusedIssue::kDirect);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  rc = minimize_fused(precision, ag, *psw, mincon, sc, lm, cs, st, FusedIssue::kDirect);
+  if (rc == STORMM_OK) {
+    rc = finish_history(mincon, sc, cs.steps, st);
+  }
+  return rc;
 }
 
 extern "C" int stormm_check_ps_writer(const stormm_synthesis *ag, const stormm_ps_writer *psw) {
@@ -1887,6 +2109,18 @@ extern "C" int stormm_time_mm_evaluation(int32_t precision, const stormm_synthes
                                          stormm_phasespace *ps, stormm_scorecard *sc,
                                          int32_t eval_force, int32_t repeats, void *stream,
                                          double *mean_ms) {
+  return stormm_time_launch(precision, ag, ps, sc, STORMM_TERMS_ALL, eval_force, repeats,
+                            stream, mean_ms);
+}
+
+extern "C" int stormm_time_launch(int32_t precision, const stormm_synthesis *ag,
+                                  stormm_phasespace *ps, stormm_scorecard *sc, int32_t terms,
+                                  int32_t eval_force, int32_t repeats, void *stream,
+                                  double *mean_ms) {
+  if (terms < STORMM_TERMS_NONBONDED || terms > STORMM_TERMS_ALL) {
+    return fail(STORMM_ERR_INVALID_ARGUMENT, "terms must be STORMM_TERMS_NONBONDED, "
+                                             "STORMM_TERMS_VALENCE or both");
+  }
   int rc = check_objects(ag, ps, sc);
   if (rc != STORMM_OK) {
     return rc;
@@ -1908,7 +2142,7 @@ extern "C" int stormm_time_mm_evaluation(int32_t precision, const stormm_synthes
   const int prec = (precision == STORMM_PREC_DOUBLE) ? 0 : 1;
   HIP_TRY(hipEventRecord(e0, st));
   for (int r = 0; r < repeats; r++) {
-    HIP_TRY(launch_mm_eval(prec, k, kTermNonbonded | kTermValence, eval_force, 1, st));
+    HIP_TRY(launch_mm_eval(prec, k, terms, eval_force, 1, st));
   }
   HIP_TRY(hipEventRecord(e1, st));
   HIP_TRY(hipEventSynchronize(e1));

@@ -42,7 +42,7 @@ class MinimizationPool:
                  dielectric: float = 78.5, salt: float = 0.0, coulomb: float = 332.0522172900,
                  shape=None, bits: Optional[Tuple[int, int, int]] = None,
                  clash_cycles: Optional[int] = None, clash_r0: float = 0.2,
-                 clash_vdw_ratio: float = 0.5):
+                 clash_vdw_ratio: float = 0.5, ntpr: int = 0):
         from . import synthesis as S
         self._L = _lib.load()
         if bits is None:
@@ -58,7 +58,7 @@ class MinimizationPool:
         cfg.coulomb_constant = coulomb
         cfg.minimize = S.minimize_controls(int(total_cycles), int(sd_cycles), float(dx0),
                                            float(conv_rms), clash_cycles, clash_r0,
-                                           clash_vdw_ratio)
+                                           clash_vdw_ratio, int(ntpr))
         sh = _shape(shape)
         if sh is not None:
             cfg.shape = sh
@@ -124,6 +124,19 @@ class MinimizationPool:
                                            None if stream is None else ctypes.c_void_p(stream)))
         return running.value
 
+    def history(self, slot: int):
+        """Potential energies (kcal/mol) of one slot's history, one every ntpr cycles from its
+        admission, and the ScoreCard rows behind them ([samples][32] int64)."""
+        cap = max(1, -(-int(self.config.minimize.total_cycles) //
+                       max(1, int(self.config.minimize.diagnostic_print_frequency))))
+        nrg = np.zeros(cap)
+        rows = np.zeros((cap, 32), dtype=np.int64)
+        n = ctypes.c_int32(0)
+        vp = ctypes.c_void_p
+        check(self._L.stormm_pool_history(self.handle, int(slot), cap, ctypes.byref(n),
+                                          nrg.ctypes.data_as(vp), rows.ctypes.data_as(vp)))
+        return nrg[:n.value], rows[:n.value]
+
     def status(self, slots: Sequence[int]):
         """(steps, done, rms force, potential energy) per slot."""
         arr = np.ascontiguousarray(np.asarray(slots, dtype=np.int32))
@@ -165,6 +178,8 @@ class Request:
     finished: float = -1.0
     energies: Optional[np.ndarray] = None
     steps: Optional[np.ndarray] = None
+    # Per conformer, its potential energies every ntpr cycles (pools with an energy history).
+    history: Optional[List[np.ndarray]] = None
 
 
 def serving_workload(seconds: float, rate: float, seed: int = 3, molecules: int = 256,
@@ -250,6 +265,8 @@ def serve(pool: MinimizationPool, requests: Sequence[Request], mode: str = "cont
                 r.finished = now
                 r.energies = nrg[pos:pos + k].copy()
                 r.steps = steps[pos:pos + k].copy()
+                if pool.config.minimize.diagnostic_print_frequency > 0:
+                    r.history = [pool.history(s)[0] for s in r.slots]
                 finished.append(r)
             pos += k
         for r in finished:

@@ -33,6 +33,10 @@ SCORECARD_STRIDE = 32
 STATE_VARIABLES = ["BOND", "ANGLE", "PROPER_DIHEDRAL", "IMPROPER_DIHEDRAL", "UREY_BRADLEY",
                    "CHARMM_IMPROPER", "CMAP", "VDW", "VDW_ONE_FOUR", "ELECTROSTATIC",
                    "ELEC_ONE_FOUR", "GENERALIZED_BORN", "RESTRAINT"]
+# StateVariable slots of the potential and total energy sums (src/Potential/
+# energy_enumerators.h:36-78).
+STATE_POTENTIAL_ENERGY = 28
+STATE_TOTAL_ENERGY = 29
 
 
 def precision_code(prec) -> int:
@@ -410,6 +414,67 @@ class ScoreCard:
                                                         ctypes.byref(cnt)))
         return (run.reshape(-1, SCORECARD_STRIDE), sq.reshape(-1, SCORECARD_STRIDE), cnt.value)
 
+    # ---- time series (ScoreCard::time_series_accumulators, src/Potential/scorecard.h:503) ----
+    def sample_count(self) -> int:
+        """ScoreCard::getSampleSize (scorecard.h:137)."""
+        n, cap = ctypes.c_int32(0), ctypes.c_int32(0)
+        check(self._L.stormm_scorecard_sample_count(self.handle, ctypes.byref(n),
+                                                    ctypes.byref(cap)))
+        return n.value
+
+    def sample_capacity(self) -> int:
+        """ScoreCard::getSampleCapacity (scorecard.h:145)."""
+        n, cap = ctypes.c_int32(0), ctypes.c_int32(0)
+        check(self._L.stormm_scorecard_sample_count(self.handle, ctypes.byref(n),
+                                                    ctypes.byref(cap)))
+        return cap.value
+
+    def reserve(self, capacity: int) -> None:
+        check(self._L.stormm_scorecard_reserve(self.handle, int(capacity)))
+
+    def reset_sample_count(self, count: int = 0) -> None:
+        check(self._L.stormm_scorecard_reset_sample_count(self.handle, int(count)))
+
+    def set_last_time_step(self, time_step: int) -> None:
+        check(self._L.stormm_scorecard_set_last_time_step(self.handle, int(time_step)))
+
+    def series(self, first: int = 0, count: Optional[int] = None):
+        """Samples [first, first + count) as ([count][nsys][32] int64 at nrg_bits, time steps);
+        slot 28 holds the potential energy sum and slot 29 the total."""
+        if count is None:
+            count = self.sample_count() - first
+        rows = np.zeros((count, self.system_count, SCORECARD_STRIDE), dtype=np.int64)
+        steps = np.zeros(count, dtype=np.int32)
+        check(self._L.stormm_scorecard_download_series(self.handle, int(first), int(count),
+                                                       _ptr(rows), _ptr(steps)))
+        return rows, steps
+
+    def report_history(self, system_index: int = 0, aspect: int = STATE_TOTAL_ENERGY):
+        """ScoreCard::reportHistory(aspect, system_index) (scorecard.cpp:1090-1102): one value
+        per sample, kcal/mol."""
+        rows, _ = self.series()
+        return rows[:, system_index, aspect].astype(np.float64) * 2.0 ** -self.nrg_bits
+
+    def report_history_over_systems(self, aspect: int = STATE_TOTAL_ENERGY):
+        """ScoreCard::reportHistory(aspect) (scorecard.cpp:1110-1134): per sample, the mean and
+        the standard deviation (n - 1 denominator) over the systems."""
+        rows, _ = self.series()
+        v = rows[:, :, aspect].astype(np.float64) * 2.0 ** -self.nrg_bits
+        std = v.std(axis=1, ddof=1) if self.system_count > 1 else np.zeros(len(v))
+        return v.mean(axis=1), std
+
+    def report_average_states(self):
+        """ScoreCard::reportAverageStates (scorecard.cpp:869-896): running sums over the sample
+        count, [nsys][32] kcal/mol."""
+        run, _, n = self.history()
+        return run / n
+
+    def report_variance_of_states(self):
+        """ScoreCard::reportVarianceOfStates (scorecard.cpp:988-1021): sqrt(n S2 - S1^2) /
+        sqrt(n (n - 1)) of the running sums S1 and S2, [nsys][32]."""
+        run, sq, n = self.history()
+        return np.sqrt(np.maximum(n * sq - run * run, 0.0)) / np.sqrt(n * (n - 1.0))
+
     def close(self):
         if getattr(self, "handle", None):
             self._L.stormm_scorecard_destroy(self.handle)
@@ -424,11 +489,13 @@ class ScoreCard:
 
 def minimize_controls(total_cycles: int = 200, sd_cycles: int = 50, dx0: float = 0.01,
                       rms_force: float = 0.0, clash_cycles: Optional[int] = None,
-                      clash_r0: float = 0.2, clash_vdw_ratio: float = 0.5) -> MinimizeControls:
+                      clash_r0: float = 0.2, clash_vdw_ratio: float = 0.5,
+                      ntpr: int = 0) -> MinimizeControls:
     """Defaults of the reference's &minimize namelist (src/Namelists/nml_minimize.h:21-30):
     maxcyc 200, ncyc 50, cdcyc 25 (capped at maxcyc), dx0 0.01, clash_r0 0.2,
-    clash_vdw_ratio 0.5."""
+    clash_vdw_ratio 0.5, ntpr 0.  ntpr > 0 commits the ScoreCard every ntpr cycles."""
     m = MinimizeControls()
+    m.diagnostic_print_frequency = ntpr
     m.total_cycles = total_cycles
     m.steepest_descent_cycles = sd_cycles
     m.initial_step = dx0
@@ -550,6 +617,19 @@ def time_mm_evaluation(prec, ag, ps, sc, eval_force=True, repeats=10, stream=Non
     check(_lib.load().stormm_time_mm_evaluation(precision_code(prec), ag.handle, ps.handle,
                                                 sc.handle, int(eval_force), int(repeats),
                                                 _stream_ptr(stream), ctypes.byref(ms)))
+    return ms.value
+
+
+TERMS = {"nonbonded": 1, "valence": 2, "all": 3}
+
+
+def time_launch(prec, ag, ps, sc, terms="all", eval_force=True, repeats=10, stream=None) -> float:
+    """Mean kernel time (ms) of `repeats` launches of the chosen terms: "nonbonded" (Coulomb,
+    Lennard-Jones and GB), "valence" or "all" (the fused evaluation)."""
+    ms = ctypes.c_double(0.0)
+    check(_lib.load().stormm_time_launch(precision_code(prec), ag.handle, ps.handle, sc.handle,
+                                         TERMS[terms], int(eval_force), int(repeats),
+                                         _stream_ptr(stream), ctypes.byref(ms)))
     return ms.value
 
 

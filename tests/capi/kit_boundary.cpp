@@ -179,6 +179,7 @@ int setup(CallerArrays &c, const stormm_synthesis_info &info, const int32_t *sta
     *words[i] = c.alloc<int64_t>(np);
     *ovrf[i] = c.alloc<int32_t>(np);
   }
+  std::memset(&c.scw, 0, sizeof(c.scw));
   c.scw.system_count = info.system_count;
   c.scw.data_stride = STORMM_SCORECARD_STRIDE;
   c.scw.nrg_bits = nrg;

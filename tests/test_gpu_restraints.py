@@ -211,9 +211,10 @@ def test_restrained_double_minimizer_tracks_the_oracle():
             assert e[s] == pytest.approx(trace[-1], rel=1e-8, abs=1e-6)
 
 
-@pytest.mark.skip(reason="the persistent minimizer's restraint path is switched off: its one run "
-                         "with restraints ended in a GPU memory fault whose cause is not yet "
-                         "found, and the mode now refuses restrained syntheses")
+@pytest.mark.skip(reason="mode 1 refuses restrained syntheses: the persistent kernel that once "
+                         "ran this case ended in a GPU memory fault whose cause was not found; "
+                         "that kernel is removed, and the fused-cycle kernels that now serve "
+                         "mode 1 are not run on this case until a cause is shown")
 @pytest.mark.parametrize("prec", ["double", "single"])
 def test_persistent_restrained_minimizer_equals_per_step(prec):
     topos, coords, rst = batch(8, 34, every=2)

@@ -13,7 +13,10 @@ against the FP64 CPU oracle are measured on a 16-system subsample.  The CPU base
 oracle (the restatement of the reference's CPU path) on a bounded sample of the same systems with
 one thread per core of the job's CPU share.
 
-Two further workloads ride on the same line (skip them with --no-extra):
+Further workloads ride on the same line (skip them with --no-extra):
+  "lj_table": the same 1024 molecules with 16 Lennard-Jones types each and stored T x T tables
+    (one pair of types off the combining rule, as an NBFIX pair is), the path an AMBER topology
+    takes: kernel time next to the combining-rule path's, and FP32 errors on 16 systems.
   "minimization": 4096 systems, N ~ U[16, 128], starts perturbed by N(0, 0.3 A), 500 cycles
     (50 steepest descent, then conjugate gradient, the first 25 clash-damped) in FP32: the fused
     cycle kernels captured whole in one HIP graph (whole batch, and once per size class), the
@@ -91,6 +94,8 @@ def parity_errors(S, oracle, topos, coords, prec, gpos, frc, nrg):
     F = ps.forces()
     X = ps.coordinates()
     worst_e, worst_et, worst_f = 0.0, 0.0, 0.0
+    # Per energy component, relative to that component (components of at least 1 kcal/mol).
+    worst_c = {}
     for s, t in enumerate(topos):
         eo_d, _, fo = oracle.eval_mm(t, X[s], True, nrg)
         eo = np.array([eo_d[k] for k in oracle.SLOT_NAMES])
@@ -98,8 +103,11 @@ def parity_errors(S, oracle, topos, coords, prec, gpos, frc, nrg):
         worst_e = max(worst_e, abs(E[s].sum() - eo.sum()) / scale)
         worst_et = max(worst_et, abs(E[s].sum() - eo.sum()) / abs(eo.sum()))
         worst_f = max(worst_f, float(np.sqrt(((F[s] - fo) ** 2).sum() / (fo ** 2).sum())))
+        for k, name in enumerate(oracle.SLOT_NAMES):
+            if abs(eo[k]) >= 1.0:
+                worst_c[name] = max(worst_c.get(name, 0.0), abs(E[s][k] - eo[k]) / abs(eo[k]))
     return {"energy_rel_sum_abs_terms": worst_e, "energy_rel_total": worst_et,
-            "force_rel_rms": worst_f}
+            "energy_rel_component": worst_c, "force_rel_rms": worst_f}
 
 
 def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu, shape=None):
@@ -208,6 +216,86 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu, shape=No
                                          f"(time scaled by pair work)",
                                "host_cores_visible": visible, "cpu_model": cpu_model(),
                                "compiler": "gcc " + " ".join(oracle.CFLAGS)}
+    return out
+
+
+def stage_breakdown(S, topos, coords, reps):
+    """Where the FP32 evaluation's time goes, from launches of the library's own entry points on
+    the headline batch: the fused evaluation with and without forces, the non-bonded launch
+    (Born integrals, radii, Coulomb + Lennard-Jones + GB pairs, radius derivatives) with and
+    without forces, the valence launch, and the non-bonded launch of the same molecules in
+    vacuum (Coulomb + Lennard-Jones pairs only)."""
+    import copy
+    out = {}
+
+    def timed(tops, runs):
+        ag = S.AtomGraphSynthesis(tops)
+        ps = S.PhaseSpaceSynthesis(ag, 32, 23)
+        sc = S.ScoreCard(ag.system_count, 26)
+        ps.upload(coords)
+        for name, terms, force in runs:
+            S.time_launch("single", ag, ps, sc, terms, force, 5)
+            out[name] = S.time_launch("single", ag, ps, sc, terms, force, reps)
+
+    timed(topos, [("fused_force_ms", "all", True), ("fused_energy_ms", "all", False),
+                  ("nonbonded_force_ms", "nonbonded", True),
+                  ("nonbonded_energy_ms", "nonbonded", False),
+                  ("valence_force_ms", "valence", True)])
+    vacuum = []
+    for t in topos:
+        v = copy.copy(t)
+        v.igb = 0
+        vacuum.append(v)
+    timed(vacuum, [("vacuum_nonbonded_force_ms", "nonbonded", True)])
+    out["gb_derivative_and_pair_forces_ms"] = (out["nonbonded_force_ms"] -
+                                               out["nonbonded_energy_ms"])
+    out["note"] = ("kernel times (HIP events, mean of the repeats) of separate launches; the "
+                   "fused evaluation overlaps valence with the Born-integral tiles, so the parts "
+                   "do not add up to it exactly")
+    return out
+
+
+def table_topologies(topos, ntypes=16):
+    """The same molecules with AMBER-style Lennard-Jones type tables: atom i takes type i mod
+    ntypes (the type's sigma and epsilon are those of its first atom), the T x T A / B tables are
+    built by Lorentz-Berthelot combining, and one pair of types departs from it by 5 % (as an
+    NBFIX pair does), so every system keeps its stored tables."""
+    import copy
+    out = []
+    for t in topos:
+        types = (np.arange(t.natom) % ntypes).astype(np.int32)
+        first = [int(np.flatnonzero(types == k)[0]) for k in range(ntypes)]
+        sig = np.asarray(t.lj_sigma, dtype=np.float64)[first]
+        eps = np.asarray(t.lj_epsilon, dtype=np.float64)[first]
+        sij = 0.5 * (sig[:, None] + sig[None, :])
+        eij = np.sqrt(eps[:, None] * eps[None, :])
+        a = 4.0 * eij * sij ** 12
+        b = 4.0 * eij * sij ** 6
+        a[0, 1] *= 1.05
+        a[1, 0] = a[0, 1]
+        v = copy.copy(t)
+        v.lj_idx, v.n_lj_types = types, ntypes
+        v.lj_sigma = v.lj_epsilon = None
+        v.lja, v.ljb = a.reshape(-1), b.reshape(-1)
+        v.lja14, v.ljb14 = a.reshape(-1).copy(), b.reshape(-1).copy()
+        out.append(v)
+    return out
+
+
+def lj_table_workload(S, oracle, topos, coords, reps):
+    """The headline batch on the stored-table Lennard-Jones path: kernel time against the
+    combining-rule path of the same molecules, and oracle errors on 16 systems."""
+    tab = table_topologies(topos)
+    ag = S.AtomGraphSynthesis(tab)
+    ps = S.PhaseSpaceSynthesis(ag, 32, 23)
+    sc = S.ScoreCard(ag.system_count, 26)
+    ps.upload(coords)
+    S.time_mm_evaluation("single", ag, ps, sc, True, 10)
+    ms = S.time_mm_evaluation("single", ag, ps, sc, True, reps)
+    out = {"systems": len(tab), "lj_types_per_system": 16,
+           "table_systems": int(ag.info.lj_table_systems), "kernel_ms": ms,
+           "system_evals_per_s": len(tab) / (ms * 1.0e-3)}
+    out["errors_fp32"] = parity_errors(S, oracle, tab[:16], coords[:16], "single", 32, 23, 26)
     return out
 
 
@@ -330,6 +418,8 @@ def main():
     bytes_alg = roofline.eval_bytes(atoms, info.valence_term_count, info.task_count, True)
     log(f"kernel {kernel_ms:.4f} ms (FP64 {kernel_ms_fp64:.4f} ms); {flops / 1e9:.3f} GFLOP "
         f"-> {achieved:.2f} TFLOP/s")
+    breakdown = stage_breakdown(S, topos, coords, max(50, args.steps // 2))
+    log(f"stage breakdown: {breakdown}")
     measured_fma = S.fma_throughput()
     measured_bw = S.copy_bandwidth(1 << 30, 10)
     log(f"measured ceilings: FP32 FMA {measured_fma:.1f} TFLOP/s, copy {measured_bw:.0f} GB/s")
@@ -385,6 +475,9 @@ def main():
     extra = {}
     if not args.no_extra:
         import oracle
+        extra["lj_table"] = lj_table_workload(S, oracle, topos, coords, max(50, args.steps // 2))
+        extra["lj_table"]["combining_rule_kernel_ms"] = kernel_ms
+        log(f"LJ tables: {extra['lj_table']}")
         extra["minimization"] = minimization_workload(S, synthetic, oracle, 4096,
                                                       args.min_cycles, not args.no_cpu)
         log(f"minimization: {extra['minimization']}")
@@ -440,6 +533,7 @@ def main():
             "frac_of_measured_fma": achieved / measured_fma if measured_fma > 0 else None,
             "measured_copy_gbps": measured_bw,
         },
+        "stage_breakdown": breakdown,
         "fp64_mode": {"kernel_ms": kernel_ms_fp64,
                       "value": info.system_count / (kernel_ms_fp64 * 1.0e-3),
                       "autotuned_shape": shape64},

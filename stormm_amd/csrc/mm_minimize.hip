@@ -368,8 +368,8 @@ __device__ __forceinline__ void store_line(const LineKit &k, int s, const LineSt
 // Line move `move` of system s at potential energy `cur` (src/MolecularMechanics/
 // line_movement.cui:67-330): records the energy, updates the state and returns the move in
 // fixed-precision coordinate units per fixed-precision direction unit (0 for no move).
-__device__ __forceinline__ double line_step(const LineKit &k, int s, int move, double cur) {
-  LineState ls = load_line(k, s);
+__device__ __forceinline__ double line_step(const LineKit &k, int s, int move, double cur,
+                                            LineState ls) {
   double mf = line_move_factor(ls, move, cur);
   store_line(k, s, ls);
   if (fabs(mf) > 1.0e-12) {
@@ -443,7 +443,8 @@ __global__ void __launch_bounds__(64 * kLineWaves) k_line_advance(LineKit k, int
   }
   double mf = 0.0;
   if (lane == 0) {
-    mf = line_step(k, s, move_number, static_cast<double>(acc) * k.inv_nrg_scale);
+    mf = line_step(k, s, move_number, static_cast<double>(acc) * k.inv_nrg_scale,
+                   load_line(k, s));
   }
   mf = __shfl(mf, 0, 64);
   if (mf == 0.0) {
@@ -489,8 +490,21 @@ __global__ void __launch_bounds__(EvalBlockLimit<T>::value)
   extern __shared__ __align__(16) unsigned char smem[];
   const WuDesc &wu = k.wu[blockIdx.x];
   const int tid = static_cast<int>(threadIdx.x);
-  const bool stopped = (tid >= wu.sys_count) ||
-                       (ck.cg.done[k.wu_sys[wu.sys_first + tid].sys_index] != 0);
+  const bool has_sys = tid < wu.sys_count;
+  const int s = has_sys ? k.wu_sys[wu.sys_first + tid].sys_index : 0;
+  const bool stopped = !has_sys || (ck.cg.done[s] != 0);
+  // The pending move's inputs (ScoreCard row and line-search state of the thread's system) are
+  // loaded here, so that their latency overlaps the unit's set-up; a stopped system does not
+  // move, nor does any system in the force evaluation of its first cycle.
+  const bool moves = (pending_move >= 0) && !stopped && !(FORCES && ck.cg.steps[s] == 0);
+  LineState ls{};
+  long long acc = 0;
+  if (moves) {
+    ls = load_line(ck.line, s);
+    for (int e = 0; e < kNrgSlots; e++) {
+      acc += k.scorecard[(static_cast<size_t>(s) * kScoreStride) + e];
+    }
+  }
   if (__syncthreads_and(stopped ? 1 : 0) != 0) {
     return;
   }
@@ -499,22 +513,12 @@ __global__ void __launch_bounds__(EvalBlockLimit<T>::value)
   init_ctx<T>(c, k, blockIdx.x, terms);
   stage_load_params<T>(c, k);
   if (pending_move >= 0) {
-    // Move factors of the unit's systems (stopped systems do not move; line_step as in
-    // k_line_advance, on the exact sum of the 13 ScoreCard components).  The coordinate stage
-    // reads them after its first barrier.
-    if (c.tid < c.wu.sys_count) {
-      const int s = c.L.sys[c.tid].sys_index;
-      double mf = 0.0;
-      const bool first_cycle = FORCES && (ck.cg.steps[s] == 0);
-      if (ck.cg.done[s] == 0 && !first_cycle) {
-        long long acc = 0;
-        for (int e = 0; e < kNrgSlots; e++) {
-          acc += k.scorecard[(static_cast<size_t>(s) * kScoreStride) + e];
-        }
-        mf = line_step(ck.line, s, pending_move,
-                       static_cast<double>(acc) * ck.line.inv_nrg_scale);
-      }
-      c.L.move[c.tid] = mf;
+    // Move factors of the unit's systems (line_step as in k_line_advance, on the exact sum of
+    // the 13 ScoreCard components).  The coordinate stage reads them after its first barrier.
+    if (has_sys) {
+      c.L.move[tid] = moves ? line_step(ck.line, s, pending_move,
+                                        static_cast<double>(acc) * ck.line.inv_nrg_scale, ls)
+                            : 0.0;
     }
     c.move_pending = true;
   }

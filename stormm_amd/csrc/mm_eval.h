@@ -719,10 +719,12 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
     // sums to fixed precision once, as the reference does per tile
     // (src/Potential/nonbonded_potential_tilegroups.cui:461-464,640-645).
     T e_qq = T(0), e_lj = T(0), e_gb = T(0);
-    int lj_off = 0, lj_nt = 0;
+    // The row of the system's A / B table that atom i's type selects (a per-lane pointer, so
+    // the steps need no kernel-argument registers to address it).
+    const double2 *ljrow = nullptr;
     if (LJTABLE) {
-      lj_off = L.sys[tt.slot].lj_off;
-      lj_nt = L.sys[tt.slot].lj_nt;
+      ljrow = k.ljab + L.sys[tt.slot].lj_off +
+              (static_cast<int>(p1i) * L.sys[tt.slot].lj_nt);
     }
     bool clash_on = false;
     T cd = T(0), ca = T(0), cb = T(0), cratio = T(0);
@@ -786,8 +788,7 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
           const T pj2 = row_ror<K>(p2j);
           T lja, ljb, sigma;
           if (LJTABLE) {
-            const double2 ab = k.ljab[lj_off + (static_cast<int>(p1i) * lj_nt) +
-                                      static_cast<int>(pj1)];
+            const double2 ab = ljrow[static_cast<int>(pj1)];
             lja = nbok ? static_cast<T>(ab.x) : T(0);
             ljb = nbok ? static_cast<T>(ab.y) : T(0);
             sigma = (ljb > T(1.0e-6)) ? P::sqrt(P::cbrt(lja / ljb)) : T(0);
@@ -810,8 +811,7 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
           const T eqq = qnb * invr;
           e_qq += eqq;
           if (LJTABLE) {
-            const double2 ab = k.ljab[lj_off + (static_cast<int>(p1i) * lj_nt) +
-                                      static_cast<int>(row_ror<K>(p1j))];
+            const double2 ab = ljrow[static_cast<int>(row_ror<K>(p1j))];
             const T lja = nbok ? static_cast<T>(ab.x) : T(0);
             const T ljb = nbok ? static_cast<T>(ab.y) : T(0);
             const T invr4 = invr2 * invr2;

@@ -177,7 +177,10 @@ def minimization_workload(S, synthetic, oracle, nsys, cycles, with_cpu, shape=No
            "per_size_class_s": t_class, "size_classes": len(classes),
            "minimizations_per_s": nsys / timings[best],
            "best_mode": best,
-           "fused_vs_per_step": timings["per_step"] / timings["fused_graph"],
+           "fused_vs_per_step": timings["per_step"] / min(timings["fused_graph"],
+                                                          timings["fused_cycle_graph"],
+                                                          timings["fused_direct"]),
+           "fused_graph_vs_per_step": timings["per_step"] / timings["fused_graph"],
            "graph_vs_per_step": timings["per_step"] / timings["graph_replay"],
            "fused_equals_per_step": bool(all(np.array_equal(energies[m], energies["per_step"])
                                              for m in ("fused_graph", "fused_cycle_graph",

@@ -94,8 +94,10 @@ struct WuSys {
   int task_first;    // template tile tasks
   int offd_count;
   int diag_count;
-  int term_first;    // template valence terms
+  int term_first;    // template valence terms: bonds, then angles, then dihedrals
   int term_count;
+  int bond_count;
+  int angl_count;
   int lj_off;        // stored LJ table (offset, type count)
   int lj_nt;
   int rst_first;     // the system's restraints in the synthesis restraint list

@@ -260,7 +260,8 @@ template <typename T> struct WuLds {
   crd_t *cx, *cy, *cz;
   int *gidx, *slot;
   WuSys *sys;
-  int *opre, *dpre, *vpre;  // per slot: first off-diagonal task, diagonal task, valence term
+  int *opre, *dpre;         // per slot: first off-diagonal task, first diagonal task
+  int *vpre;                // per kind (bonds, angles, dihedrals): each slot's first term
   int *work;                // per-stage work-item counters (kStageCounters)
   T *clash_d, *clash_r;     // per-system clash distance and sigma ratio of this evaluation
   int *clash_on;            // per-system flag: damped potentials apply
@@ -273,7 +274,7 @@ template <typename T> struct WuLds {
            (3 * kMaxWuSystems * sizeof(long long)) + (kMaxWuSystems * sizeof(double)) +
            (9 * m * sizeof(T)) +
            (3 * m * sizeof(crd_t)) + (2 * m * sizeof(int)) + (kMaxWuSystems * sizeof(WuSys)) +
-           (3 * (kMaxWuSystems + 1) * sizeof(int)) + (kStageCounters * sizeof(int)) +
+           (5 * (kMaxWuSystems + 1) * sizeof(int)) + (kStageCounters * sizeof(int)) +
            (2 * kMaxWuSystems * sizeof(T)) + (kMaxWuSystems * sizeof(int)) +
            (static_cast<size_t>(waves) * kNearQueueStride * sizeof(unsigned));
   }
@@ -314,7 +315,7 @@ template <typename T> struct WuLds {
     sys = reinterpret_cast<WuSys *>(take(kMaxWuSystems * sizeof(WuSys)));
     opre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
     dpre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
-    vpre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
+    vpre = reinterpret_cast<int *>(take(3 * (kMaxWuSystems + 1) * sizeof(int)));
     work = reinterpret_cast<int *>(take(kStageCounters * sizeof(int)));
     clash_on = reinterpret_cast<int *>(take(kMaxWuSystems * sizeof(int)));
     nearq = reinterpret_cast<unsigned *>(take(static_cast<size_t>(waves) * kNearQueueStride *

@@ -58,18 +58,28 @@ __device__ __forceinline__ void stage_load_params(StageCtx<T> &c, const MMKit &k
   }
   __syncthreads();
   if (c.tid == 0) {
-    int o = 0, d = 0, v = 0;
+    // Valence terms are taken kind by kind (all bonds of the unit, then all angles, then all
+    // dihedrals), so that a chunk of 64 runs one kind's code.
+    constexpr int K1 = kMaxWuSystems + 1;
+    int o = 0, d = 0, vb = 0, va = 0, vd = 0;
     for (int s = 0; s < wu.sys_count; s++) {
+      const WuSys &ws = L.sys[s];
       L.opre[s] = o;
       L.dpre[s] = d;
-      L.vpre[s] = v;
-      o += L.sys[s].offd_count;
-      d += L.sys[s].diag_count;
-      v += L.sys[s].term_count;
+      L.vpre[s] = vb;
+      L.vpre[K1 + s] = va;
+      L.vpre[2 * K1 + s] = vd;
+      o += ws.offd_count;
+      d += ws.diag_count;
+      vb += ws.bond_count;
+      va += ws.angl_count;
+      vd += ws.term_count - ws.bond_count - ws.angl_count;
     }
     L.opre[wu.sys_count] = o;
     L.dpre[wu.sys_count] = d;
-    L.vpre[wu.sys_count] = v;
+    L.vpre[wu.sys_count] = vb;
+    L.vpre[K1 + wu.sys_count] = va;
+    L.vpre[2 * K1 + wu.sys_count] = vd;
   }
   __syncthreads();
   // The unit's tile tasks, read once per evaluation: off-diagonal tasks of every system, then
@@ -403,7 +413,29 @@ __device__ __forceinline__ int stage_item(int i, int quads, int chunks) {
 }
 
 template <typename T, bool FORCES, bool CLASH>
-__device__ void valence_term(StageCtx<T> &c, const MMKit &k, int t);
+__device__ void valence_term(StageCtx<T> &c, const MMKit &k, int kind, int t);
+
+// Chunks of 64 valence terms of the unit: its bonds, then its angles, then its dihedrals, each
+// kind starting a chunk of its own.
+template <typename T> __device__ __forceinline__ int valence_chunks(const StageCtx<T> &c) {
+  constexpr int K1 = kMaxWuSystems + 1;
+  const int n = c.wu.sys_count;
+  return ((c.L.vpre[n] + 63) >> 6) + ((c.L.vpre[K1 + n] + 63) >> 6) +
+         ((c.L.vpre[2 * K1 + n] + 63) >> 6);
+}
+
+// Valence chunk `chunk` (wavefront-uniform), one term per lane.
+template <typename T, bool FORCES, bool CLASH>
+__device__ __forceinline__ void valence_chunk(StageCtx<T> &c, const MMKit &k, int chunk,
+                                              int lane) {
+  constexpr int K1 = kMaxWuSystems + 1;
+  const int n = c.wu.sys_count;
+  const int cb = (c.L.vpre[n] + 63) >> 6;
+  const int ca = (c.L.vpre[K1 + n] + 63) >> 6;
+  const int kind = (chunk < cb) ? kVBond : (chunk < cb + ca) ? kVAngle : kVDihedral;
+  const int local = chunk - ((kind == kVBond) ? 0 : (kind == kVAngle) ? cb : cb + ca);
+  valence_term<T, FORCES, CLASH>(c, k, kind, (local << 6) + lane);
+}
 
 // Near-field pair queue of a wavefront.  Entries pack the LDS indices of the pair (9 bits each)
 // and which of the two directions (onto i, onto j) lies within 4 screened radii.
@@ -498,7 +530,7 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k, int val_chunks) {
   for (int item = next_item(&L.work[0], lane); item < items; item = next_item(&L.work[0], lane)) {
     const int it = stage_item(item, quads, val_chunks);
     if (it >= 0) {
-      valence_term<T, FORCES, CLASH>(c, k, (it << 6) + lane);
+      valence_chunk<T, FORCES, CLASH>(c, k, it, lane);
       continue;
     }
     const TileTask tt(wave_task(c, k, ((-1 - it) << 2) + ((c.tid >> 4) & 3)), il);
@@ -1587,8 +1619,9 @@ __device__ double eval_restraint(const V &v, const stormm_restraint &t, int base
 
 // Valence term t of the work unit (nothing when t is past the unit's last term).
 template <typename T, bool FORCES, bool CLASH>
-__device__ void valence_term(StageCtx<T> &c, const MMKit &k, int t) {
-  if (t >= c.wu.term_count) {
+__device__ void valence_term(StageCtx<T> &c, const MMKit &k, int kind, int t) {
+  const int *pre = c.L.vpre + (kind * (kMaxWuSystems + 1));
+  if (t >= pre[c.wu.sys_count]) {
     return;
   }
   const double fscale = ldexp(1.0, k.frc_bits);
@@ -1603,10 +1636,12 @@ __device__ void valence_term(StageCtx<T> &c, const MMKit &k, int t) {
   v.fx = c.L.fx;
   v.fy = c.L.fy;
   v.fz = c.L.fz;
-  const int slot = find_slot(c.L.vpre, c.wu.sys_count, t);
+  // The system's terms of this kind, in its template's bonds / angles / dihedrals order.
+  const int slot = find_slot(pre, c.wu.sys_count, t);
   const WuSys &ws = c.L.sys[slot];
-  const int4 vt = k.vterms[ws.term_first + (t - c.L.vpre[slot])];
-  const int kind = vt.x;
+  const int kind_first = (kind == kVBond) ? 0 :
+                         (kind == kVAngle) ? ws.bond_count : ws.bond_count + ws.angl_count;
+  const int4 vt = k.vterms[ws.term_first + kind_first + (t - pre[slot])];
   const int i = ws.lds_base + (vt.y & 0xffff), j = ws.lds_base + (vt.y >> 16);
   const int kk = ws.lds_base + (vt.z & 0xffff), l = ws.lds_base + (vt.z >> 16);
   long long *nrg = &c.L.nrg[slot * 16];
@@ -1657,10 +1692,10 @@ __device__ void valence_term(StageCtx<T> &c, const MMKit &k, int t) {
 template <typename T, bool FORCES, bool CLASH>
 __device__ void stage_valence(StageCtx<T> &c, const MMKit &k) {
   const int lane = c.tid & 63;
-  const int chunks = (c.wu.term_count + 63) >> 6;
+  const int chunks = valence_chunks(c);
   for (int item = next_item(&c.L.work[2], lane); item < chunks;
        item = next_item(&c.L.work[2], lane)) {
-    valence_term<T, FORCES, CLASH>(c, k, (item << 6) + lane);
+    valence_chunk<T, FORCES, CLASH>(c, k, item, lane);
   }
   __syncthreads();
 }
@@ -1711,7 +1746,7 @@ __device__ __forceinline__ void evaluate_wu(StageCtx<T> &c, const MMKit &k, int 
   // With GB the valence terms ride along with the Born-integral tiles.
   const bool val = (terms & kTermValence) != 0;
   if (c.gb) {
-    stage_psi<T, FORCES, CLASH>(c, k, val ? ((c.wu.term_count + 63) >> 6) : 0);
+    stage_psi<T, FORCES, CLASH>(c, k, val ? valence_chunks(c) : 0);
     stage_radii<T, FORCES>(c, k, 0.0);
   }
   if (terms & kTermNonbonded) {

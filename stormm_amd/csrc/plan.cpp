@@ -321,6 +321,9 @@ static int add_template(const stormm_topology_batch *b, const SystemTerms &t, in
                                     (scee > 0.0 && scnb > 0.0) ? 1.0 / scnb : 0.0));
   }
   tp.term_count = static_cast<int>(P.vterms.size()) - tp.term_first;
+  tp.bond_count = t.bond_count;
+  tp.angl_count = t.angl_count;
+  tp.dihe_count = t.dihe_count;
   // Cost model: rotation steps of the three pair passes plus valence work.
   tp.cost = (16.0 * tp.offd_count) + (9.0 * tp.diag_count) +
             0.25 * (t.bond_count + t.angl_count + 2 * t.dihe_count) + 0.1 * n;
@@ -783,6 +786,8 @@ int append_work_unit(const std::vector<int> &systems, int lj_table,
     ws.diag_count = tp.diag_count;
     ws.term_first = tp.term_first;
     ws.term_count = tp.term_count;
+    ws.bond_count = tp.bond_count;
+    ws.angl_count = tp.angl_count;
     ws.lj_off = tp.lj_off;
     ws.lj_nt = tp.lj_nt;
     ws.rst_first = (rst_first != nullptr) ? (*rst_first)[s] : 0;

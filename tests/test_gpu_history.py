@@ -235,3 +235,37 @@ def test_pool_history_of_systems_stopped_by_the_rms_force():
         want = -(-int(st_) // ntpr) + (1 if (st_ % ntpr == 0 and st_ < cycles) else 0)
         assert len(hist) == want, (slot, st_, len(hist))
         assert hist[-1] < hist[0] and e < hist[0]
+
+
+def test_noalloc_minimization_writes_the_callers_series():
+    """The allocation-free entry point records the same history as the kit entry point in mode
+    1, including the samples that systems stopped by the RMS force repeat, and refuses a series
+    without room."""
+    from test_gpu_kits import CallerState, DevArray
+    import ctypes
+
+    L = _lib.load()
+    gpos, frc, nrg = BITS["single"]
+    topos, coords = synthetic.config2(nsys=9, seed=40)
+    ag = S.AtomGraphSynthesis(topos)
+    fixed = [np.rint(np.asarray(c) * 2.0 ** gpos).astype(np.int64) for c in coords]
+    cycles, ntpr = 120, 10
+    mc = S.minimize_controls(cycles, 10, 0.01, 1.0, clash_cycles=0, ntpr=ntpr)
+    series = {}
+    for name in ("kit", "noalloc"):
+        st = CallerState(ag, gpos, frc, nrg, fixed)
+        rows = DevArray(np.zeros((12, ag.system_count, 32), dtype=np.int64))
+        st.scw.time_series = rows.ptr.value
+        st.scw.sample_capacity = 11
+        args = (_lib.PREC_SINGLE, ag.handle, ctypes.byref(st.psw), ctypes.byref(mc),
+                ctypes.byref(st.scw), ctypes.byref(st.lmw))
+        call = ((lambda: L.stormm_launch_minimization_kit(*args, 1, None)) if name == "kit"
+                else (lambda: L.stormm_launch_minimization_kit_noalloc(*args, None)))
+        with pytest.raises(_lib.StormmError, match="time series holds 11"):
+            _lib.check(call())
+        st.scw.sample_capacity = 12
+        _lib.check(call())
+        _lib.check(L.stormm_device_synchronize())
+        series[name] = rows.get()
+    assert np.array_equal(series["kit"], series["noalloc"])
+    assert (series["kit"][:, :, 28] != 0).all()

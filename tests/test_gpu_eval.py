@@ -99,6 +99,40 @@ def test_ragged_and_extreme_sizes(prec):
     check_against_oracle(topos, res, prec)
 
 
+def without_terms(topo, kinds):
+    """A copy of a topology with every valence term of the given kinds removed."""
+    import copy
+    t = copy.copy(topo)
+    # Exclusions follow the bonds on both sides (the device derives them from the bonds).
+    t._excl_cache = None
+    if "bonds" in kinds:
+        t.bonds = np.zeros((0, 2), np.int32)
+        t.bond_k = t.bond_r0 = np.zeros(0)
+    if "angles" in kinds:
+        t.angles = np.zeros((0, 3), np.int32)
+        t.angle_k = t.angle_theta = np.zeros(0)
+    if "dihedrals" in kinds:
+        t.dihedrals = np.zeros((0, 4), np.int32)
+        t.dihe_amp = t.dihe_phase = t.dihe_freq = np.zeros(0)
+        t.dihe_improper = np.zeros(0, np.int32)
+        t.dihe_scee = t.dihe_scnb = np.zeros(0)
+    return t
+
+
+@pytest.mark.parametrize("prec", ["double", "single"])
+def test_systems_lacking_a_kind_of_valence_term(prec):
+    """Work units take valence terms kind by kind over their systems; systems without any
+    bonds, angles or dihedrals (in every combination, next to complete ones) must be skipped
+    correctly.  A system without bonds has no exclusions either, so its bonded neighbours
+    meet as non-bonded pairs at bond distances (large but finite energies)."""
+    topos, coords = synthetic.uniform_batch(14, 12, 60, seed=37)
+    drop = [(), ("dihedrals",), ("angles",), ("bonds",), ("angles", "dihedrals"),
+            ("bonds", "dihedrals"), ("bonds", "angles"), ("bonds", "angles", "dihedrals")]
+    cut = [without_terms(t, drop[i % len(drop)]) for i, t in enumerate(topos)]
+    res = evaluate(cut, coords, prec)
+    check_against_oracle(cut, res, prec)
+
+
 @pytest.mark.parametrize("prec", ["double", "single"])
 def test_separate_launches_equal_fused_launch(prec):
     topos, coords = synthetic.config1(nsys=24, seed=2)

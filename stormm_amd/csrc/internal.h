@@ -51,10 +51,10 @@ constexpr int kMaxEvalBlock = 1024;
 constexpr int kMaxEvalBlockDouble = 512;
 // Entries of each wavefront's LDS queue of near-field Born-integral pairs.  A wave drains 64
 // entries at a time whenever the queue holds at least 64 after a group of four rotation steps,
-// each of which adds at most 64.
+// each of which adds at most 64 (and stores its lanes without an entry just past the new
+// ones).
 constexpr int kNearQueue = 320;
-// Each wavefront's queue is followed by 64 slots that take the stores of lanes without an entry.
-constexpr int kNearQueueStride = kNearQueue + 64;
+constexpr int kNearQueueStride = kNearQueue;
 constexpr int kStageCounters = 3;  // work-item counters of the psi, derivative and valence stages
 
 // StateVariable slots, src/Potential/energy_enumerators.h:36-75.

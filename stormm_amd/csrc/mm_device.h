@@ -265,7 +265,7 @@ template <typename T> struct WuLds {
   int *work;                // per-stage work-item counters (kStageCounters)
   T *clash_d, *clash_r;     // per-system clash distance and sigma ratio of this evaluation
   int *clash_on;            // per-system flag: damped potentials apply
-  unsigned *nearq;          // kNearQueueStride entries per wavefront: the queue, then 64 spare
+  unsigned *nearq;          // kNearQueueStride entries per wavefront: its near-pair queue
 
   static __host__ __device__ size_t bytes(int maxa, int waves) {
     const size_t m = static_cast<size_t>(maxa);

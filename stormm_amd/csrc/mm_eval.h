@@ -448,26 +448,34 @@ __device__ __forceinline__ unsigned near_entry(int i, int j, bool near_i, bool n
 }
 
 // Appends the entries of the lanes with `need` set, in lane order; n is wave-uniform.  Every
-// lane stores, the lanes without an entry into a slot of their own past the queue's kNearQueue
-// entries (lane - below counts them), so the rotation step has no branch.
-__device__ __forceinline__ void nearq_push(unsigned *q, int &n, bool need, unsigned entry,
-                                           int lane) {
-  const unsigned long long m = __ballot(need);
+// lane stores: the lanes with an entry to n, n + 1, ..., the others, also in lane order, to
+// the slots after them, which later pushes overwrite (before a group of steps the queue holds
+// at most 63 entries and a group pushes at most 4 x 64, so every store stays inside the
+// queue's stride).  The step thus has no branch, which would split it and keep the rotated
+// operands from folding into their instructions.
+__device__ __forceinline__ void nearq_push(unsigned *q, int &n, bool near_i, bool near_j,
+                                           unsigned entry, int lane) {
+  // The mask is the union of the two directions' ballots (each a comparison's lane mask).
+  const bool need = near_i || near_j;
+  const unsigned long long m = __builtin_amdgcn_ballot_w64(near_i) |
+                               __builtin_amdgcn_ballot_w64(near_j);
   const int below = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(m >> 32),
                                               __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(m),
                                                                         0u));
-  q[(need ? n : (kNearQueue + lane - (2 * below))) + below] = entry;
-  n += __popcll(m);
+  const int count = __popcll(m);
+  q[n + (need ? below : (count + (lane - below)))] = entry;
+  n += count;
   __builtin_amdgcn_wave_barrier();
 }
 
 // The Born-integral and derivative stages run their rotation steps in groups of kStepGroup with
 // compile-time rotation counts inside a group; the j-side data move kStepGroup lanes between
 // groups, and the near-pair queue is drained after each group, taking 64 entries at a time from
-// its end.  Before a group the queue holds at most 63 entries, so kNearQueue = 63 + 4 * 64 + 1
-// suffices.
+// its end.  Before a group the queue holds at most 63 entries, so a group's last push starts at
+// entry 63 + (kStepGroup - 1) x 64 at most and stores its 64 lanes below that plus 64.
 constexpr int kStepGroup = 4;
-static_assert(kNearQueue >= 64 + (kStepGroup * 64), "near queue too short for a step group");
+static_assert(63 + ((kStepGroup - 1) * 64) + 64 <= kNearQueue,
+              "near queue too short for a step group");
 
 // Geometry of a queued pair, recomputed from the LDS coordinates.
 template <typename T> struct NearPair {
@@ -521,7 +529,8 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k, int val_chunks) {
   WuLds<T> &L = c.L;
   const int il = c.tid & 15;
   const int lane = c.tid & 63;
-  const int wave = c.tid >> 6;
+  // Wavefront-uniform, so the queue's base address stays in scalar registers.
+  const int wave = __builtin_amdgcn_readfirstlane(c.tid >> 6);
   unsigned *q = L.nearq + (wave * kNearQueueStride);
   int qn = 0;
   const double psi_scale = ldexp(1.0, kPsiBits);
@@ -571,7 +580,7 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k, int val_chunks) {
         const T tj = psi_series(x_j) * (si3 * invr4);
         acci -= far_i ? ti : T(0);
         accj = row_rotate(accj) - (far_j ? tj : T(0));
-        nearq_push(q, qn, !(far_i && far_j),
+        nearq_push(q, qn, !far_i, !far_j,
                    ei | static_cast<unsigned>(row_ror<K>(ej)) | (far_i ? 0u : kNearI) |
                        (far_j ? 0u : kNearJ),
                    lane);
@@ -992,7 +1001,8 @@ __device__ __forceinline__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
   WuLds<T> &L = c.L;
   const int il = c.tid & 15;
   const int lane = c.tid & 63;
-  const int wave = c.tid >> 6;
+  // Wavefront-uniform, so the queue's base address stays in scalar registers.
+  const int wave = __builtin_amdgcn_readfirstlane(c.tid >> 6);
   unsigned *q = L.nearq + (wave * kNearQueueStride);
   int qn = 0;
   const T frc_scale = T(ldexp(1.0, k.frc_bits));
@@ -1046,7 +1056,7 @@ __device__ __forceinline__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
         gxj = row_rotate(gxj) + fdx;
         gyj = row_rotate(gyj) + fdy;
         gzj = row_rotate(gzj) + fdz;
-        nearq_push(q, qn, !(far_i && far_j),
+        nearq_push(q, qn, !far_i, !far_j,
                    ei | static_cast<unsigned>(row_ror<K>(ej)) | (far_i ? 0u : kNearI) |
                        (far_j ? 0u : kNearJ),
                    lane);

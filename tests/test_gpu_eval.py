@@ -123,13 +123,20 @@ def without_terms(topo, kinds):
 def test_systems_lacking_a_kind_of_valence_term(prec):
     """Work units take valence terms kind by kind over their systems; systems without any
     bonds, angles or dihedrals (in every combination, next to complete ones) must be skipped
-    correctly.  A system without bonds has no exclusions either, so its bonded neighbours
-    meet as non-bonded pairs at bond distances (large but finite energies)."""
+    correctly.  A system without bonds has no exclusions either, so its atoms are spread to
+    twice their distances from the centroid: its former bond partners then meet as ordinary
+    non-bonded pairs at 2-3 A, with forces inside the range of the fixed-precision
+    accumulators (40 bits in double precision hold up to 2^23 kcal/mol/A)."""
     topos, coords = synthetic.uniform_batch(14, 12, 60, seed=37)
     drop = [(), ("dihedrals",), ("angles",), ("bonds",), ("angles", "dihedrals"),
             ("bonds", "dihedrals"), ("bonds", "angles"), ("bonds", "angles", "dihedrals")]
-    cut = [without_terms(t, drop[i % len(drop)]) for i, t in enumerate(topos)]
-    res = evaluate(cut, coords, prec)
+    cut, xyz = [], []
+    for i, (t, x) in enumerate(zip(topos, coords)):
+        kinds = drop[i % len(drop)]
+        cut.append(without_terms(t, kinds))
+        x = np.asarray(x, dtype=np.float64)
+        xyz.append(x.mean(axis=0) + 2.0 * (x - x.mean(axis=0)) if "bonds" in kinds else x)
+    res = evaluate(cut, xyz, prec)
     check_against_oracle(cut, res, prec)
 
 

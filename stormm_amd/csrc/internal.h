@@ -69,9 +69,11 @@ enum TermMask : int { kTermNonbonded = 1, kTermValence = 2 };
 // Valence term kinds (VTerm.x).
 enum VKind : int { kVBond = 0, kVAngle = 1, kVDihedral = 2 };
 
-// Work-unit descriptor, 8 ints.  The unit's tile tasks are the off-diagonal tasks of all of its
-// systems followed by their diagonal tasks (so that wavefronts given only diagonal tiles run the
-// short rotation); its valence terms are those of its systems in order.
+// Work-unit descriptor, 8 ints.  The unit's tile tasks are the clean off-diagonal tasks of all of
+// its systems, then their other off-diagonal tasks, then their diagonal tasks (so that most
+// wavefronts hold four tasks of one kind: clean tiles run steps without pair masks, and
+// wavefronts given only diagonal tiles run the short rotation); its valence terms are those of
+// its systems in order.
 struct WuDesc {
   int sys_first;     // index into the work-unit system list
   int sys_count;
@@ -91,8 +93,9 @@ struct WuSys {
   int lds_base;      // first LDS atom of the system in the unit
   int global_start;  // first atom in the global per-atom arrays
   int natom;
-  int task_first;    // template tile tasks
-  int offd_count;
+  int task_first;    // template tile tasks: clean off-diagonal, other off-diagonal, diagonal
+  int offd_count;    // off-diagonal tasks, the first clean_count of them clean (16 x 16 atoms,
+  int clean_count;   // no excluded pair)
   int diag_count;
   int term_first;    // template valence terms: bonds, then angles, then dihedrals
   int term_count;

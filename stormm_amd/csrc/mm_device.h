@@ -130,6 +130,13 @@ __device__ __forceinline__ void unrolled_steps(F &&f, std::integer_sequence<int,
   ((f(std::integral_constant<int, K>{}), __builtin_amdgcn_sched_barrier(0)), ...);
 }
 
+// Makes x final at this point: an empty statement that reads and rewrites its register, so that
+// the arithmetic producing x is not moved below it (into a later block, where the rotated
+// operands that arithmetic reads would each need a separate move).
+template <typename T> __device__ __forceinline__ void settle(T &x) {
+  asm volatile("" : "+v"(x));
+}
+
 // Sum of a 64-bit integer over the 16 lanes of a row.
 __device__ __forceinline__ long long row_sum(long long v) {
   v += __shfl_xor(v, 8, 16);
@@ -163,12 +170,16 @@ __device__ __forceinline__ void lds_add(long long *addr, long long v) {
 // Squared distance given to an absent pair in the Born-integral and derivative tile loops: far
 // beyond 4 screened radii of any atom, with 1 / r^4 below the smallest denormal, so both far-field
 // terms round to exactly zero and the pair never enters the near-field queue.
+// kCrd is the coordinate (on all three axes, +kCrd for an absent i atom, -kCrd for an absent j
+// atom) that puts an absent atom of a tile as far away, with or without a partner present.
 template <typename T> struct FarAway;
 template <> struct FarAway<float> {
   static constexpr float kR2 = 1.0e30f;
+  static constexpr float kCrd = 1.0e15f;
 };
 template <> struct FarAway<double> {
   static constexpr double kR2 = 1.0e200;
+  static constexpr double kCrd = 1.0e100;
 };
 
 // Far-field series in x = (s / r)^2: the Born-integral term is -s^3 / r^4 times psi_series(x),
@@ -260,7 +271,8 @@ template <typename T> struct WuLds {
   crd_t *cx, *cy, *cz;
   int *gidx, *slot;
   WuSys *sys;
-  int *opre, *dpre;         // per slot: first off-diagonal task, first diagonal task
+  int *cpre, *opre, *dpre;  // per slot: first clean off-diagonal task, first other off-diagonal
+                            // task (counted among those), first diagonal task
   int *vpre;                // per kind (bonds, angles, dihedrals): each slot's first term
   int *work;                // per-stage work-item counters (kStageCounters)
   T *clash_d, *clash_r;     // per-system clash distance and sigma ratio of this evaluation
@@ -274,7 +286,7 @@ template <typename T> struct WuLds {
            (3 * kMaxWuSystems * sizeof(long long)) + (kMaxWuSystems * sizeof(double)) +
            (9 * m * sizeof(T)) +
            (3 * m * sizeof(crd_t)) + (2 * m * sizeof(int)) + (kMaxWuSystems * sizeof(WuSys)) +
-           (5 * (kMaxWuSystems + 1) * sizeof(int)) + (kStageCounters * sizeof(int)) +
+           (6 * (kMaxWuSystems + 1) * sizeof(int)) + (kStageCounters * sizeof(int)) +
            (2 * kMaxWuSystems * sizeof(T)) + (kMaxWuSystems * sizeof(int)) +
            (static_cast<size_t>(waves) * kNearQueueStride * sizeof(unsigned));
   }
@@ -313,6 +325,7 @@ template <typename T> struct WuLds {
     gidx = reinterpret_cast<int *>(take(m * sizeof(int)));
     slot = reinterpret_cast<int *>(take(m * sizeof(int)));
     sys = reinterpret_cast<WuSys *>(take(kMaxWuSystems * sizeof(WuSys)));
+    cpre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
     opre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
     dpre = reinterpret_cast<int *>(take((kMaxWuSystems + 1) * sizeof(int)));
     vpre = reinterpret_cast<int *>(take(3 * (kMaxWuSystems + 1) * sizeof(int)));

@@ -61,20 +61,23 @@ __device__ __forceinline__ void stage_load_params(StageCtx<T> &c, const MMKit &k
     // Valence terms are taken kind by kind (all bonds of the unit, then all angles, then all
     // dihedrals), so that a chunk of 64 runs one kind's code.
     constexpr int K1 = kMaxWuSystems + 1;
-    int o = 0, d = 0, vb = 0, va = 0, vd = 0;
+    int cl = 0, o = 0, d = 0, vb = 0, va = 0, vd = 0;
     for (int s = 0; s < wu.sys_count; s++) {
       const WuSys &ws = L.sys[s];
+      L.cpre[s] = cl;
       L.opre[s] = o;
       L.dpre[s] = d;
       L.vpre[s] = vb;
       L.vpre[K1 + s] = va;
       L.vpre[2 * K1 + s] = vd;
-      o += ws.offd_count;
+      cl += ws.clean_count;
+      o += ws.offd_count - ws.clean_count;
       d += ws.diag_count;
       vb += ws.bond_count;
       va += ws.angl_count;
       vd += ws.term_count - ws.bond_count - ws.angl_count;
     }
+    L.cpre[wu.sys_count] = cl;
     L.opre[wu.sys_count] = o;
     L.dpre[wu.sys_count] = d;
     L.vpre[wu.sys_count] = vb;
@@ -82,13 +85,20 @@ __device__ __forceinline__ void stage_load_params(StageCtx<T> &c, const MMKit &k
     L.vpre[2 * K1 + wu.sys_count] = vd;
   }
   __syncthreads();
-  // The unit's tile tasks, read once per evaluation: off-diagonal tasks of every system, then
-  // the diagonal ones, each with its system's LDS base added and its slot in the low bits of y.
+  // The unit's tile tasks, read once per evaluation: the clean off-diagonal tasks of every
+  // system, then their other off-diagonal tasks, then the diagonal ones, each with its system's
+  // LDS base added and its slot in the low bits of y.
+  const int nclean = L.cpre[wu.sys_count];
   for (int t = c.tid; t < wu.task_count; t += c.nthr) {
     int s, idx;
-    if (t < wu.offd_count) {
-      s = find_slot(L.opre, wu.sys_count, t);
-      idx = L.sys[s].task_first + (t - L.opre[s]);
+    if (t < nclean) {
+      s = find_slot(L.cpre, wu.sys_count, t);
+      idx = L.sys[s].task_first + (t - L.cpre[s]);
+    }
+    else if (t < wu.offd_count) {
+      const int to = t - nclean;
+      s = find_slot(L.opre, wu.sys_count, to);
+      idx = L.sys[s].task_first + L.sys[s].clean_count + (to - L.opre[s]);
     }
     else {
       const int td = t - wu.offd_count;
@@ -346,7 +356,10 @@ struct TileTask {
   }
 };
 
-// Coordinates of lane il's i atom and first j atom of a tile, in Angstroms.
+// Coordinates of lane il's i atom and first j atom of a tile, in Angstroms.  The padding slots
+// of a partial tile (lanes past ni on the i side, past nj on the j side) are put far away, so
+// that their pairs are far-field pairs with terms of exactly zero in the Born-integral and
+// derivative loops, which need no pair mask on off-diagonal tiles.
 template <typename T>
 __device__ __forceinline__ void tile_coords(const StageCtx<T> &c, const TileTask &tt, T &xi,
                                             T &yi, T &zi, T &xj, T &yj, T &zj) {
@@ -354,12 +367,18 @@ __device__ __forceinline__ void tile_coords(const StageCtx<T> &c, const TileTask
   const auto ox = tile_centre(L.cx[tt.ib], L.cx[tt.jb]);
   const auto oy = tile_centre(L.cy[tt.ib], L.cy[tt.jb]);
   const auto oz = tile_centre(L.cz[tt.ib], L.cz[tt.jb]);
-  xi = tile_crd(L.cx[tt.i], ox, c.crd_scale);
-  yi = tile_crd(L.cy[tt.i], oy, c.crd_scale);
-  zi = tile_crd(L.cz[tt.i], oz, c.crd_scale);
-  xj = tile_crd(L.cx[tt.j0], ox, c.crd_scale);
-  yj = tile_crd(L.cy[tt.j0], oy, c.crd_scale);
-  zj = tile_crd(L.cz[tt.j0], oz, c.crd_scale);
+  const bool jvalid = (c.tid & 15) < tt.nj;
+  xi = tt.ivalid ? tile_crd(L.cx[tt.i], ox, c.crd_scale) : FarAway<T>::kCrd;
+  yi = tt.ivalid ? tile_crd(L.cy[tt.i], oy, c.crd_scale) : FarAway<T>::kCrd;
+  zi = tt.ivalid ? tile_crd(L.cz[tt.i], oz, c.crd_scale) : FarAway<T>::kCrd;
+  xj = jvalid ? tile_crd(L.cx[tt.j0], ox, c.crd_scale) : -FarAway<T>::kCrd;
+  yj = jvalid ? tile_crd(L.cy[tt.j0], oy, c.crd_scale) : -FarAway<T>::kCrd;
+  zj = jvalid ? tile_crd(L.cz[tt.j0], oz, c.crd_scale) : -FarAway<T>::kCrd;
+}
+
+// True on every lane when `b` holds on every active lane of the wavefront.
+__device__ __forceinline__ bool wave_all(bool b) {
+  return __ballot(!b) == 0ULL;
 }
 
 __device__ __forceinline__ bool step_bit(unsigned mask, int ks) {
@@ -556,47 +575,61 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k, int val_chunks) {
     int ej = tt.j0 << 9;
     T acci = T(0), accj = T(0);
     unsigned vbits = tt.valid;
-    for (int g = 0; g < tt.nsteps; g += kStepGroup) {
-      auto step = [&](auto kc) {
-        constexpr int K = decltype(kc)::value;
-        if (g + K >= tt.nsteps) {
-          return;
+    // Off-diagonal tiles need no pair mask (their padding atoms sit far away); a wavefront whose
+    // four tiles are all off-diagonal runs the 16 steps without one.  On diagonal tiles the mask
+    // removes each lane's own atom and the second copy of each pair.
+    auto run = [&](auto mc) {
+      constexpr bool MASKED = decltype(mc)::value;
+      const int nsteps = MASKED ? tt.nsteps : kTile;
+      for (int g = 0; g < nsteps; g += kStepGroup) {
+        auto step = [&](auto kc) {
+          constexpr int K = decltype(kc)::value;
+          if (MASKED && g + K >= nsteps) {
+            return;
+          }
+          // An absent pair is put far away: both directions are then far-field with terms
+          // that round to exactly zero, so no validity test follows.
+          const bool valid = !MASKED || (((vbits >> K) & 1u) != 0u);
+          const T dx = row_ror<K>(xj) - xi;
+          const T dy = row_ror<K>(yj) - yi;
+          const T dz = row_ror<K>(zj) - zi;
+          const T r2c = (dx * dx) + (dy * dy) + (dz * dz);
+          const T r2 = valid ? r2c : FarAway<T>::kR2;
+          const T invr2 = P::rcp(r2);
+          const T invr4 = invr2 * invr2;
+          const T x_i = row_ror<K>(sj2) * invr2;
+          const T x_j = si2 * invr2;
+          const bool far_i = x_i < T(0.0625);
+          const bool far_j = x_j < T(0.0625);
+          const T ti = psi_series(x_i) * (row_ror<K>(sj3) * invr4);
+          const T tj = psi_series(x_j) * (si3 * invr4);
+          acci -= far_i ? ti : T(0);
+          accj = row_rotate(accj) - (far_j ? tj : T(0));
+          nearq_push(q, qn, !far_i, !far_j,
+                     ei | static_cast<unsigned>(row_ror<K>(ej)) | (far_i ? 0u : kNearI) |
+                         (far_j ? 0u : kNearJ),
+                     lane);
+        };
+        unrolled_steps(step, std::integer_sequence<int, 0, 1, 2, 3>{});
+        settle(acci);
+        vbits >>= kStepGroup;
+        xj = row_ror<kStepGroup>(xj);
+        yj = row_ror<kStepGroup>(yj);
+        zj = row_ror<kStepGroup>(zj);
+        sj2 = row_ror<kStepGroup>(sj2);
+        sj3 = row_ror<kStepGroup>(sj3);
+        ej = row_ror<kStepGroup>(ej);
+        while (qn >= 64) {
+          psi_drain(c, q + (qn - 64), 64, lane);
+          qn -= 64;
         }
-        // An absent pair is put far away: both directions are then far-field with terms
-        // that round to exactly zero, so no validity test follows.
-        const bool valid = ((vbits >> K) & 1u) != 0u;
-        const T dx = row_ror<K>(xj) - xi;
-        const T dy = row_ror<K>(yj) - yi;
-        const T dz = row_ror<K>(zj) - zi;
-        const T r2c = (dx * dx) + (dy * dy) + (dz * dz);
-        const T r2 = valid ? r2c : FarAway<T>::kR2;
-        const T invr2 = P::rcp(r2);
-        const T invr4 = invr2 * invr2;
-        const T x_i = row_ror<K>(sj2) * invr2;
-        const T x_j = si2 * invr2;
-        const bool far_i = x_i < T(0.0625);
-        const bool far_j = x_j < T(0.0625);
-        const T ti = psi_series(x_i) * (row_ror<K>(sj3) * invr4);
-        const T tj = psi_series(x_j) * (si3 * invr4);
-        acci -= far_i ? ti : T(0);
-        accj = row_rotate(accj) - (far_j ? tj : T(0));
-        nearq_push(q, qn, !far_i, !far_j,
-                   ei | static_cast<unsigned>(row_ror<K>(ej)) | (far_i ? 0u : kNearI) |
-                       (far_j ? 0u : kNearJ),
-                   lane);
-      };
-      unrolled_steps(step, std::integer_sequence<int, 0, 1, 2, 3>{});
-      vbits >>= kStepGroup;
-      xj = row_ror<kStepGroup>(xj);
-      yj = row_ror<kStepGroup>(yj);
-      zj = row_ror<kStepGroup>(zj);
-      sj2 = row_ror<kStepGroup>(sj2);
-      sj3 = row_ror<kStepGroup>(sj3);
-      ej = row_ror<kStepGroup>(ej);
-      while (qn >= 64) {
-        psi_drain(c, q + (qn - 64), 64, lane);
-        qn -= 64;
       }
+    };
+    if (wave_all(!tt.diag)) {
+      run(std::false_type{});
+    }
+    else {
+      run(std::true_type{});
     }
     if (tt.ivalid) {
       lds_add(&L.psi[tt.i], Prec<double>::fixed(static_cast<double>(acci) * psi_scale));
@@ -780,128 +813,151 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
     // wavefront's rows can differ): they rotate every step to stay within the register budget.
     constexpr int kGroup = CLASH ? 1 : kStepGroup;
     unsigned vbits = tt.valid, nbbits = nbmask;
-    for (int g = 0; g < tt.nsteps; g += kGroup) {
-      auto step = [&](auto kc) {
-        constexpr int K = decltype(kc)::value;
-        if (g + K >= tt.nsteps) {
-          return;
-        }
-        const bool valid = ((vbits >> K) & 1u) != 0u;
-        const bool nbok = ((nbbits >> K) & 1u) != 0u;
-        const T dx = row_ror<K>(xj) - xi;
-        const T dy = row_ror<K>(yj) - yi;
-        const T dz = row_ror<K>(zj) - zi;
-        const T r2c = (dx * dx) + (dy * dy) + (dz * dz);
-        const T r2 = valid ? r2c : T(1);
-        const T qiqj = row_ror<K>(qj) * qi;
-        T fmag = T(0);
+    // A wavefront whose four tiles are all clean (off-diagonal, 16 x 16 atoms, no excluded pair)
+    // runs the 16 steps without pair masks.
+    auto run = [&](auto mc, auto kc) {
+      constexpr bool MASKED = decltype(mc)::value;
+      constexpr bool KAPPA = decltype(kc)::value;
+      const int nsteps = MASKED ? tt.nsteps : kTile;
+      for (int g = 0; g < nsteps; g += kGroup) {
+        auto step = [&](auto sc) {
+          constexpr int K = decltype(sc)::value;
+          if (MASKED && g + K >= nsteps) {
+            return;
+          }
+          const bool valid = !MASKED || (((vbits >> K) & 1u) != 0u);
+          const bool nbok = !MASKED || (((nbbits >> K) & 1u) != 0u);
+          const T dx = row_ror<K>(xj) - xi;
+          const T dy = row_ror<K>(yj) - yi;
+          const T dz = row_ror<K>(zj) - zi;
+          const T r2c = (dx * dx) + (dy * dy) + (dz * dz);
+          const T r2 = valid ? r2c : T(1);
+          const T qiqj = row_ror<K>(qj) * qi;
+          T fmag = T(0);
+          if (GB) {
+            const T qgb = valid ? qiqj : T(0);
+            const T rr = row_ror<K>(Rj) * Ri;
+            const T efac = GbExp<T>::eval(r2 * (row_ror<K>(iRj) * iRi));
+            const T fgb2 = r2 + (rr * efac);
+            const T fgbi = P::rsqrt(fgb2);
+            // egb = qq (1 - exp(-kappa fgb) / diel) / fgb, temp6 = -(dE/dfgb) / fgb.
+            T egb, temp6;
+            if (KAPPA) {
+              const T fgbk = -kap * fgb2 * fgbi;
+              const T expmkf = P::exp(fgbk) * inv_diel;
+              const T dielfac = T(1) - expmkf;
+              const T qf = qgb * fgbi;
+              egb = qf * dielfac;
+              temp6 = qf * fgbi * fgbi * (dielfac + (fgbk * expmkf));
+            }
+            else {
+              egb = qgb * fgbi * dielfac0;
+              temp6 = egb * fgbi * fgbi;
+            }
+            e_gb -= egb;
+            if (FORCES) {
+              fmag += temp6 * (T(1) - (T(0.25) * efac));
+              const T temp5 = T(0.5) * efac * temp6 * (rr + (T(0.25) * r2));
+              dri += temp5;
+              drj = row_rotate(drj) + temp5;
+            }
+          }
+          const T qnb = nbok ? qiqj : T(0);
+          if (CLASH && clash_on) {
+            const T pj1 = row_ror<K>(p1j);
+            const T pj2 = row_ror<K>(p2j);
+            T lja, ljb, sigma;
+            if (LJTABLE) {
+              const double2 ab = ljrow[static_cast<int>(pj1)];
+              lja = nbok ? static_cast<T>(ab.x) : T(0);
+              ljb = nbok ? static_cast<T>(ab.y) : T(0);
+              sigma = (ljb > T(1.0e-6)) ? P::sqrt(P::cbrt(lja / ljb)) : T(0);
+            }
+            else {
+              const T sig = p1i + pj1;
+              const T sig2 = sig * sig;
+              const T sig6 = sig2 * sig2 * sig2;
+              const T e4 = nbok ? p2i * pj2 : T(0);
+              ljb = e4 * sig6;
+              lja = ljb * sig6;
+              sigma = (ljb > T(1.0e-6)) ? sig : T(0);
+            }
+            clash_pair<T, FORCES>(P::sqrt(r2), qnb, lja, ljb, sigma, cd, ca, cb, cratio, e_qq, e_lj,
+                                  fmag);
+          }
+          else {
+            const T invr = P::rsqrt(r2);
+            const T invr2 = invr * invr;
+            const T eqq = qnb * invr;
+            e_qq += eqq;
+            if (LJTABLE) {
+              const double2 ab = ljrow[static_cast<int>(row_ror<K>(p1j))];
+              const T lja = nbok ? static_cast<T>(ab.x) : T(0);
+              const T ljb = nbok ? static_cast<T>(ab.y) : T(0);
+              const T invr4 = invr2 * invr2;
+              e_lj += ((lja * invr4 * invr4) - (ljb * invr2)) * invr4;
+              if (FORCES) {
+                fmag += ((T(6) * ljb) - (T(12) * lja * invr4 * invr2)) * invr4 * invr4;
+              }
+            }
+            else {
+              const T sig = row_ror<K>(p1j) + p1i;
+              const T s2 = sig * sig * invr2;
+              const T s6 = s2 * s2 * s2;
+              const T e4c = row_ror<K>(p2j) * p2i;
+              const T e4 = nbok ? e4c : T(0);
+              const T e4s6 = e4 * s6;
+              e_lj += e4s6 * (s6 - T(1));
+              if (FORCES) {
+                fmag += e4s6 * (T(6) - (T(12) * s6)) * invr2;
+              }
+            }
+            if (FORCES) {
+              fmag -= eqq * invr2;
+            }
+          }
+          if (FORCES) {
+            const T fdx = fmag * dx;
+            const T fdy = fmag * dy;
+            const T fdz = fmag * dz;
+            fxi += fdx;
+            fyi += fdy;
+            fzi += fdz;
+            fxj = row_rotate(fxj) - fdx;
+            fyj = row_rotate(fyj) - fdy;
+            fzj = row_rotate(fzj) - fdz;
+          }
+        };
+        unrolled_steps(step, std::make_integer_sequence<int, kGroup>{});
+        vbits >>= kGroup;
+        nbbits >>= kGroup;
+        xj = row_ror<kGroup>(xj);
+        yj = row_ror<kGroup>(yj);
+        zj = row_ror<kGroup>(zj);
+        qj = row_ror<kGroup>(qj);
+        p1j = row_ror<kGroup>(p1j);
+        p2j = row_ror<kGroup>(p2j);
         if (GB) {
-          const T qgb = valid ? qiqj : T(0);
-          const T rr = row_ror<K>(Rj) * Ri;
-          const T efac = GbExp<T>::eval(r2 * (row_ror<K>(iRj) * iRi));
-          const T fgb2 = r2 + (rr * efac);
-          const T fgbi = P::rsqrt(fgb2);
-          // egb = qq (1 - exp(-kappa fgb) / diel) / fgb, temp6 = -(dE/dfgb) / fgb.
-          T egb, temp6;
-          if (use_kappa) {
-            const T fgbk = -kap * fgb2 * fgbi;
-            const T expmkf = P::exp(fgbk) * inv_diel;
-            const T dielfac = T(1) - expmkf;
-            const T qf = qgb * fgbi;
-            egb = qf * dielfac;
-            temp6 = qf * fgbi * fgbi * (dielfac + (fgbk * expmkf));
-          }
-          else {
-            egb = qgb * fgbi * dielfac0;
-            temp6 = egb * fgbi * fgbi;
-          }
-          e_gb -= egb;
-          if (FORCES) {
-            fmag += temp6 * (T(1) - (T(0.25) * efac));
-            const T temp5 = T(0.5) * efac * temp6 * (rr + (T(0.25) * r2));
-            dri += temp5;
-            drj = row_rotate(drj) + temp5;
-          }
+          Rj = row_ror<kGroup>(Rj);
+          iRj = row_ror<kGroup>(iRj);
         }
-        const T qnb = nbok ? qiqj : T(0);
-        if (CLASH && clash_on) {
-          const T pj1 = row_ror<K>(p1j);
-          const T pj2 = row_ror<K>(p2j);
-          T lja, ljb, sigma;
-          if (LJTABLE) {
-            const double2 ab = ljrow[static_cast<int>(pj1)];
-            lja = nbok ? static_cast<T>(ab.x) : T(0);
-            ljb = nbok ? static_cast<T>(ab.y) : T(0);
-            sigma = (ljb > T(1.0e-6)) ? P::sqrt(P::cbrt(lja / ljb)) : T(0);
-          }
-          else {
-            const T sig = p1i + pj1;
-            const T sig2 = sig * sig;
-            const T sig6 = sig2 * sig2 * sig2;
-            const T e4 = nbok ? p2i * pj2 : T(0);
-            ljb = e4 * sig6;
-            lja = ljb * sig6;
-            sigma = (ljb > T(1.0e-6)) ? sig : T(0);
-          }
-          clash_pair<T, FORCES>(P::sqrt(r2), qnb, lja, ljb, sigma, cd, ca, cb, cratio, e_qq, e_lj,
-                                fmag);
-        }
-        else {
-          const T invr = P::rsqrt(r2);
-          const T invr2 = invr * invr;
-          const T eqq = qnb * invr;
-          e_qq += eqq;
-          if (LJTABLE) {
-            const double2 ab = ljrow[static_cast<int>(row_ror<K>(p1j))];
-            const T lja = nbok ? static_cast<T>(ab.x) : T(0);
-            const T ljb = nbok ? static_cast<T>(ab.y) : T(0);
-            const T invr4 = invr2 * invr2;
-            e_lj += ((lja * invr4 * invr4) - (ljb * invr2)) * invr4;
-            if (FORCES) {
-              fmag += ((T(6) * ljb) - (T(12) * lja * invr4 * invr2)) * invr4 * invr4;
-            }
-          }
-          else {
-            const T sig = row_ror<K>(p1j) + p1i;
-            const T s2 = sig * sig * invr2;
-            const T s6 = s2 * s2 * s2;
-            const T e4c = row_ror<K>(p2j) * p2i;
-            const T e4 = nbok ? e4c : T(0);
-            const T e4s6 = e4 * s6;
-            e_lj += e4s6 * (s6 - T(1));
-            if (FORCES) {
-              fmag += e4s6 * (T(6) - (T(12) * s6)) * invr2;
-            }
-          }
-          if (FORCES) {
-            fmag -= eqq * invr2;
-          }
-        }
-        if (FORCES) {
-          const T fdx = fmag * dx;
-          const T fdy = fmag * dy;
-          const T fdz = fmag * dz;
-          fxi += fdx;
-          fyi += fdy;
-          fzi += fdz;
-          fxj = row_rotate(fxj) - fdx;
-          fyj = row_rotate(fyj) - fdy;
-          fzj = row_rotate(fzj) - fdz;
-        }
-      };
-      unrolled_steps(step, std::make_integer_sequence<int, kGroup>{});
-      vbits >>= kGroup;
-      nbbits >>= kGroup;
-      xj = row_ror<kGroup>(xj);
-      yj = row_ror<kGroup>(yj);
-      zj = row_ror<kGroup>(zj);
-      qj = row_ror<kGroup>(qj);
-      p1j = row_ror<kGroup>(p1j);
-      p2j = row_ror<kGroup>(p2j);
-      if (GB) {
-        Rj = row_ror<kGroup>(Rj);
-        iRj = row_ror<kGroup>(iRj);
       }
+    };
+    // The kappa choice is made here too, so that no branch splits the steps.
+    const bool clean = wave_all(!tt.diag && tt.ni == kTile && tt.nj == kTile && tk.z < 0);
+    if (GB && use_kappa) {
+      if (clean) {
+        run(std::false_type{}, std::true_type{});
+      }
+      else {
+        run(std::true_type{}, std::true_type{});
+      }
+    }
+    else if (clean) {
+      run(std::false_type{}, std::false_type{});
+    }
+    else {
+      run(std::true_type{}, std::false_type{});
     }
     if (FORCES) {
       if (tt.ivalid) {
@@ -1023,57 +1079,71 @@ __device__ __forceinline__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
     T fxi = T(0), fyi = T(0), fzi = T(0);
     T gxj = T(0), gyj = T(0), gzj = T(0);
     unsigned vbits = tt.valid;
-    for (int g = 0; g < tt.nsteps; g += kStepGroup) {
-      auto step = [&](auto kc) {
-        constexpr int K = decltype(kc)::value;
-        if (g + K >= tt.nsteps) {
-          return;
+    // Wavefronts of four off-diagonal tiles run without a pair mask, as in stage_psi.
+    auto run = [&](auto mc) {
+      constexpr bool MASKED = decltype(mc)::value;
+      const int nsteps = MASKED ? tt.nsteps : kTile;
+      for (int g = 0; g < nsteps; g += kStepGroup) {
+        auto step = [&](auto kc) {
+          constexpr int K = decltype(kc)::value;
+          if (MASKED && g + K >= nsteps) {
+            return;
+          }
+          // Absent pairs are put far away, as in stage_psi.
+          const bool valid = !MASKED || (((vbits >> K) & 1u) != 0u);
+          const T dx = row_ror<K>(xj) - xi;
+          const T dy = row_ror<K>(yj) - yi;
+          const T dz = row_ror<K>(zj) - zi;
+          const T r2c = (dx * dx) + (dy * dy) + (dz * dz);
+          const T r2 = valid ? r2c : FarAway<T>::kR2;
+          const T invr2 = P::rcp(r2);
+          const T invr6 = invr2 * invr2 * invr2;
+          const T x_i = row_ror<K>(sj2) * invr2;
+          const T x_j = si2 * invr2;
+          const bool far_i = x_i < T(0.0625);
+          const bool far_j = x_j < T(0.0625);
+          // Both series are evaluated on every lane and selected, which keeps the step free of
+          // branches.
+          const T ti = dpsi_series(x_i) * (row_ror<K>(sj3) * (invr6 * sdi));
+          const T tj = dpsi_series(x_j) * (row_ror<K>(sdj) * (invr6 * si3));
+          const T fmag = (far_i ? ti : T(0)) + (far_j ? tj : T(0));
+          const T fdx = fmag * dx;
+          const T fdy = fmag * dy;
+          const T fdz = fmag * dz;
+          fxi -= fdx;
+          fyi -= fdy;
+          fzi -= fdz;
+          gxj = row_rotate(gxj) + fdx;
+          gyj = row_rotate(gyj) + fdy;
+          gzj = row_rotate(gzj) + fdz;
+          nearq_push(q, qn, !far_i, !far_j,
+                     ei | static_cast<unsigned>(row_ror<K>(ej)) | (far_i ? 0u : kNearI) |
+                         (far_j ? 0u : kNearJ),
+                     lane);
+        };
+        unrolled_steps(step, std::integer_sequence<int, 0, 1, 2, 3>{});
+        settle(fxi);
+        settle(fyi);
+        settle(fzi);
+        vbits >>= kStepGroup;
+        xj = row_ror<kStepGroup>(xj);
+        yj = row_ror<kStepGroup>(yj);
+        zj = row_ror<kStepGroup>(zj);
+        sj2 = row_ror<kStepGroup>(sj2);
+        sj3 = row_ror<kStepGroup>(sj3);
+        sdj = row_ror<kStepGroup>(sdj);
+        ej = row_ror<kStepGroup>(ej);
+        while (qn >= 64) {
+          deriv_drain(c, q + (qn - 64), 64, lane, k.frc_bits);
+          qn -= 64;
         }
-        // Absent pairs are put far away, as in stage_psi.
-        const bool valid = ((vbits >> K) & 1u) != 0u;
-        const T dx = row_ror<K>(xj) - xi;
-        const T dy = row_ror<K>(yj) - yi;
-        const T dz = row_ror<K>(zj) - zi;
-        const T r2c = (dx * dx) + (dy * dy) + (dz * dz);
-        const T r2 = valid ? r2c : FarAway<T>::kR2;
-        const T invr2 = P::rcp(r2);
-        const T invr6 = invr2 * invr2 * invr2;
-        const T x_i = row_ror<K>(sj2) * invr2;
-        const T x_j = si2 * invr2;
-        const bool far_i = x_i < T(0.0625);
-        const bool far_j = x_j < T(0.0625);
-        // Both series are evaluated on every lane and selected, which keeps the step free of
-        // branches.
-        const T ti = dpsi_series(x_i) * (row_ror<K>(sj3) * (invr6 * sdi));
-        const T tj = dpsi_series(x_j) * (row_ror<K>(sdj) * (invr6 * si3));
-        const T fmag = (far_i ? ti : T(0)) + (far_j ? tj : T(0));
-        const T fdx = fmag * dx;
-        const T fdy = fmag * dy;
-        const T fdz = fmag * dz;
-        fxi -= fdx;
-        fyi -= fdy;
-        fzi -= fdz;
-        gxj = row_rotate(gxj) + fdx;
-        gyj = row_rotate(gyj) + fdy;
-        gzj = row_rotate(gzj) + fdz;
-        nearq_push(q, qn, !far_i, !far_j,
-                   ei | static_cast<unsigned>(row_ror<K>(ej)) | (far_i ? 0u : kNearI) |
-                       (far_j ? 0u : kNearJ),
-                   lane);
-      };
-      unrolled_steps(step, std::integer_sequence<int, 0, 1, 2, 3>{});
-      vbits >>= kStepGroup;
-      xj = row_ror<kStepGroup>(xj);
-      yj = row_ror<kStepGroup>(yj);
-      zj = row_ror<kStepGroup>(zj);
-      sj2 = row_ror<kStepGroup>(sj2);
-      sj3 = row_ror<kStepGroup>(sj3);
-      sdj = row_ror<kStepGroup>(sdj);
-      ej = row_ror<kStepGroup>(ej);
-      while (qn >= 64) {
-        deriv_drain(c, q + (qn - 64), 64, lane, k.frc_bits);
-        qn -= 64;
       }
+    };
+    if (wave_all(!tt.diag)) {
+      run(std::false_type{});
+    }
+    else {
+      run(std::true_type{});
     }
     if (tt.ivalid) {
       lds_add(&L.fx[tt.i], P::fixed(fxi * frc_scale));

@@ -224,14 +224,15 @@ static int add_template(const stormm_topology_batch *b, const SystemTerms &t, in
       }
     }
   }
-  // Tile tasks: off-diagonal tiles first, then the diagonal ones.
+  // Tile tasks: the clean off-diagonal tiles (16 x 16 atoms, no excluded pair) first, then the
+  // other off-diagonal tiles, then the diagonal ones.
   const int nt = round_up(n, kTile) / kTile;
   tp.task_first = static_cast<int>(P.tasks.size());
-  for (int pass = 0; pass < 2; pass++) {
+  for (int pass = 0; pass < 3; pass++) {
     for (int ti = 0; ti < nt; ti++) {
       for (int tj = ti; tj < nt; tj++) {
         const bool diag = (ti == tj);
-        if (diag != (pass == 1)) {
+        if (diag != (pass == 2)) {
           continue;
         }
         const int ni = std::min(kTile, n - ti * kTile);
@@ -250,6 +251,9 @@ static int add_template(const stormm_topology_batch *b, const SystemTerms &t, in
             }
           }
         }
+        if (!diag && ((pass == 0) != (!any && ni == kTile && nj == kTile))) {
+          continue;
+        }
         Task tk;
         tk.x = (ti * kTile) | ((tj * kTile) << 16);
         tk.y = ((diag ? 1 : 0) << 8) | (ni << 12) | (nj << 20);
@@ -267,6 +271,7 @@ static int add_template(const stormm_topology_batch *b, const SystemTerms &t, in
         }
         else {
           tp.offd_count++;
+          tp.clean_count += (pass == 0) ? 1 : 0;
         }
       }
     }
@@ -783,6 +788,7 @@ int append_work_unit(const std::vector<int> &systems, int lj_table,
     ws.natom = atom_counts[s];
     ws.task_first = tp.task_first;
     ws.offd_count = tp.offd_count;
+    ws.clean_count = tp.clean_count;
     ws.diag_count = tp.diag_count;
     ws.term_first = tp.term_first;
     ws.term_count = tp.term_count;

@@ -31,7 +31,7 @@ struct TopoTemplate {
   int natom = 0;
   int batch_id = 0;       // the call that built it; -1 once that batch's arrays are gone
   int source_system = 0;  // a system of that batch with this topology
-  int task_first = 0, offd_count = 0, diag_count = 0;
+  int task_first = 0, offd_count = 0, clean_count = 0, diag_count = 0;
   int term_first = 0, term_count = 0;
   int bond_count = 0, angl_count = 0, dihe_count = 0;
   int lj_off = 0, lj_nt = 0;

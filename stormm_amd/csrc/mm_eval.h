@@ -323,17 +323,19 @@ __device__ __forceinline__ unsigned step_mask(unsigned jmask, int il) {
 }
 
 // Number of rotation steps the wave runs for its current tasks: 16 when any of its rows holds
-// an off-diagonal tile, 9 when all hold diagonal tiles.
+// an off-diagonal tile, 8 when all hold diagonal tiles.
 __device__ __forceinline__ int wave_steps(bool diag) {
   const unsigned long long offd = __ballot(!diag);
-  return (offd != 0ULL) ? 16 : 9;
+  return (offd != 0ULL) ? 16 : 8;
 }
 
-// A tile task as seen by one lane of its 16-lane row.  Bit ks of `valid` is set when the lane's
-// pair at rotation step ks exists: both atoms are present, and a diagonal tile takes each pair
-// once (steps 1..7 on every lane, step 8 on lanes 0..7).
+// A tile task as seen by one lane of its 16-lane row.  Lane il starts with j lane jl0: il on an
+// off-diagonal tile, il - 1 (mod 16) on a diagonal one, so that a diagonal tile's step ks meets
+// the atom ks + 1 lanes back and its first 8 steps hold every pair of the tile once (the 8th
+// step, which meets each pair twice, on lanes 0..7 only) and no atom paired with itself.  Bit ks
+// of `valid` is set when the lane's pair at rotation step ks exists.
 struct TileTask {
-  int ib, jb, slot, ni, nj, i, j0, nsteps;
+  int ib, jb, slot, ni, nj, i, jl0, j0, nsteps;
   bool diag, ivalid;
   unsigned valid;
   __device__ __forceinline__ TileTask(const Task &tk, int il) {
@@ -345,14 +347,19 @@ struct TileTask {
     nj = (tk.y >> 20) & 0xff;
     ivalid = il < ni;
     i = ib + il;
-    j0 = jb + il;
+    jl0 = (il - (diag ? 1 : 0)) & 15;
+    j0 = jb + jl0;
     const unsigned jm = (nj >= 16) ? 0xffffu : ((1u << nj) - 1u);
-    unsigned v = step_mask(jm, il);
+    unsigned v = step_mask(jm, jl0);
     if (diag) {
-      v &= (il < 8) ? 0x1feu : 0xfeu;
+      v &= (il < 8) ? 0xffu : 0x7fu;
     }
     valid = ivalid ? v : 0u;
     nsteps = wave_steps(diag);
+  }
+  // The j lane whose sums a lane holds after the last step (the j sums move one lane per step).
+  __device__ __forceinline__ int jl_home() const {
+    return (jl0 - (nsteps - 1)) & 15;
   }
 };
 
@@ -367,7 +374,7 @@ __device__ __forceinline__ void tile_coords(const StageCtx<T> &c, const TileTask
   const auto ox = tile_centre(L.cx[tt.ib], L.cx[tt.jb]);
   const auto oy = tile_centre(L.cy[tt.ib], L.cy[tt.jb]);
   const auto oz = tile_centre(L.cz[tt.ib], L.cz[tt.jb]);
-  const bool jvalid = (c.tid & 15) < tt.nj;
+  const bool jvalid = tt.jl0 < tt.nj;
   xi = tt.ivalid ? tile_crd(L.cx[tt.i], ox, c.crd_scale) : FarAway<T>::kCrd;
   yi = tt.ivalid ? tile_crd(L.cy[tt.i], oy, c.crd_scale) : FarAway<T>::kCrd;
   zi = tt.ivalid ? tile_crd(L.cz[tt.i], oz, c.crd_scale) : FarAway<T>::kCrd;
@@ -634,8 +641,8 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k, int val_chunks) {
     if (tt.ivalid) {
       lds_add(&L.psi[tt.i], Prec<double>::fixed(static_cast<double>(acci) * psi_scale));
     }
-    // After the last step the j sums sit nsteps - 1 lanes to the right of their atoms.
-    const int jl_home = (il - (tt.nsteps - 1)) & 15;
+    // After the last step the j sums sit nsteps - 1 lanes right of the lanes that loaded them.
+    const int jl_home = tt.jl_home();
     if (jl_home < tt.nj) {
       lds_add(&L.psi[tt.jb + jl_home], Prec<double>::fixed(static_cast<double>(accj) *
                                                            psi_scale));
@@ -779,7 +786,7 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
     const Task tk = fetch_task(c, k, t);
     const TileTask tt(tk, il);
     const unsigned emask = (tk.z >= 0 && tt.ivalid) ? k.masks[tk.z + il] : 0u;
-    const unsigned nbmask = tt.valid & ~step_mask(emask, il);
+    const unsigned nbmask = tt.valid & ~step_mask(emask, tt.jl0);
     T xi, yi, zi, xj, yj, zj;
     tile_coords(c, tt, xi, yi, zi, xj, yj, zj);
     const T qi = L.q[tt.i], p1i = L.p1[tt.i], p2i = L.p2[tt.i], Ri = L.rad[tt.i];
@@ -968,8 +975,8 @@ __device__ void stage_pairs(StageCtx<T> &c, const MMKit &k) {
           lds_add(&L.dedr[tt.i], P::fixed(Ri * dri * dedr_scale));
         }
       }
-      // After the last step the j sums sit nsteps - 1 lanes to the right of their atoms.
-      const int jl_home = (il - (tt.nsteps - 1)) & 15;
+      // After the last step the j sums sit nsteps - 1 lanes right of the lanes that loaded them.
+      const int jl_home = tt.jl_home();
       if (jl_home < tt.nj) {
         const int j = tt.jb + jl_home;
         lds_add(&L.fx[j], P::fixed(fxj * frc_scale));
@@ -1150,7 +1157,7 @@ __device__ __forceinline__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
       lds_add(&L.fy[tt.i], P::fixed(fyi * frc_scale));
       lds_add(&L.fz[tt.i], P::fixed(fzi * frc_scale));
     }
-    const int jl_home = (il - (tt.nsteps - 1)) & 15;
+    const int jl_home = tt.jl_home();
     if (jl_home < tt.nj) {
       const int j = tt.jb + jl_home;
       lds_add(&L.fx[j], P::fixed(gxj * frc_scale));

@@ -329,7 +329,9 @@ static int add_template(const stormm_topology_batch *b, const SystemTerms &t, in
   tp.bond_count = t.bond_count;
   tp.angl_count = t.angl_count;
   tp.dihe_count = t.dihe_count;
-  // Cost model: rotation steps of the three pair passes plus valence work.
+  // Cost model: rotation steps of the three pair passes plus valence work.  A diagonal tile runs
+  // 8 steps but counts 9: its wavefronts run the masked steps and take half as many pairs per
+  // task, so set-up and sums weigh more (measured: the units pack better for it).
   tp.cost = (16.0 * tp.offd_count) + (9.0 * tp.diag_count) +
             0.25 * (t.bond_count + t.angl_count + 2 * t.dihe_count) + 0.1 * n;
   P.templates.push_back(std::move(tp));

@@ -474,13 +474,15 @@ __device__ __forceinline__ unsigned near_entry(int i, int j, bool near_i, bool n
 }
 
 // Appends the entries of the lanes with `need` set, in lane order; n is wave-uniform.  Every
-// lane stores: the lanes with an entry to n, n + 1, ..., the others, also in lane order, to
-// the slots after them, which later pushes overwrite (before a group of steps the queue holds
-// at most 63 entries and a group pushes at most 4 x 64, so every store stays inside the
-// queue's stride).  The step thus has no branch, which would split it and keep the rotated
-// operands from folding into their instructions.
+// lane stores: the lanes with an entry to n, n + 1, ..., the others to the slots after them
+// (the r-th lane without an entry, counting from lane 0, to n + 63 - r, which is at least
+// n + count because at most 64 - count lanes have none), which later pushes overwrite (before a
+// group of steps the queue holds at most 63 entries and a group pushes at most 4 x 64, so every
+// store stays inside the queue's stride).  The step thus has no branch, which would split it
+// and keep the rotated operands from folding into their instructions.  rlane4 is 4 (63 - lane),
+// a byte offset.
 __device__ __forceinline__ void nearq_push(unsigned *q, int &n, bool near_i, bool near_j,
-                                           unsigned entry, int lane) {
+                                           unsigned entry, int rlane4) {
   // The mask is the union of the two directions' ballots (each a comparison's lane mask).
   const bool need = near_i || near_j;
   const unsigned long long m = __builtin_amdgcn_ballot_w64(near_i) |
@@ -488,9 +490,10 @@ __device__ __forceinline__ void nearq_push(unsigned *q, int &n, bool near_i, boo
   const int below = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(m >> 32),
                                               __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(m),
                                                                         0u));
-  const int count = __popcll(m);
-  q[n + (need ? below : (count + (lane - below)))] = entry;
-  n += count;
+  // For a lane without an entry, below + 63 - lane = 63 - r.
+  unsigned char *slot = reinterpret_cast<unsigned char *>(q + n) + (below << 2);
+  *reinterpret_cast<unsigned *>(slot + (need ? 0 : rlane4)) = entry;
+  n += __popcll(m);
   __builtin_amdgcn_wave_barrier();
 }
 
@@ -555,6 +558,7 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k, int val_chunks) {
   WuLds<T> &L = c.L;
   const int il = c.tid & 15;
   const int lane = c.tid & 63;
+  const int rlane4 = (63 - lane) << 2;
   // Wavefront-uniform, so the queue's base address stays in scalar registers.
   const int wave = __builtin_amdgcn_readfirstlane(c.tid >> 6);
   unsigned *q = L.nearq + (wave * kNearQueueStride);
@@ -615,7 +619,7 @@ __device__ void stage_psi(StageCtx<T> &c, const MMKit &k, int val_chunks) {
           nearq_push(q, qn, !far_i, !far_j,
                      ei | static_cast<unsigned>(row_ror<K>(ej)) | (far_i ? 0u : kNearI) |
                          (far_j ? 0u : kNearJ),
-                     lane);
+                     rlane4);
         };
         unrolled_steps(step, std::integer_sequence<int, 0, 1, 2, 3>{});
         settle(acci);
@@ -1064,6 +1068,7 @@ __device__ __forceinline__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
   WuLds<T> &L = c.L;
   const int il = c.tid & 15;
   const int lane = c.tid & 63;
+  const int rlane4 = (63 - lane) << 2;
   // Wavefront-uniform, so the queue's base address stays in scalar registers.
   const int wave = __builtin_amdgcn_readfirstlane(c.tid >> 6);
   unsigned *q = L.nearq + (wave * kNearQueueStride);
@@ -1126,7 +1131,7 @@ __device__ __forceinline__ void stage_deriv(StageCtx<T> &c, const MMKit &k) {
           nearq_push(q, qn, !far_i, !far_j,
                      ei | static_cast<unsigned>(row_ror<K>(ej)) | (far_i ? 0u : kNearI) |
                          (far_j ? 0u : kNearJ),
-                     lane);
+                     rlane4);
         };
         unrolled_steps(step, std::integer_sequence<int, 0, 1, 2, 3>{});
         settle(fxi);
